@@ -1,0 +1,183 @@
+/*
+ * plumed2_amd: MI355X (gfx950) implementation of PLUMED's COORDINATION
+ * collective variable and of the METAD bias grid that consumes it.
+ *
+ * C ABI.  Plain pointers and sizes, integer status codes, no exceptions.
+ * Every context owns its device buffers and one HIP stream (or uses the stream
+ * the caller sets); contexts are independent, so several PLUMED instances in
+ * different threads may each own theirs.  Errors set a thread-local message
+ * readable with p2a_last_error().
+ *
+ * Units and layouts follow the reference:
+ *   positions      AoS double [natoms][3], the requested atoms in the order
+ *                  GROUPA then GROUPB (CoordinationBase.cpp:101
+ *                  requestAtoms(nl->getFullAtomList()))
+ *   box            double[9], row i = lattice vector i (Pbc::setBox,
+ *                  Pbc.cpp:165)
+ *   derivatives    double [natoms][3], d CV / d x  (Colvar.h:73-103 layout
+ *                  3*i+k)
+ *   virial         double[9], the box derivative  -sum dd (x) d, row-major
+ *                  (CoordinationBase.cpp:200-206, setBoxDerivatives)
+ */
+#ifndef PLUMED2_AMD_H
+#define PLUMED2_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
+
+#define P2A_OK 0
+#define P2A_ERR_ARG (-1)      /* invalid argument or input syntax          */
+#define P2A_ERR_HIP (-2)      /* HIP runtime error                          */
+#define P2A_ERR_NOMEM (-3)    /* device allocation failed                   */
+#define P2A_ERR_UNSUPPORTED (-4) /* valid input this build does not handle  */
+#define P2A_ERR_GRID (-5)     /* lookup outside the METAD grid (Grid.cpp:108-118) */
+
+/* Library version string. */
+const char *p2a_version(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char *p2a_last_error(void);
+/* Number of visible HIP devices (0 when there is no GPU). */
+int p2a_device_count(void);
+
+/* ------------------------------------------------------------------------
+ * Switching function
+ * ------------------------------------------------------------------------ */
+
+/* switch types (SwitchingFunction.h:36-57) */
+enum {
+  P2A_SW_RATIONAL_FIXED = 0, /* fixedRational<N>, 2N==M, N,M even, D_0==0, N<=12 */
+  P2A_SW_RATIONAL = 1,       /* general rational (fast/simplified flags)         */
+  P2A_SW_EXP = 2,
+  P2A_SW_GAUSSIAN = 3,
+  P2A_SW_FASTGAUSSIAN = 4,   /* GAUSSIAN with R_0=1, D_0=0                       */
+  P2A_SW_SMAP = 5,
+  P2A_SW_CUBIC = 6,
+  P2A_SW_TANH = 7,
+  P2A_SW_COSINUS = 8,
+  P2A_SW_Q = 9
+};
+
+/* Fully resolved switching-function parameters (SwitchingFunction.h:58-94
+ * switchContainers::Data plus the dispatch decision of rationalFactory). */
+typedef struct p2a_switch {
+  int32_t type;
+  int32_t fast, simplified;
+  int32_t nn, mm, nnf, mmf;
+  int32_t a, b;
+  int32_t pad_;
+  double d0, dmax, dmax_2, invr0, invr0_2, stretch, shift;
+  double preRes, preDfunc, preSecDev, preDfuncF, preSecDevF;
+  double c, d;
+  double beta, lambda, ref;
+} p2a_switch;
+
+/* Parse the body of a SWITCH={...} keyword.  Replaces
+ * SwitchingFunction::set(const std::string&, std::string&)
+ * (src/tools/SwitchingFunction.cpp:1055-1159): same keywords (R_0 D_0 D_MAX
+ * NN MM A B BETA LAMBDA REF NOSTRETCH STRETCH), same defaults, same error
+ * texts ("could not parse D_MAX", "R_0 is required for RATIONAL", "found the
+ * following rogue keywords in switching function input : ...").  CUSTOM /
+ * MATHEVAL (lepton) returns P2A_ERR_UNSUPPORTED. */
+int p2a_switch_parse(const char *definition, p2a_switch *out);
+
+/* NN/MM/R_0/D_0 form.  Replaces SwitchingFunction::set(int,int,double,double)
+ * (src/tools/SwitchingFunction.cpp:1176-1184): MM=0 means 2*NN, D_MAX set
+ * where s=1e-5, always stretched. */
+int p2a_switch_rational(int nn, int mm, double r0, double d0, p2a_switch *out);
+
+/* One-line description, as SwitchingFunction::description(). */
+int p2a_switch_describe(const p2a_switch *sw, char *buf, size_t len);
+
+/* ------------------------------------------------------------------------
+ * COORDINATION
+ * ------------------------------------------------------------------------ */
+
+enum { P2A_NL_NONE = 0, P2A_NL_CLASSIC = 1, P2A_NL_CELLS = 2 };
+
+/* Parsed action keywords of one COORDINATION line (Coordination.cpp:115-157,
+ * CoordinationBase.cpp:30-131).  Atom lists hold 0-based absolute indices.
+ * Arrays are owned by the library (p2a_coord_spec_free). */
+typedef struct p2a_coord_spec {
+  int64_t *groupa;
+  int64_t na;
+  int64_t *groupb;
+  int64_t nb;
+  int32_t pbc;       /* !NOPBC                                 */
+  int32_t pair;      /* PAIR                                   */
+  int32_t serial;    /* SERIAL (accepted, no effect on the GPU) */
+  int32_t nl_mode;   /* P2A_NL_*                               */
+  double nl_cutoff;  /* NL_CUTOFF                              */
+  int32_t nl_stride; /* NL_STRIDE                              */
+  int32_t pad_;
+  p2a_switch sw;
+} p2a_coord_spec;
+
+/* Parse the keywords of a COORDINATION action line (label and action name
+ * optional): GROUPA GROUPB (serial numbers, ranges a-b, a-b:s), NN MM D_0 R_0
+ * SWITCH NOPBC PAIR SERIAL NLIST NLISTCELLS NL_CUTOFF NL_STRIDE.  Error texts
+ * follow the reference ("R_0 should be explicitly specified and positive",
+ * "NL_CUTOFF should be explicitly specified and positive", ...).  Host only. */
+int p2a_coord_parse(const char *line, p2a_coord_spec *out);
+void p2a_coord_spec_free(p2a_coord_spec *spec);
+
+typedef struct p2a_coord p2a_coord;
+
+/* Create a COORDINATION context on HIP device `device` from a parsed spec.
+ * Replaces the construction in CoordinationBase::CoordinationBase
+ * (src/colvar/CoordinationBase.cpp:42-131) + Coordination::Coordination
+ * (src/colvar/Coordination.cpp:128-157). */
+int p2a_coord_create(p2a_coord **ctx, int device, const p2a_coord_spec *spec);
+int p2a_coord_destroy(p2a_coord *ctx);
+
+/* Use the caller's HIP stream (hipStream_t passed as void*); NULL restores the
+ * context's own stream. */
+int p2a_coord_set_stream(p2a_coord *ctx, void *stream);
+
+/* One step of CoordinationBase::prepare + calculate
+ * (src/colvar/CoordinationBase.cpp:137-232, NeighborList::prepare
+ * NeighborList.cpp:433-456): rebuilds the neighbour structure on the first
+ * call and when step % NL_STRIDE == 0, then evaluates value, derivatives and
+ * virial.  Host buffers; synchronous.  Any output pointer may be NULL. */
+int p2a_coord_calculate(p2a_coord *ctx, int64_t step, const double *box9, const double *pos,
+                        double *value, double *deriv, double *virial);
+
+/* Same with device-resident positions and outputs (value: 1 double, deriv:
+ * 3*natoms doubles, virial: 9 doubles, all device pointers; any may be NULL).
+ * box9 is a host array.  Asynchronous on the context's stream. */
+int p2a_coord_calculate_device(p2a_coord *ctx, int64_t step, const double *box9, const double *pos_dev,
+                               double *value_dev, double *deriv_dev, double *virial_dev);
+
+typedef struct p2a_coord_stats {
+  int64_t natoms;
+  int64_t contributing_pairs; /* pairs with r^2 <= D_MAX^2 at the last step   */
+  int64_t clusters_i, clusters_j;
+  int64_t list_entries;       /* cluster-pair list length at the last rebuild */
+  int64_t rebuilds;
+  int32_t path;               /* 0 cluster-pair kernel, 1 all-pairs FP64, 2 PAIR */
+  int32_t pad_;
+  double pair_kernel_ms;      /* summed pair-kernel time since profiling was enabled */
+  int64_t pair_kernel_launches;
+} p2a_coord_stats;
+
+/* Statistics of the last step.  Synchronises the context's stream. */
+int p2a_coord_get_stats(p2a_coord *ctx, p2a_coord_stats *stats);
+/* Record HIP events around every pair-kernel launch (on the launch stream) and
+ * accumulate their durations into pair_kernel_ms; on!=0 also resets the sums. */
+int p2a_coord_set_profiling(p2a_coord *ctx, int on);
+
+#if defined(__GNUC__)
+#pragma GCC visibility pop
+#endif
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PLUMED2_AMD_H */

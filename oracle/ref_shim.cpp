@@ -1,0 +1,200 @@
+// C entry points over the reference's own tools classes, compiled from the
+// reference sources where they lie (see oracle/Makefile).  TEST INFRASTRUCTURE
+// ONLY: the tests use this library to pin the FP64 oracle; no product code
+// loads it.  The pair loop in ref_coordination restates
+// src/colvar/CoordinationBase.cpp:142-232 around the reference NeighborList,
+// Pbc and SwitchingFunction objects (the Action/Colvar layer needs the whole
+// PLUMED kernel and is not built here).
+#include "tools/SwitchingFunction.h"
+#include "tools/Pbc.h"
+#include "tools/Tensor.h"
+#include "tools/NeighborList.h"
+#include "tools/Communicator.h"
+#include "tools/AtomNumber.h"
+#include "tools/Grid.h"
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <memory>
+
+using namespace PLMD;
+
+#define REF_EXPORT extern "C" __attribute__((visibility("default")))
+
+static thread_local std::string g_err;
+
+REF_EXPORT const char* ref_last_error() { return g_err.c_str(); }
+
+REF_EXPORT void* ref_switch_create(const char* def) {
+  auto* sf = new SwitchingFunction();
+  std::string err;
+  try {
+    sf->set(std::string(def), err);
+  } catch (std::exception& e) {
+    err = e.what();
+  }
+  g_err = err;
+  if (!err.empty()) { delete sf; return nullptr; }
+  return sf;
+}
+
+REF_EXPORT void* ref_switch_create_nnmm(int nn, int mm, double r0, double d0) {
+  auto* sf = new SwitchingFunction();
+  sf->set(nn, mm, r0, d0);
+  return sf;
+}
+
+REF_EXPORT void ref_switch_destroy(void* h) { delete static_cast<SwitchingFunction*>(h); }
+
+REF_EXPORT void ref_switch_eval_sqr(void* h, int n, const double* d2, double* val, double* dfunc) {
+  auto* sf = static_cast<SwitchingFunction*>(h);
+  for (int i = 0; i < n; i++) val[i] = sf->calculateSqr(d2[i], dfunc[i]);
+}
+
+REF_EXPORT void ref_switch_eval(void* h, int n, const double* d, double* val, double* dfunc) {
+  auto* sf = static_cast<SwitchingFunction*>(h);
+  for (int i = 0; i < n; i++) val[i] = sf->calculate(d[i], dfunc[i]);
+}
+
+REF_EXPORT double ref_switch_dmax(void* h) { return static_cast<SwitchingFunction*>(h)->get_dmax(); }
+
+static Tensor box_from(const double* b9) {
+  return Tensor(b9[0], b9[1], b9[2], b9[3], b9[4], b9[5], b9[6], b9[7], b9[8]);
+}
+
+REF_EXPORT void ref_pbc_distance(const double* b9, int n, const double* v1, const double* v2, double* out) {
+  Pbc pbc;
+  pbc.setBox(box_from(b9));
+  for (int i = 0; i < n; i++) {
+    Vector d = pbc.distance(Vector(v1[3 * i], v1[3 * i + 1], v1[3 * i + 2]),
+                            Vector(v2[3 * i], v2[3 * i + 1], v2[3 * i + 2]));
+    out[3 * i] = d[0]; out[3 * i + 1] = d[1]; out[3 * i + 2] = d[2];
+  }
+}
+
+REF_EXPORT void ref_pbc_reduced(const double* b9, double* out9) {
+  Pbc pbc;
+  pbc.setBox(box_from(b9));
+  // distance of the three unit scaled vectors is not the reduced box; expose the
+  // reduced box through realToScaled/scaledToReal consistency instead
+  Tensor b = pbc.getBox();
+  for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) out9[3 * i + j] = b[i][j];
+}
+
+// nlmode: 0 = no neighbour list, 1 = NLIST (classic), 2 = NLISTCELLS.
+// pos holds the requested atoms (GROUPA then GROUPB); group atoms are numbered
+// by absidx.  Returns the number of pairs in the list, or -1 on error.
+REF_EXPORT int64_t ref_coordination(void* sfh, const double* b9, int pbc_on, const double* pos,
+                                    const int64_t* absidx, int nA, int nB, int dopair,
+                                    int nlmode, double nl_cut,
+                                    double* value, double* deriv, double* virial) {
+  try {
+    auto* sf = static_cast<SwitchingFunction*>(sfh);
+    Pbc pbc;
+    pbc.setBox(box_from(b9));
+    Communicator comm;
+    std::vector<AtomNumber> ga(nA), gb(nB);
+    for (int i = 0; i < nA; i++) ga[i].setIndex(absidx[i]);
+    for (int i = 0; i < nB; i++) gb[i].setIndex(absidx[nA + i]);
+    std::unique_ptr<NeighborList> nl;
+    bool serial = true;
+    if (nB > 0) {
+      if (nlmode) nl = std::make_unique<NeighborList>(ga, gb, serial, dopair, pbc_on, pbc, comm, nl_cut, 1, nlmode == 2);
+      else nl = std::make_unique<NeighborList>(ga, gb, serial, dopair, pbc_on, pbc, comm);
+    } else {
+      if (nlmode) nl = std::make_unique<NeighborList>(ga, serial, pbc_on, pbc, comm, nl_cut, 1, nlmode == 2);
+      else nl = std::make_unique<NeighborList>(ga, serial, pbc_on, pbc, comm);
+    }
+    const int nat = nA + nB;
+    std::vector<Vector> x(nat);
+    for (int i = 0; i < nat; i++) x[i] = Vector(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    if (nlmode) nl->update(x);
+    double ncoord = 0.0;
+    Tensor vir;
+    std::vector<Vector> der(nat);
+    const unsigned nn = nl->size();
+    for (unsigned k = 0; k < nn; ++k) {
+      auto pr = nl->getClosePair(k);
+      unsigned i0 = pr.first, i1 = pr.second;
+      if (absidx[i0] == absidx[i1]) continue;
+      Vector d = pbc_on ? pbc.distance(x[i0], x[i1]) : delta(x[i0], x[i1]);
+      double dfunc = 0.;
+      ncoord += sf->calculateSqr(d.modulo2(), dfunc);
+      Vector dd(dfunc * d);
+      Tensor vv(dd, d);
+      der[i0] -= dd;
+      der[i1] += dd;
+      vir -= vv;
+    }
+    *value = ncoord;
+    for (int i = 0; i < nat; i++) for (int k = 0; k < 3; k++) deriv[3 * i + k] = der[i][k];
+    for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) virial[3 * a + b] = vir[a][b];
+    return nn;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// ---- Grid (src/tools/Grid.h) with spline and derivatives -------------------
+REF_EXPORT void* ref_grid_create(int dim, const double* gmin, const double* gmax, const unsigned* nbin, const int* pbc) {
+  std::vector<std::string> names, smin, smax, pmin, pmax;
+  std::vector<unsigned> nb;
+  std::vector<bool> isp;
+  for (int i = 0; i < dim; i++) {
+    names.push_back("x" + std::to_string(i));
+    char b1[64], b2[64];
+    std::snprintf(b1, sizeof b1, "%.17g", gmin[i]);
+    std::snprintf(b2, sizeof b2, "%.17g", gmax[i]);
+    smin.push_back(b1); smax.push_back(b2);
+    pmin.push_back(b1); pmax.push_back(b2);
+    nb.push_back(nbin[i]);
+    isp.push_back(pbc[i] != 0);
+  }
+  return new Grid("bias", names, smin, smax, nb, true, true, isp, pmin, pmax);
+}
+REF_EXPORT void ref_grid_destroy(void* h) { delete static_cast<Grid*>(h); }
+REF_EXPORT uint64_t ref_grid_size(void* h) { return static_cast<Grid*>(h)->getSize(); }
+REF_EXPORT void ref_grid_set(void* h, uint64_t index, double v, const double* der) {
+  auto* g = static_cast<Grid*>(h);
+  std::vector<double> d(der, der + g->getDimension());
+  double old;
+  std::vector<double> oldd(g->getDimension());
+  old = g->getValueAndDerivatives(index, oldd);
+  for (unsigned i = 0; i < d.size(); i++) d[i] -= oldd[i];
+  g->addValueAndDerivatives(index, v - old, d);
+}
+REF_EXPORT int ref_grid_spline(void* h, const double* x, double* value, double* der) {
+  try {
+    auto* g = static_cast<Grid*>(h);
+    std::vector<double> xx(x, x + g->getDimension()), dd(g->getDimension());
+    *value = g->getValueAndDerivatives(xx, dd);
+    for (unsigned i = 0; i < dd.size(); i++) der[i] = dd[i];
+    return 0;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+// neighbours of the node containing x with nneigh bins per direction
+// (GridBase::getNeighbors); returns the count, writes at most cap indices
+REF_EXPORT int64_t ref_grid_neighbors(void* h, const double* x, const unsigned* nneigh, uint64_t* out, int64_t cap) {
+  try {
+    auto* g = static_cast<Grid*>(h);
+    std::vector<double> xx(x, x + g->getDimension());
+    std::vector<unsigned> nn(nneigh, nneigh + g->getDimension());
+    auto v = g->getNeighbors(xx, nn);
+    for (int64_t i = 0; i < (int64_t)v.size() && i < cap; i++) out[i] = v[i];
+    return (int64_t)v.size();
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+REF_EXPORT void ref_grid_point(void* h, uint64_t index, double* x) {
+  auto* g = static_cast<Grid*>(h);
+  std::vector<double> xx(g->getDimension());
+  g->getPoint(index, xx);
+  for (unsigned i = 0; i < xx.size(); i++) x[i] = xx[i];
+}

@@ -1,0 +1,11 @@
+"""plumed2_amd: MI355X-native COORDINATION collective variable and METAD grid.
+
+The arithmetic lives in the HIP library ``libplumed2amd.so`` (C ABI in
+``include/plumed2_amd.h``); this package is the host-side mirror of the
+reference's action interface.
+"""
+from ._capi import Plumed2AmdError, lib  # noqa: F401
+from .coordination import (Coordination, CoordinationSpec, describe_switch,  # noqa: F401
+                           parse_switch, rational_switch)
+
+__version__ = "0.1.0"

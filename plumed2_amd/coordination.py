@@ -1,0 +1,149 @@
+"""COORDINATION action, host side.
+
+Mirrors the keyword interface of the reference action
+(src/colvar/Coordination.cpp:113-157, src/colvar/CoordinationBase.cpp:30-232):
+an input line such as
+
+    c: COORDINATION GROUPA=1-1000 GROUPB=1001-2000 SWITCH={RATIONAL R_0=0.3 D_MAX=1.0}
+
+builds the action; ``calculate(positions, box, step)`` returns the value, the
+derivatives with respect to the requested atoms (GROUPA then GROUPB, the order
+of CoordinationBase's requestAtoms) and the virial (box derivative).  All
+arithmetic runs in the HIP library; errors raise Plumed2AmdError with the
+reference's messages.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as A
+
+
+def parse_switch(definition: str) -> A.p2a_switch:
+    """SWITCH={...} body -> resolved parameters (SwitchingFunction::set(string))."""
+    sw = A.p2a_switch()
+    A.check(A.lib().p2a_switch_parse(definition.encode(), C.byref(sw)))
+    return sw
+
+
+def rational_switch(nn: int, mm: int, r0: float, d0: float = 0.0) -> A.p2a_switch:
+    """NN/MM/R_0/D_0 form (SwitchingFunction::set(nn,mm,r0,d0))."""
+    sw = A.p2a_switch()
+    A.check(A.lib().p2a_switch_rational(int(nn), int(mm), float(r0), float(d0), C.byref(sw)))
+    return sw
+
+
+def describe_switch(sw: A.p2a_switch) -> str:
+    buf = C.create_string_buffer(256)
+    A.check(A.lib().p2a_switch_describe(C.byref(sw), buf, 256))
+    return buf.value.decode()
+
+
+class CoordinationSpec:
+    """Parsed keywords of one COORDINATION line."""
+
+    def __init__(self, line: str):
+        self._spec = A.p2a_coord_spec()
+        A.check(A.lib().p2a_coord_parse(line.encode(), C.byref(self._spec)))
+        s = self._spec
+        self.groupa = np.ctypeslib.as_array(s.groupa, shape=(s.na,)).copy() if s.na else np.zeros(0, np.int64)
+        self.groupb = np.ctypeslib.as_array(s.groupb, shape=(s.nb,)).copy() if s.nb else np.zeros(0, np.int64)
+        self.pbc = bool(s.pbc)
+        self.pair = bool(s.pair)
+        self.nl_mode = int(s.nl_mode)
+        self.nl_cutoff = float(s.nl_cutoff)
+        self.nl_stride = int(s.nl_stride)
+        self.switch = s.sw
+
+    def __del__(self):
+        try:
+            A.lib().p2a_coord_spec_free(C.byref(self._spec))
+        except Exception:
+            pass
+
+    @property
+    def natoms(self):
+        return len(self.groupa) + len(self.groupb)
+
+
+def _box9(box):
+    b = np.ascontiguousarray(np.asarray(box, dtype=np.float64).reshape(-1))
+    if b.size == 3:
+        b = np.diag(b).reshape(-1).copy()
+    if b.size != 9:
+        raise ValueError("box must have 3 (orthorhombic) or 9 entries")
+    return b
+
+
+class Coordination:
+    """GPU COORDINATION action built from a reference-syntax input line."""
+
+    def __init__(self, line: str, device: int = 0):
+        self.spec = CoordinationSpec(line)
+        self.device = device
+        self._ctx = C.c_void_p()
+        A.check(A.lib().p2a_coord_create(C.byref(self._ctx), int(device), C.byref(self.spec._spec)))
+
+    def close(self):
+        if getattr(self, "_ctx", None) is not None and self._ctx.value:
+            A.lib().p2a_coord_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def natoms(self):
+        return self.spec.natoms
+
+    def calculate(self, positions, box, step: int = 0):
+        """Host arrays in, host arrays out: (value, deriv[natoms,3], virial[3,3])."""
+        pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1, 3))
+        if pos.shape[0] != self.natoms:
+            raise ValueError(f"expected {self.natoms} positions, got {pos.shape[0]}")
+        b = _box9(box)
+        val = C.c_double()
+        der = np.empty((self.natoms, 3))
+        vir = np.empty(9)
+        A.check(A.lib().p2a_coord_calculate(self._ctx, int(step), b.ctypes.data, pos.ctypes.data, C.addressof(val),
+                                            der.ctypes.data, vir.ctypes.data))
+        return val.value, der, vir.reshape(3, 3)
+
+    def set_stream(self, stream_handle: int | None):
+        A.check(A.lib().p2a_coord_set_stream(self._ctx, C.c_void_p(stream_handle or 0)))
+
+    def calculate_device(self, pos_ptr: int, box, step: int, value_ptr: int, deriv_ptr: int, virial_ptr: int):
+        """Device pointers (ints) in and out; asynchronous on the context stream."""
+        b = _box9(box)
+        A.check(A.lib().p2a_coord_calculate_device(self._ctx, int(step), b.ctypes.data, C.c_void_p(pos_ptr),
+                                                   C.c_void_p(value_ptr), C.c_void_p(deriv_ptr),
+                                                   C.c_void_p(virial_ptr)))
+
+    def calculate_torch(self, pos, box, step: int = 0, out=None):
+        """torch.float64 CUDA tensor [natoms,3] -> (value[1], deriv[natoms,3], virial[3,3]) CUDA tensors,
+        enqueued on torch's current stream."""
+        import torch
+
+        if not (pos.is_cuda and pos.dtype == torch.float64 and pos.is_contiguous()):
+            raise ValueError("positions must be a contiguous float64 CUDA tensor")
+        if out is None:
+            out = (torch.empty(1, dtype=torch.float64, device=pos.device),
+                   torch.empty((self.natoms, 3), dtype=torch.float64, device=pos.device),
+                   torch.empty((3, 3), dtype=torch.float64, device=pos.device))
+        val, der, vir = out
+        self.set_stream(torch.cuda.current_stream(pos.device).cuda_stream)
+        self.calculate_device(pos.data_ptr(), box, step, val.data_ptr(), der.data_ptr(), vir.data_ptr())
+        return val, der, vir
+
+    def stats(self):
+        st = A.p2a_coord_stats()
+        A.check(A.lib().p2a_coord_get_stats(self._ctx, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_ if k != "pad_"}
+
+    def set_profiling(self, on: bool):
+        A.check(A.lib().p2a_coord_set_profiling(self._ctx, int(bool(on))))
