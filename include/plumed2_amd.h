@@ -165,12 +165,14 @@ typedef struct p2a_coord_stats {
   int32_t pad_;
   double pair_kernel_ms;      /* summed pair-kernel time since profiling was enabled */
   int64_t pair_kernel_launches;
+  int64_t total_contributing_pairs; /* summed over the steps since profiling was enabled */
 } p2a_coord_stats;
 
 /* Statistics of the last step.  Synchronises the context's stream. */
 int p2a_coord_get_stats(p2a_coord *ctx, p2a_coord_stats *stats);
 /* Record HIP events around every pair-kernel launch (on the launch stream) and
- * accumulate their durations into pair_kernel_ms; on!=0 also resets the sums. */
+ * accumulate their durations into pair_kernel_ms; on!=0 also resets the sums
+ * (kernel time and total_contributing_pairs). */
 int p2a_coord_set_profiling(p2a_coord *ctx, int on);
 
 #if defined(__GNUC__)

@@ -357,6 +357,13 @@ def ref():
         R.ref_grid_neighbors.argtypes = [C.c_void_p, _dp, _u32p, _u64p, C.c_int64]
         R.ref_grid_neighbors.restype = C.c_int64
         R.ref_grid_point.argtypes = [C.c_void_p, C.c_uint64, _dp]
+        R.ref_coordctx_create.argtypes = [C.c_void_p, _dp, _i64p, C.c_int, C.c_int, C.c_int, C.c_double]
+        R.ref_coordctx_create.restype = C.c_void_p
+        R.ref_coordctx_destroy.argtypes = [C.c_void_p]
+        R.ref_coordctx_update.argtypes = [C.c_void_p, _dp]
+        R.ref_coordctx_update.restype = C.c_int64
+        R.ref_coordctx_calc.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
+        R.ref_coordctx_calc.restype = C.c_int64
         _REF = R
     return _REF
 
@@ -419,6 +426,39 @@ def ref_coordination(sw: RefSwitch, box, pos, nA, nB=0, *, absidx=None, use_pbc=
     if n < 0:
         raise RuntimeError(ref().ref_last_error().decode())
     return val.value, deriv.reshape(nat, 3), vir.reshape(3, 3), int(n)
+
+
+class RefCoordination:
+    """Reference NeighborList (cells/classic) + the CoordinationBase pair loop
+    with OpenMP, for timing the CPU baseline."""
+
+    def __init__(self, sw: RefSwitch, box, nA, nB=0, *, nlmode=2, nl_cut=1.5, absidx=None):
+        nat = nA + nB
+        absidx = np.arange(nat, dtype=np.int64) if absidx is None else np.ascontiguousarray(absidx, dtype=np.int64)
+        self.sw = sw
+        self.nat = nat
+        self.h = ref().ref_coordctx_create(sw.h, np.ascontiguousarray(np.asarray(box, dtype=np.float64).reshape(9)),
+                                           absidx, int(nA), int(nB), int(nlmode), float(nl_cut))
+        if not self.h:
+            raise RuntimeError(ref().ref_last_error().decode())
+
+    def __del__(self):
+        try:
+            ref().ref_coordctx_destroy(self.h)
+        except Exception:
+            pass
+
+    def update(self, pos):
+        return ref().ref_coordctx_update(self.h, np.ascontiguousarray(pos, dtype=np.float64).reshape(-1))
+
+    def calc(self, pos, nthreads=1, want_deriv=False):
+        val = C.c_double()
+        der = np.zeros(3 * self.nat) if want_deriv else None
+        vir = np.zeros(9) if want_deriv else None
+        n = ref().ref_coordctx_calc(self.h, np.ascontiguousarray(pos, dtype=np.float64).reshape(-1), int(nthreads),
+                                    C.byref(val), None if der is None else der.ctypes.data,
+                                    None if vir is None else vir.ctypes.data)
+        return val.value, der, vir, int(n)
 
 
 class RefGrid:

@@ -17,6 +17,9 @@
 #include <string>
 #include <vector>
 #include <memory>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 using namespace PLMD;
 
@@ -71,15 +74,6 @@ REF_EXPORT void ref_pbc_distance(const double* b9, int n, const double* v1, cons
                             Vector(v2[3 * i], v2[3 * i + 1], v2[3 * i + 2]));
     out[3 * i] = d[0]; out[3 * i + 1] = d[1]; out[3 * i + 2] = d[2];
   }
-}
-
-REF_EXPORT void ref_pbc_reduced(const double* b9, double* out9) {
-  Pbc pbc;
-  pbc.setBox(box_from(b9));
-  // distance of the three unit scaled vectors is not the reduced box; expose the
-  // reduced box through realToScaled/scaledToReal consistency instead
-  Tensor b = pbc.getBox();
-  for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) out9[3 * i + j] = b[i][j];
 }
 
 // nlmode: 0 = no neighbour list, 1 = NLIST (classic), 2 = NLISTCELLS.
@@ -197,4 +191,98 @@ REF_EXPORT void ref_grid_point(void* h, uint64_t index, double* x) {
   std::vector<double> xx(g->getDimension());
   g->getPoint(index, xx);
   for (unsigned i = 0; i < xx.size(); i++) x[i] = xx[i];
+}
+
+// ---- timing harness for the CPU baseline ------------------------------------
+// A persistent reference NeighborList (cells or classic) so that the rebuild
+// (NeighborList::update) and the per-step pair loop can be timed separately;
+// the loop follows CoordinationBase::calculate (:162-217) with per-thread
+// private derivatives merged in a critical section.
+struct RefCoordCtx {
+  SwitchingFunction* sf;
+  Pbc pbc;
+  Communicator comm;
+  std::vector<AtomNumber> ga, gb;
+  std::unique_ptr<NeighborList> nl;
+  std::vector<int64_t> absidx;
+  int nat;
+};
+
+REF_EXPORT void* ref_coordctx_create(void* sfh, const double* b9, const int64_t* absidx, int nA, int nB,
+                                     int nlmode, double nl_cut) {
+  try {
+    auto* c = new RefCoordCtx();
+    c->sf = static_cast<SwitchingFunction*>(sfh);
+    c->pbc.setBox(box_from(b9));
+    c->ga.resize(nA);
+    c->gb.resize(nB);
+    for (int i = 0; i < nA; i++) c->ga[i].setIndex(absidx[i]);
+    for (int i = 0; i < nB; i++) c->gb[i].setIndex(absidx[nA + i]);
+    c->absidx.assign(absidx, absidx + nA + nB);
+    c->nat = nA + nB;
+    if (nB > 0) c->nl = std::make_unique<NeighborList>(c->ga, c->gb, false, false, true, c->pbc, c->comm, nl_cut, 10, nlmode == 2);
+    else c->nl = std::make_unique<NeighborList>(c->ga, false, true, c->pbc, c->comm, nl_cut, 10, nlmode == 2);
+    return c;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+REF_EXPORT void ref_coordctx_destroy(void* h) { delete static_cast<RefCoordCtx*>(h); }
+
+REF_EXPORT int64_t ref_coordctx_update(void* h, const double* pos) {
+  auto* c = static_cast<RefCoordCtx*>(h);
+  std::vector<Vector> x(c->nat);
+  for (int i = 0; i < c->nat; i++) x[i] = Vector(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+  c->nl->update(x);
+  return c->nl->size();
+}
+
+REF_EXPORT int64_t ref_coordctx_calc(void* h, const double* pos, int nthreads, double* value, double* deriv,
+                                     double* virial) {
+  auto* c = static_cast<RefCoordCtx*>(h);
+  const int nat = c->nat;
+  std::vector<Vector> x(nat);
+  for (int i = 0; i < nat; i++) x[i] = Vector(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+  double ncoord = 0.;
+  Tensor vir;
+  std::vector<Vector> der(nat);
+  const unsigned nn = c->nl->size();
+  int64_t ncontrib = 0;
+  const double dmax2 = c->sf->get_dmax2();
+#pragma omp parallel num_threads(nthreads)
+  {
+    std::vector<Vector> od(nat);
+    Tensor ov;
+    int64_t oc = 0;
+#pragma omp for reduction(+ : ncoord) nowait
+    for (unsigned k = 0; k < nn; ++k) {
+      auto pr = c->nl->getClosePair(k);
+      unsigned i0 = pr.first, i1 = pr.second;
+      if (c->absidx[i0] == c->absidx[i1]) continue;
+      Vector d = c->pbc.distance(x[i0], x[i1]);
+      double dfunc = 0.;
+      double m2 = d.modulo2();
+      ncoord += c->sf->calculateSqr(m2, dfunc);
+      if (m2 <= dmax2) oc++;
+      Vector dd(dfunc * d);
+      Tensor vv(dd, d);
+      od[i0] -= dd;
+      od[i1] += dd;
+      ov -= vv;
+    }
+#pragma omp critical
+    {
+      for (int i = 0; i < nat; i++) der[i] += od[i];
+      vir += ov;
+      ncontrib += oc;
+    }
+  }
+  *value = ncoord;
+  if (deriv)
+    for (int i = 0; i < nat; i++) for (int k = 0; k < 3; k++) deriv[3 * i + k] = der[i][k];
+  if (virial)
+    for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) virial[3 * a + b] = vir[a][b];
+  return ncontrib;
 }

@@ -45,7 +45,7 @@ class p2a_coord_stats(C.Structure):
     _fields_ = [("natoms", C.c_int64), ("contributing_pairs", C.c_int64), ("clusters_i", C.c_int64),
                 ("clusters_j", C.c_int64), ("list_entries", C.c_int64), ("rebuilds", C.c_int64),
                 ("path", C.c_int32), ("pad_", C.c_int32), ("pair_kernel_ms", C.c_double),
-                ("pair_kernel_launches", C.c_int64)]
+                ("pair_kernel_launches", C.c_int64), ("total_contributing_pairs", C.c_int64)]
 
 
 _lib = None
@@ -87,6 +87,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIBPATH):
             raise ImportError(f"{LIBPATH} is not built: run __graft_entry__.build() or python plumed2_amd/build_lib.py")
+        # torch ships its own HIP runtime under the soname libamdhip64.so.7 but
+        # links it by file name; loaded after ours it would bring a second
+        # runtime into the process.  Loading torch first lets this library bind
+        # to the runtime already mapped, so device pointers are shared.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIBPATH)
         for name, (args, res) in {**_SIGS, **OPTIONAL_SIGS}.items():
             fn = getattr(L, name)

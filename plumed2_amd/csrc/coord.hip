@@ -272,9 +272,16 @@ __global__ void k_origin(int ncl, const int* __restrict__ slot_loc, const int4* 
                                (float)(0.5 * (hi[2] - lo[2])), 0.f);
 }
 
+// Entry code bits: j cluster (bits 0-29), "count" (bit 30: this entry adds
+// to the value and the virial; each unordered pair of atoms is counted by
+// exactly one entry), "diagonal" (bit 31: i cluster paired with itself
+// without image, pairs with j == i are skipped and counted when j > i).
+constexpr int kCjMask = 0x3FFFFFFF;
+constexpr int kCountBit = 0x40000000;
+
 struct ListArgs {
   int nclI, ncxI, ncyI, ncxJ, ncyJ;
-  int same;        // single group: half list
+  int mode;        // 0 one group, 1 two groups (counting side), 2 two groups (other side)
   float rlist2;
   float rb[2];     // rlist * |reciprocal row| for a, b
   float A[9];      // lattice vectors (FP32) for the search
@@ -327,11 +334,6 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
           float gy = fmaxf(fabsf(jc.y + Ty - ic.y) - ih.y - jh.y, 0.f);
           float gz = fmaxf(fabsf(jc.z + Tz - ic.z) - ih.z - jh.z, 0.f);
           pass = gx * gx + gy * gy + gz * gz < L.rlist2;
-          if (L.same) {
-            bool zero = (mx == 0 && my == 0 && mz == 0);
-            bool pos = (mx > 0) || (mx == 0 && (my > 0 || (my == 0 && mz > 0)));
-            pass = pass && (cj > ci || (cj == ci && (zero || pos)));
-          }
         }
         unsigned long long bal = __ballot(pass);
         if (FILL && pass) {
@@ -340,10 +342,12 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
           double Tx = mx * L.Ad[0] + my * L.Ad[3] + mz * L.Ad[6];
           double Ty = mx * L.Ad[1] + my * L.Ad[4] + mz * L.Ad[7];
           double Tz = mx * L.Ad[2] + my * L.Ad[5] + mz * L.Ad[8];
-          int img = (mx + 1) + 3 * (my + 1) + 9 * (mz + 1);
-          bool diag = L.same && cj == ci && mx == 0 && my == 0 && mz == 0;
+          bool zero = (mx == 0 && my == 0 && mz == 0);
+          bool diag = L.mode == 0 && cj == ci && zero;
+          bool positive = (mx > 0) || (mx == 0 && (my > 0 || (my == 0 && mz > 0)));
+          bool counted = L.mode == 1 || (L.mode == 0 && (cj > ci || (cj == ci && positive)));
           Entry en;
-          en.code = cj | (img << 26) | (diag ? (int)0x80000000u : 0);
+          en.code = cj | (counted ? kCountBit : 0) | (diag ? (int)0x80000000u : 0);
           en.dx = (float)(oj.x + Tx - oi.x);
           en.dy = (float)(oj.y + Ty - oi.y);
           en.dz = (float)(oj.z + Tz - oi.z);
@@ -407,205 +411,6 @@ __global__ void k_bbox(int ncl, const float4* __restrict__ p, float4* __restrict
   bb[2 * c + 1] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]), 0.5f * (hi[2] - lo[2]), 0.f);
 }
 
-// s and dfunc of fixedRational<2P> on r^2 in FP32
-template <int P>
-__device__ __forceinline__ void sw_fixed(const SwF& f, float r2, float& s, float& df) {
-  float x = r2 * f.invr0_2;
-  float xp = 1.0f;
-#pragma unroll
-  for (int k = 1; k < P; k++) xp *= x;  // x^(P-1)
-  float res = __builtin_amdgcn_rcpf(fmaf(xp, x, 1.0f));
-  s = fmaf(res, f.stretch, f.shift);
-  df = f.cder * xp * res * res;
-}
-
-struct PairArgs {
-  int nitems;
-  int twogroup;
-  int offI, offJ;  // group offsets in the requested-atom list
-  float dmax2, prune2;
-  SwF f;
-  p2a_switch sw;   // FP64 parameters (generic switch and the cut band)
-  BoxArg box;
-};
-
-// FP64 re-evaluation of the squared distance of one pair, used only when the
-// FP32 value lies within a narrow band around D_MAX^2 so that the inclusive
-// cut of the reference is decided in double precision.
-__device__ double exact_r2(const PairArgs& A, const double* __restrict__ pos, const int4* __restrict__ infoI,
-                           const int4* __restrict__ infoJ, int reqi, int reqj, int img) {
-  const double* xi = pos + 3 * (size_t)reqi;
-  const double* xj = pos + 3 * (size_t)reqj;
-  int4 mi = infoI[reqi - A.offI], mj = infoJ[reqj - A.offJ];
-  int ix = img % 3 - 1, iy = (img / 3) % 3 - 1, iz = img / 9 - 1;
-  double r2 = 0.0;
-  for (int d = 0; d < 3; d++) {
-    double wi = xi[d] - (mi.x * A.box.a[d] + mi.y * A.box.a[3 + d] + mi.z * A.box.a[6 + d]);
-    double wj = xj[d] - (mj.x * A.box.a[d] + mj.y * A.box.a[3 + d] + mj.z * A.box.a[6 + d]) +
-                (ix * A.box.a[d] + iy * A.box.a[3 + d] + iz * A.box.a[6 + d]);
-    double dd = wj - wi;
-    r2 += dd * dd;
-  }
-  return r2;
-}
-
-// One wave per work item (i cluster, chunk of <=64 list entries).
-// Lane l evaluates the pair (i = l&7 of cluster ci, j = l>>3 of cluster cj).
-template <int P, bool GENERIC>
-__global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __restrict__ items,
-                                                       const int* __restrict__ estart,
-                                                       const Entry* __restrict__ ents,
-                                                       const float4* __restrict__ posI,
-                                                       const float4* __restrict__ posJ,
-                                                       const float4* __restrict__ bbI, const float4* __restrict__ bbJ,
-                                                       const int* __restrict__ absI, const int* __restrict__ absJ,
-                                                       const double* __restrict__ pos64,
-                                                       const int4* __restrict__ infoI, const int4* __restrict__ infoJ,
-                                                       double* __restrict__ deriv, double* __restrict__ partial) {
-  const int lane = threadIdx.x & 63;
-  const int item = blockIdx.x * (PAIR_BLOCK / WAVE) + (threadIdx.x >> 6);
-  if (item >= A.nitems) return;
-  const int2 it = items[item];
-  const int ci = it.x;
-  const int e0 = estart[ci] + it.y * 64;
-  const int e1 = min(estart[ci + 1], e0 + 64);
-  const int ii = lane & 7, jj = lane >> 3;
-  const float4 pi = posI[ci * CL + ii];
-  const int reqi = __float_as_int(pi.w);
-  const int ai = A.twogroup ? absI[ci * CL + ii] : 0;
-
-  // lane-parallel pruning of the chunk against the current bounding boxes
-  Entry my = {0, 0.f, 0.f, 0.f};
-  bool pass = false;
-  {
-    int e = e0 + lane;
-    if (e < e1) {
-      my = ents[e];
-      int cj = my.code & 0x03FFFFFF;
-      float4 ic = bbI[2 * ci], ih = bbI[2 * ci + 1];
-      float4 jc = bbJ[2 * cj], jh = bbJ[2 * cj + 1];
-      float gx = fmaxf(fabsf(jc.x + my.dx - ic.x) - ih.x - jh.x, 0.f);
-      float gy = fmaxf(fabsf(jc.y + my.dy - ic.y) - ih.y - jh.y, 0.f);
-      float gz = fmaxf(fabsf(jc.z + my.dz - ic.z) - ih.z - jh.z, 0.f);
-      pass = gx * gx + gy * gy + gz * gz <= A.prune2;
-    }
-  }
-  unsigned long long mask = __ballot(pass);
-
-  float cv = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
-  float vxx = 0.f, vxy = 0.f, vxz = 0.f, vyy = 0.f, vyz = 0.f, vzz = 0.f;
-  int npair = 0;
-  const float band = 4.0e-6f * A.dmax2;
-
-  while (mask) {
-    const int k = __ffsll((long long)mask) - 1;
-    mask &= mask - 1;
-    const int code = __builtin_amdgcn_readlane(my.code, k);
-    const float Dx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my.dx), k));
-    const float Dy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my.dy), k));
-    const float Dz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my.dz), k));
-    const int cj = code & 0x03FFFFFF;
-    const float4 pj = posJ[cj * CL + jj];
-    const float dx = (pj.x + Dx) - pi.x;
-    const float dy = (pj.y + Dy) - pi.y;
-    const float dz = (pj.z + Dz) - pi.z;
-    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    bool cut = r2 <= A.dmax2;
-    if (__builtin_expect(__ballot(fabsf(r2 - A.dmax2) < band) != 0ull, 0)) {
-      const int reqj = __float_as_int(pj.w);
-      if (fabsf(r2 - A.dmax2) < band && reqi >= 0 && reqj >= 0) {
-        double r2d = exact_r2(A, pos64, infoI, infoJ, reqi, reqj, (code >> 26) & 31);
-        cut = switch_inside(A.sw, r2d);
-      }
-    }
-    if (code < 0) cut = cut && (jj > ii);  // diagonal cluster pair: j > i only
-    if (A.twogroup) cut = cut && (absJ[cj * CL + jj] != ai);
-    float s, df;
-    if (GENERIC) {
-      double dfd;
-      double sd = switch_calc_sqr(A.sw, (double)r2, dfd);
-      s = (float)sd;
-      df = (float)dfd;
-    } else {
-      sw_fixed<P>(A.f, r2, s, df);
-    }
-    s = cut ? s : 0.f;
-    df = cut ? df : 0.f;
-    npair += __popcll(__ballot(cut));
-    const float gx = df * dx, gy = df * dy, gz = df * dz;
-    cv += s;
-    fx += gx;
-    fy += gy;
-    fz += gz;
-    vxx = fmaf(gx, dx, vxx);
-    vxy = fmaf(gx, dy, vxy);
-    vxz = fmaf(gx, dz, vxz);
-    vyy = fmaf(gy, dy, vyy);
-    vyz = fmaf(gy, dz, vyz);
-    vzz = fmaf(gz, dz, vzz);
-    // derivative of atom j: +sum_i g over the 8 lanes sharing j
-    const float sx = sum8(gx), sy = sum8(gy), sz = sum8(gz);
-    if (ii == 0) {
-      const int reqj = __float_as_int(pj.w);
-      if (reqj >= 0) {
-        unsafeAtomicAdd(&deriv[3 * (size_t)reqj + 0], (double)sx);
-        unsafeAtomicAdd(&deriv[3 * (size_t)reqj + 1], (double)sy);
-        unsafeAtomicAdd(&deriv[3 * (size_t)reqj + 2], (double)sz);
-      }
-    }
-  }
-  // derivative of atom i: -sum over the 8 lanes sharing i (lanes ii + 8k)
-  double dfx = fx, dfy = fy, dfz = fz;
-  for (int o = 8; o < 64; o <<= 1) {
-    dfx += __shfl_xor(dfx, o, 64);
-    dfy += __shfl_xor(dfy, o, 64);
-    dfz += __shfl_xor(dfz, o, 64);
-  }
-  if (jj == 0 && reqi >= 0) {
-    unsafeAtomicAdd(&deriv[3 * (size_t)reqi + 0], -dfx);
-    unsafeAtomicAdd(&deriv[3 * (size_t)reqi + 1], -dfy);
-    unsafeAtomicAdd(&deriv[3 * (size_t)reqi + 2], -dfz);
-  }
-  double r[7] = {cv, vxx, vxy, vxz, vyy, vyz, vzz};
-  for (int q = 0; q < 7; q++) r[q] = wave_sum(r[q]);
-  if (lane == 0) {
-    double* out = partial + (size_t)item * 8;
-    for (int q = 0; q < 7; q++) out[q] = r[q];
-    out[7] = (double)npair;
-  }
-}
-
-// deterministic sum of the per-item partials -> value, virial, pair count
-__global__ void k_finalize(int nitems, const double* __restrict__ partial, double* __restrict__ value,
-                           double* __restrict__ virial, double* __restrict__ stats_out) {
-  __shared__ double s[8][256];
-  const int t = threadIdx.x;
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int i = t; i < nitems; i += 256)
-    for (int q = 0; q < 8; q++) acc[q] += partial[(size_t)i * 8 + q];
-  for (int q = 0; q < 8; q++) s[q][t] = acc[q];
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (t < o)
-      for (int q = 0; q < 8; q++) s[q][t] += s[q][t + o];
-    __syncthreads();
-  }
-  if (t == 0) {
-    if (value) *value = s[0][0];
-    if (virial) {
-      // virial = -sum dd (x) d (CoordinationBase.cpp:200-206), symmetric
-      double xx = -s[1][0], xy = -s[2][0], xz = -s[3][0], yy = -s[4][0], yz = -s[5][0], zz = -s[6][0];
-      virial[0] = xx; virial[1] = xy; virial[2] = xz;
-      virial[3] = xy; virial[4] = yy; virial[5] = yz;
-      virial[6] = xz; virial[7] = yz; virial[8] = zz;
-    }
-    stats_out[0] = s[7][0];
-  }
-}
-
-// ------------------------------------------------------------------------
-// FP64 fallback kernels
-// ------------------------------------------------------------------------
 struct PbcArg {
   int type;
   double box[9], inv[9], red[9], ired[9];
@@ -652,6 +457,226 @@ __device__ void pbc_distance(const PbcArg& P, const double* a, const double* b, 
   }
 }
 
+// s and dfunc of fixedRational<2P> on r^2 in FP32
+template <int P>
+__device__ __forceinline__ void sw_fixed(const SwF& f, float r2, float& s, float& df) {
+  float x = r2 * f.invr0_2;
+  float xp = 1.0f;
+#pragma unroll
+  for (int k = 1; k < P; k++) xp *= x;  // x^(P-1)
+  float res = __builtin_amdgcn_rcpf(fmaf(xp, x, 1.0f));
+  s = fmaf(res, f.stretch, f.shift);
+  df = f.cder * xp * res * res;
+}
+
+struct PairArgs {
+  int nitems;
+  int twogroup;
+  int offI, offJ;  // group offsets in the requested-atom list
+  float dmax2, prune2;
+  SwF f;
+  p2a_switch sw;   // FP64 parameters (generic switch and the cut band)
+  const PbcArg* pbc;  // device copy of the prepared cell (FP64 cut decisions)
+};
+
+// FP64 re-evaluation of the squared distance of one pair, used only when the
+// FP32 value lies within a narrow band around D_MAX^2: the inclusive cut of
+// the reference is then decided on the distance Pbc::distance itself would
+// produce from the raw positions (including the rounding of Tools::pbc).
+// The pair is oriented as the reference lists it (GROUPA atom first; lower
+// index first inside one group) because the rounding of Tools::pbc is not
+// symmetric under d -> -d.
+__device__ double exact_r2(const PbcArg* __restrict__ P, const double* __restrict__ pos, int reqi, int reqj) {
+  double d[3];
+  const int a = min(reqi, reqj), b = max(reqi, reqj);
+  pbc_distance(*P, pos + 3 * (size_t)a, pos + 3 * (size_t)b, d);
+  return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+}
+
+// One wave per work item (i cluster, chunk of <=64 list entries).
+// Lane l evaluates the pair (i = l&7 of cluster ci, j = l>>3 of cluster cj).
+template <int P, bool GENERIC>
+__global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __restrict__ items,
+                                                       const int* __restrict__ estart,
+                                                       const Entry* __restrict__ ents,
+                                                       const float4* __restrict__ posI,
+                                                       const float4* __restrict__ posJ,
+                                                       const float4* __restrict__ bbI, const float4* __restrict__ bbJ,
+                                                       const int* __restrict__ absI, const int* __restrict__ absJ,
+                                                       const double* __restrict__ pos64,
+                                                       double* __restrict__ fpart, double* __restrict__ partial) {
+  const int lane = threadIdx.x & 63;
+  const int item = blockIdx.x * (PAIR_BLOCK / WAVE) + (threadIdx.x >> 6);
+  if (item >= A.nitems) return;
+  const int2 it = items[item];
+  const int ci = it.x;
+  const int e0 = estart[ci] + it.y * 64;
+  const int e1 = min(estart[ci + 1], e0 + 64);
+  const int ii = lane & 7, jj = lane >> 3;
+  const float4 pi = posI[ci * CL + ii];
+  const int reqi = __float_as_int(pi.w);
+  const int ai = A.twogroup ? absI[ci * CL + ii] : 0;
+
+  // lane-parallel pruning of the chunk against the current bounding boxes
+  Entry my = {0, 0.f, 0.f, 0.f};
+  bool pass = false;
+  {
+    int e = e0 + lane;
+    if (e < e1) {
+      my = ents[e];
+      int cj = my.code & kCjMask;
+      float4 ic = bbI[2 * ci], ih = bbI[2 * ci + 1];
+      float4 jc = bbJ[2 * cj], jh = bbJ[2 * cj + 1];
+      float gx = fmaxf(fabsf(jc.x + my.dx - ic.x) - ih.x - jh.x, 0.f);
+      float gy = fmaxf(fabsf(jc.y + my.dy - ic.y) - ih.y - jh.y, 0.f);
+      float gz = fmaxf(fabsf(jc.z + my.dz - ic.z) - ih.z - jh.z, 0.f);
+      pass = gx * gx + gy * gy + gz * gz <= A.prune2;
+    }
+  }
+  unsigned long long mask = __ballot(pass);
+
+  float cv = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
+  float vxx = 0.f, vxy = 0.f, vxz = 0.f, vyy = 0.f, vyz = 0.f, vzz = 0.f;
+  int npair = 0;
+  const float band = 4.0e-6f * A.dmax2;
+
+  while (mask) {
+    const int k = __ffsll((long long)mask) - 1;
+    mask &= mask - 1;
+    const int code = __builtin_amdgcn_readlane(my.code, k);
+    const float Dx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my.dx), k));
+    const float Dy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my.dy), k));
+    const float Dz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my.dz), k));
+    const int cj = code & kCjMask;
+    const float4 pj = posJ[cj * CL + jj];
+    const float dx = (pj.x + Dx) - pi.x;
+    const float dy = (pj.y + Dy) - pi.y;
+    const float dz = (pj.z + Dz) - pi.z;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    bool cut = r2 <= A.dmax2;
+    if (__builtin_expect(__ballot(fabsf(r2 - A.dmax2) < band) != 0ull, 0)) {
+      const int reqj = __float_as_int(pj.w);
+      if (fabsf(r2 - A.dmax2) < band && reqi >= 0 && reqj >= 0) {
+        double r2d = exact_r2(A.pbc, pos64, reqi, reqj);
+        cut = switch_inside(A.sw, r2d);
+      }
+    }
+    const bool diag = code < 0;
+    if (diag) cut = cut && (jj != ii);  // an atom is not paired with itself
+    if (A.twogroup) cut = cut && (absJ[cj * CL + jj] != ai);
+    float s, df;
+    if (GENERIC) {
+      double dfd;
+      double sd = switch_calc_sqr(A.sw, (double)r2, dfd);
+      s = (float)sd;
+      df = (float)dfd;
+    } else {
+      sw_fixed<P>(A.f, r2, s, df);
+    }
+    df = cut ? df : 0.f;
+    const float gx = df * dx, gy = df * dy, gz = df * dz;
+    // derivative of atom i over every partner (the full list holds both
+    // orientations of each cluster pair)
+    fx += gx;
+    fy += gy;
+    fz += gz;
+    if (code & kCountBit) {
+      // value and virial once per unordered pair
+      s = cut ? s : 0.f;
+      npair += __popcll(__ballot(cut));
+      cv += s;
+      vxx = fmaf(gx, dx, vxx);
+      vxy = fmaf(gx, dy, vxy);
+      vxz = fmaf(gx, dz, vxz);
+      vyy = fmaf(gy, dy, vyy);
+      vyz = fmaf(gy, dz, vyz);
+      vzz = fmaf(gz, dz, vzz);
+    } else if (diag) {
+      const bool c2 = cut && (jj > ii);
+      const float w = c2 ? 1.f : 0.f;
+      npair += __popcll(__ballot(c2));
+      cv = fmaf(w, s, cv);
+      const float hx = w * gx, hy = w * gy, hz = w * gz;
+      vxx = fmaf(hx, dx, vxx);
+      vxy = fmaf(hx, dy, vxy);
+      vxz = fmaf(hx, dz, vxz);
+      vyy = fmaf(hy, dy, vyy);
+      vyz = fmaf(hy, dz, vyz);
+      vzz = fmaf(hz, dz, vzz);
+    }
+  }
+  // derivative of atom i: -sum over the 8 lanes sharing i (lanes ii + 8k);
+  // chunks of one cluster are combined by k_gather in a fixed order
+  double dfx = fx, dfy = fy, dfz = fz;
+  for (int o = 8; o < 64; o <<= 1) {
+    dfx += __shfl_xor(dfx, o, 64);
+    dfy += __shfl_xor(dfy, o, 64);
+    dfz += __shfl_xor(dfz, o, 64);
+  }
+  if (jj == 0) {
+    double* fp = fpart + (size_t)item * (CL * 3) + ii * 3;
+    fp[0] = -dfx;
+    fp[1] = -dfy;
+    fp[2] = -dfz;
+  }
+  double r[7] = {cv, vxx, vxy, vxz, vyy, vyz, vzz};
+  for (int q = 0; q < 7; q++) r[q] = wave_sum(r[q]);
+  if (lane == 0) {
+    double* out = partial + (size_t)item * 8;
+    for (int q = 0; q < 7; q++) out[q] = r[q];
+    out[7] = (double)npair;
+  }
+}
+
+// derivatives of the atoms of every i cluster: sum of its chunks' partials
+__global__ void k_gather(int ncl, const float4* __restrict__ p, const int* __restrict__ item_start,
+                         const double* __restrict__ fpart, double* __restrict__ deriv) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncl * CL * 3) return;
+  const int c = t / (CL * 3), r = t % (CL * 3);
+  const int req = __float_as_int(p[c * CL + r / 3].w);
+  if (req < 0) return;
+  double sum = 0.0;
+  for (int it = item_start[c]; it < item_start[c + 1]; it++) sum += fpart[(size_t)it * (CL * 3) + r];
+  deriv[3 * (size_t)req + r % 3] = sum;
+}
+
+// deterministic sum of the per-item partials -> value, virial, pair count
+__global__ void k_finalize(int nitems, const double* __restrict__ partial, double* __restrict__ value,
+                           double* __restrict__ virial, double* __restrict__ stats_out) {
+  __shared__ double s[8][512];
+  const int t = threadIdx.x;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const double4* p4 = reinterpret_cast<const double4*>(partial);
+  for (int i = t; i < nitems; i += 512) {
+    double4 a = p4[2 * (size_t)i], b = p4[2 * (size_t)i + 1];
+    acc[0] += a.x; acc[1] += a.y; acc[2] += a.z; acc[3] += a.w;
+    acc[4] += b.x; acc[5] += b.y; acc[6] += b.z; acc[7] += b.w;
+  }
+  for (int q = 0; q < 8; q++) s[q][t] = acc[q];
+  __syncthreads();
+  for (int o = 256; o > 0; o >>= 1) {
+    if (t < o)
+      for (int q = 0; q < 8; q++) s[q][t] += s[q][t + o];
+    __syncthreads();
+  }
+  if (t == 0) {
+    if (value) *value = s[0][0];
+    if (virial) {
+      // virial = -sum dd (x) d (CoordinationBase.cpp:200-206), symmetric
+      double xx = -s[1][0], xy = -s[2][0], xz = -s[3][0], yy = -s[4][0], yz = -s[5][0], zz = -s[6][0];
+      virial[0] = xx; virial[1] = xy; virial[2] = xz;
+      virial[3] = xy; virial[4] = yy; virial[5] = yz;
+      virial[6] = xz; virial[7] = yz; virial[8] = zz;
+    }
+    stats_out[0] = s[7][0];
+    stats_out[1] += s[7][0];  // running total since the last reset
+  }
+}
+
+// ------------------------------------------------------------------------
+// FP64 fallback kernels
+// ------------------------------------------------------------------------
 // Every requested atom accumulates its own derivative from all its partners
 // (no atomics); the value and the virial are taken once per pair.
 __global__ void k_allpairs(int nA, int nB, int use_pbc, PbcArg P, p2a_switch sw, const double* __restrict__ pos,
@@ -771,6 +796,16 @@ __global__ void k_pairmode(int nA, int use_pbc, PbcArg P, p2a_switch sw, const d
 // ------------------------------------------------------------------------
 // host context
 // ------------------------------------------------------------------------
+struct Dir {  // one cluster-pair list: i clusters of group I against group J
+  int I = 0, J = 0;
+  DBuf<int> ecount, estart, istart;
+  DBuf<Entry> ents;
+  DBuf<int2> items;
+  DBuf<double> fpart;  // per item: derivative partials of the 8 i atoms
+  int nitems = 0, item_offset = 0;
+  int64_t nentries = 0;
+};
+
 struct Group {
   int off = 0, n = 0;
   int ncx = 1, ncy = 1, ncol = 1;
@@ -797,10 +832,9 @@ struct p2a_coord {
   double box_at_build[9] = {0};
   bool built = false;
   p2a::Group g[2];
-  p2a::DBuf<int> totals;       // [0..1] atoms/clusters, [2] entries, [3] items
-  p2a::DBuf<int> ecount, estart, istart;
-  p2a::DBuf<p2a::Entry> ents;
-  p2a::DBuf<int2> items;
+  p2a::DBuf<int> totals;       // [4..7] atoms/clusters per group, [8..11] entries/items per direction
+  p2a::Dir dir[2];
+  int ndir = 0;
   int nitems = 0;
   int64_t nentries = 0;
   p2a::DBuf<double> partial, stats, pos_buf, deriv_buf, out_buf;
@@ -813,6 +847,8 @@ struct p2a_coord {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
   bool has_duplicates = false;  // an absolute index listed twice in one group
   p2a::PbcHost pbc;
+  p2a::PbcArg pbc_stage;
+  p2a::DBuf<p2a::PbcArg> pbc_dev;
   double pbc_box[9] = {0};
   bool pbc_valid = false;
 };
@@ -888,7 +924,7 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
       return rc;
   }
   int rc;
-  if ((rc = c->totals.reserve(8))) return rc;
+  if ((rc = c->totals.reserve(16))) return rc;
   hipStream_t s = c->stream;
   for (int gi = 0; gi < ngroups; gi++) {
     Group& G = c->g[gi];
@@ -919,53 +955,71 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
       hipLaunchKernelGGL(k_origin, dim3(grid1(G.ncl, 128)), dim3(128), 0, s, G.ncl, G.slot_loc.p, G.info.p, pos_dev,
                          G.off, B, G.origin.p, G.bb0.p);
   }
-  // cluster-pair list: i clusters from the larger group when there are two
-  int I = 0, J = 0;
-  if (ngroups == 2) {
-    I = (c->g[1].n > c->g[0].n) ? 1 : 0;
-    J = 1 - I;
-  }
-  Group& GI = c->g[I];
-  Group& GJ = c->g[J];
-  ListArgs L;
-  L.nclI = GI.ncl;
-  L.ncxI = GI.ncx;
-  L.ncyI = GI.ncy;
-  L.ncxJ = GJ.ncx;
-  L.ncyJ = GJ.ncy;
-  L.same = (ngroups == 1);
-  L.rlist2 = (float)(c->rlist * c->rlist);
-  L.rb[0] = (float)(c->rlist / w[0]);
-  L.rb[1] = (float)(c->rlist / w[1]);
-  for (int k = 0; k < 9; k++) {
-    L.A[k] = (float)box9[k];
-    L.Ad[k] = box9[k];
-  }
-  if ((rc = c->ecount.reserve(GI.ncl + 1)) || (rc = c->estart.reserve(GI.ncl + 1)) || (rc = c->istart.reserve(GI.ncl + 1)))
-    return rc;
+  // full cluster-pair lists: one direction for a single group, A->B (the
+  // counting side) and B->A for two groups
+  c->ndir = ngroups;
+  int64_t tot_entries = 0;
   const int wpb = 256 / WAVE;
-  if (GI.ncl > 0) {
-    hipLaunchKernelGGL(k_list<0>, dim3(grid1(GI.ncl, wpb)), dim3(256), 0, s, L, GI.cl_col.p, GI.bb0.p, GJ.bb0.p,
-                       GJ.col_cl.p, GI.origin.p, GJ.origin.p, c->ecount.p, (const int*)nullptr, (Entry*)nullptr);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, GI.ncl, c->ecount.p, c->estart.p, c->totals.p, 2, 0);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, GI.ncl, c->ecount.p, c->istart.p, c->totals.p, 3, 1);
-  } else {
-    P2A_HIP(hipMemsetAsync(c->totals.p + 2, 0, 2 * sizeof(int), s));
+  ListArgs Ls[2];
+  for (int d = 0; d < c->ndir; d++) {
+    Dir& D = c->dir[d];
+    D.I = (ngroups == 2) ? d : 0;
+    D.J = (ngroups == 2) ? 1 - d : 0;
+    Group& GI = c->g[D.I];
+    Group& GJ = c->g[D.J];
+    ListArgs& L = Ls[d];
+    L.nclI = GI.ncl;
+    L.ncxI = GI.ncx;
+    L.ncyI = GI.ncy;
+    L.ncxJ = GJ.ncx;
+    L.ncyJ = GJ.ncy;
+    L.mode = (ngroups == 1) ? 0 : (d == 0 ? 1 : 2);
+    L.rlist2 = (float)(c->rlist * c->rlist);
+    L.rb[0] = (float)(c->rlist / w[0]);
+    L.rb[1] = (float)(c->rlist / w[1]);
+    for (int k = 0; k < 9; k++) {
+      L.A[k] = (float)box9[k];
+      L.Ad[k] = box9[k];
+    }
+    if ((rc = D.ecount.reserve(GI.ncl + 1)) || (rc = D.estart.reserve(GI.ncl + 1)) ||
+        (rc = D.istart.reserve(GI.ncl + 1)))
+      return rc;
+    if (GI.ncl > 0) {
+      hipLaunchKernelGGL(k_list<0>, dim3(grid1(GI.ncl, wpb)), dim3(256), 0, s, L, GI.cl_col.p, GI.bb0.p, GJ.bb0.p,
+                         GJ.col_cl.p, GI.origin.p, GJ.origin.p, D.ecount.p, (const int*)nullptr, (Entry*)nullptr);
+      hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, GI.ncl, D.ecount.p, D.estart.p, c->totals.p, 8 + 2 * d, 0);
+      hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, GI.ncl, D.ecount.p, D.istart.p, c->totals.p, 9 + 2 * d, 1);
+    } else {
+      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, c->totals.p + 8 + 2 * d, 2, 0);
+    }
   }
   if ((rc = launch_check("rebuild: list count"))) return rc;
-  P2A_HIP(hipMemcpyAsync(hostTot, c->totals.p, sizeof(hostTot), hipMemcpyDeviceToHost, s));
+  int hostT[16];
+  P2A_HIP(hipMemcpyAsync(hostT, c->totals.p, sizeof(hostT), hipMemcpyDeviceToHost, s));
   P2A_HIP(hipStreamSynchronize(s));
-  c->nentries = hostTot[2];
-  c->nitems = hostTot[3];
-  if ((rc = c->ents.reserve(std::max<int64_t>(c->nentries, 1))) || (rc = c->items.reserve(std::max(c->nitems, 1))) ||
-      (rc = c->partial.reserve(8 * (size_t)std::max(c->nitems, 1))))
-    return rc;
-  if (GI.ncl > 0) {
-    hipLaunchKernelGGL(k_list<1>, dim3(grid1(GI.ncl, wpb)), dim3(256), 0, s, L, GI.cl_col.p, GI.bb0.p, GJ.bb0.p,
-                       GJ.col_cl.p, GI.origin.p, GJ.origin.p, c->ecount.p, (const int*)c->estart.p, c->ents.p);
-    hipLaunchKernelGGL(k_items, dim3(grid1(GI.ncl, 256)), dim3(256), 0, s, GI.ncl, c->ecount.p, c->istart.p,
-                       c->items.p);
+  c->nitems = 0;
+  for (int d = 0; d < c->ndir; d++) {
+    Dir& D = c->dir[d];
+    Group& GI = c->g[D.I];
+    Group& GJ = c->g[D.J];
+    D.nentries = hostT[8 + 2 * d];
+    D.nitems = hostT[9 + 2 * d];
+    D.item_offset = c->nitems;
+    c->nitems += D.nitems;
+    tot_entries += D.nentries;
+    if ((rc = D.ents.reserve(std::max<int64_t>(D.nentries, 1))) || (rc = D.items.reserve(std::max(D.nitems, 1))) ||
+        (rc = D.fpart.reserve((size_t)CL * 3 * std::max(D.nitems, 1))))
+      return rc;
+    if (GI.ncl > 0) {
+      hipLaunchKernelGGL(k_list<1>, dim3(grid1(GI.ncl, wpb)), dim3(256), 0, s, Ls[d], GI.cl_col.p, GI.bb0.p,
+                         GJ.bb0.p, GJ.col_cl.p, GI.origin.p, GJ.origin.p, D.ecount.p, (const int*)D.estart.p,
+                         D.ents.p);
+      hipLaunchKernelGGL(k_items, dim3(grid1(GI.ncl, 256)), dim3(256), 0, s, GI.ncl, D.ecount.p, D.istart.p,
+                         D.items.p);
+    }
   }
+  c->nentries = tot_entries;
+  if ((rc = c->partial.reserve(8 * (size_t)std::max(c->nitems, 1)))) return rc;
   // absolute index per slot (two groups: pairs of one atom with itself are skipped)
   if (ngroups == 2) {
     std::vector<int> loc[2];
@@ -988,8 +1042,36 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
   return P2A_OK;
 }
 
+static void fill_pbc_arg(const PbcHost& h, PbcArg& P) {
+  std::memset(&P, 0, sizeof(P));
+  P.type = h.type;
+  std::memcpy(P.box, h.box, sizeof(P.box));
+  std::memcpy(P.inv, h.inv, sizeof(P.inv));
+  std::memcpy(P.red, h.reduced, sizeof(P.red));
+  std::memcpy(P.ired, h.invReduced, sizeof(P.ired));
+  std::memcpy(P.nshift, h.nshift, sizeof(P.nshift));
+  std::memcpy(P.shifts, h.shifts, sizeof(P.shifts));
+}
+
+// prepared cell for this box, on the host (fallback kernels take it by value)
+// and on the device (the pair kernel reads it for its FP64 cut decisions)
+static int upload_pbc(p2a_coord* c, const double* box9) {
+  int rc;
+  if (c->pbc_valid && std::memcmp(c->pbc_box, box9, sizeof(c->pbc_box)) == 0 && c->pbc_dev.p) return P2A_OK;
+  if ((rc = c->pbc_dev.reserve(1))) return rc;
+  // the previous upload may still read the staging copy
+  P2A_HIP(hipStreamSynchronize(c->stream));
+  pbc_setup(box9, c->pbc);
+  std::memcpy(c->pbc_box, box9, sizeof(c->pbc_box));
+  c->pbc_valid = true;
+  fill_pbc_arg(c->pbc, c->pbc_stage);
+  P2A_HIP(hipMemcpyAsync(c->pbc_dev.p, &c->pbc_stage, sizeof(PbcArg), hipMemcpyHostToDevice, c->stream));
+  return P2A_OK;
+}
+
 static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, double* value_dev, double* deriv_dev,
                      double* virial_dev) {
+  int rc;
   hipStream_t s = c->stream;
   const int ngroups = c->spec.nb > 0 ? 2 : 1;
   BoxArg B;
@@ -1001,22 +1083,10 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
                        G.info.p, pos_dev, G.off, B, G.origin.p, G.p.p);
     hipLaunchKernelGGL(k_bbox, dim3(grid1(G.ncl, 256)), dim3(256), 0, s, G.ncl, G.p.p, G.bb.p);
   }
-  hipLaunchKernelGGL(k_fill_double, dim3(std::min(grid1(3 * (size_t)c->natoms, 256), 2048)), dim3(256), 0, s,
-                     deriv_dev, 3 * (size_t)c->natoms);
-  int I = 0, J = 0;
-  if (ngroups == 2) {
-    I = (c->g[1].n > c->g[0].n) ? 1 : 0;
-    J = 1 - I;
-  }
-  Group& GI = c->g[I];
-  Group& GJ = c->g[J];
   const p2a_switch& sw = c->spec.sw;
   PairArgs A;
   std::memset(&A, 0, sizeof(A));
-  A.nitems = c->nitems;
   A.twogroup = ngroups == 2;
-  A.offI = GI.off;
-  A.offJ = GJ.off;
   A.dmax2 = (float)sw.dmax_2;
   double pr = sw.dmax + 1e-4;
   A.prune2 = (float)(pr * pr);
@@ -1026,7 +1096,8 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
   A.f.shift = (float)sw.shift;
   A.f.cder = (float)(-(double)sw.nn * sw.invr0_2 * sw.stretch);
   A.sw = sw;
-  A.box = B;
+  if ((rc = upload_pbc(c, box9))) return rc;
+  A.pbc = c->pbc_dev.p;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->profiling) {
     if (c->ev_pool.size() < 2) {
@@ -1042,13 +1113,17 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     c->ev_pool.pop_back();
     P2A_HIP(hipEventRecord(e0, s));
   }
-  if (c->nitems > 0) {
-    dim3 grid(grid1(c->nitems, PAIR_BLOCK / WAVE)), block(PAIR_BLOCK);
-    const int* absI = GI.abs_slot.p;
-    const int* absJ = GJ.abs_slot.p;
-#define P2A_PAIRS(P, GEN)                                                                                          \
-  hipLaunchKernelGGL((k_pairs<P, GEN>), grid, block, 0, s, A, c->items.p, c->estart.p, c->ents.p, GI.p.p, GJ.p.p, \
-                     GI.bb.p, GJ.bb.p, absI, absJ, pos_dev, GI.info.p, GJ.info.p, deriv_dev, c->partial.p)
+  for (int d = 0; d < c->ndir; d++) {
+    Dir& D = c->dir[d];
+    if (D.nitems == 0) continue;
+    Group& GI = c->g[D.I];
+    Group& GJ = c->g[D.J];
+    A.nitems = D.nitems;
+    dim3 grid(grid1(D.nitems, PAIR_BLOCK / WAVE)), block(PAIR_BLOCK);
+    double* part = c->partial.p + 8 * (size_t)D.item_offset;
+#define P2A_PAIRS(P, GEN)                                                                                       \
+  hipLaunchKernelGGL((k_pairs<P, GEN>), grid, block, 0, s, A, D.items.p, D.estart.p, D.ents.p, GI.p.p, GJ.p.p, \
+                     GI.bb.p, GJ.bb.p, GI.abs_slot.p, GJ.abs_slot.p, pos_dev, D.fpart.p, part)
     if (sw.type == P2A_SW_RATIONAL_FIXED) {
       switch (sw.nn / 2) {
         case 1: P2A_PAIRS(1, false); break;
@@ -1068,9 +1143,15 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     P2A_HIP(hipEventRecord(e1, s));
     c->pending.emplace_back(e0, e1);
   }
-  int rc;
   if ((rc = launch_check("pair kernel"))) return rc;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, c->nitems, c->partial.p, value_dev, virial_dev,
+  for (int d = 0; d < c->ndir; d++) {
+    Dir& D = c->dir[d];
+    Group& GI = c->g[D.I];
+    if (GI.ncl == 0) continue;
+    hipLaunchKernelGGL(k_gather, dim3(grid1((size_t)GI.ncl * CL * 3, 256)), dim3(256), 0, s, GI.ncl, GI.p.p,
+                       D.istart.p, D.fpart.p, deriv_dev);
+  }
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(512), 0, s, c->nitems, c->partial.p, value_dev, virial_dev,
                      c->stats.p);
   return launch_check("finalize");
 }
@@ -1078,20 +1159,9 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
 static int run_fallback(p2a_coord* c, const double* box9, const double* pos_dev, double* value_dev,
                         double* deriv_dev, double* virial_dev) {
   hipStream_t s = c->stream;
-  if (!c->pbc_valid || std::memcmp(c->pbc_box, box9, sizeof(c->pbc_box)) != 0) {
-    pbc_setup(box9, c->pbc);
-    std::memcpy(c->pbc_box, box9, sizeof(c->pbc_box));
-    c->pbc_valid = true;
-  }
-  PbcArg P;
-  std::memset(&P, 0, sizeof(P));
-  P.type = c->pbc.type;
-  std::memcpy(P.box, c->pbc.box, sizeof(P.box));
-  std::memcpy(P.inv, c->pbc.inv, sizeof(P.inv));
-  std::memcpy(P.red, c->pbc.reduced, sizeof(P.red));
-  std::memcpy(P.ired, c->pbc.invReduced, sizeof(P.ired));
-  std::memcpy(P.nshift, c->pbc.nshift, sizeof(P.nshift));
-  std::memcpy(P.shifts, c->pbc.shifts, sizeof(P.shifts));
+  int rc0;
+  if ((rc0 = upload_pbc(c, box9))) return rc0;
+  const PbcArg& P = c->pbc_stage;
   const int nA = (int)c->spec.na, nB = (int)c->spec.nb;
   int nblocks;
   int rc;
@@ -1107,7 +1177,7 @@ static int run_fallback(p2a_coord* c, const double* box9, const double* pos_dev,
                        pos_dev, c->abs_dev.p, deriv_dev, c->partial.p);
   }
   if ((rc = launch_check("fp64 pair kernel"))) return rc;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, std::max(nblocks, 1), c->partial.p, value_dev, virial_dev,
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(512), 0, s, std::max(nblocks, 1), c->partial.p, value_dev, virial_dev,
                      c->stats.p);
   return launch_check("finalize");
 }
@@ -1116,7 +1186,10 @@ static int calc_device(p2a_coord* c, int64_t step, const double* box9, const dou
                        double* deriv_dev, double* virial_dev) {
   int rc;
   if (!box9) return fail(P2A_ERR_ARG, "box is required");
-  if ((rc = c->stats.reserve(8))) return rc;
+  if (!c->stats.p) {
+    if ((rc = c->stats.reserve(8))) return rc;
+    hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(64), 0, c->stream, c->stats.p, (size_t)8);
+  }
   double* vdev = value_dev;
   double* wdev = virial_dev;
   double* ddev = deriv_dev;
@@ -1274,14 +1347,14 @@ int p2a_coord_get_stats(p2a_coord* c, p2a_coord_stats* st) {
   std::memset(st, 0, sizeof(*st));
   st->natoms = c->natoms;
   if (c->stats.p) {
-    double d[8];
-    P2A_HIP(hipMemcpy(d, c->stats.p, sizeof(double), hipMemcpyDeviceToHost));
+    double d[2];
+    P2A_HIP(hipMemcpy(d, c->stats.p, 2 * sizeof(double), hipMemcpyDeviceToHost));
     st->contributing_pairs = (int64_t)d[0];
+    st->total_contributing_pairs = (int64_t)d[1];
   }
   const int ngroups = c->spec.nb > 0 ? 2 : 1;
-  int I = (ngroups == 2 && c->g[1].n > c->g[0].n) ? 1 : 0;
-  st->clusters_i = c->g[I].ncl;
-  st->clusters_j = c->g[ngroups == 2 ? 1 - I : I].ncl;
+  st->clusters_i = c->g[0].ncl;
+  st->clusters_j = c->g[ngroups - 1].ncl;
   st->list_entries = c->nentries;
   st->rebuilds = c->rebuilds;
   st->path = c->path;
@@ -1300,6 +1373,10 @@ int p2a_coord_set_profiling(p2a_coord* c, int on) {
   if (on) {
     c->prof_ms = 0.0;
     c->prof_launches = 0;
+    if (c->stats.p) {
+      hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(64), 0, c->stream, c->stats.p + 1, (size_t)1);
+      P2A_HIP(hipStreamSynchronize(c->stream));
+    }
   }
   return P2A_OK;
 }

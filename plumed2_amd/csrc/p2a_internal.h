@@ -16,7 +16,7 @@ std::string switch_description(const p2a_switch& s);
 
 // Periodic cell as Pbc::setBox prepares it (Pbc.cpp:165-212): type, inverse,
 // reduced lattice and the per-octant shift lists of Pbc::buildShifts.
-constexpr int kMaxShift = 16;
+constexpr int kMaxShift = 8;
 struct PbcHost {
   int type = 0;  // 0 unset, 1 orthorhombic, 2 generic
   double box[9] = {0}, inv[9] = {0}, reduced[9] = {0}, invReduced[9] = {0};

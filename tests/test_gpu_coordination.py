@@ -1,0 +1,288 @@
+"""GPU parity of COORDINATION (HIP library through the C ABI) against the FP64
+oracle and the reference regtest golden files.
+
+Tolerances (FP32 pair arithmetic, FP64 accumulation):
+  CV      |dCV| <= 1e-6 |CV|
+  forces  max |dF| <= 1e-5 max |F|
+  virial  max |dW| <= 1e-5 max |W|
+FP64 paths (NOPBC, PAIR, unbounded D_MAX) are held to 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from fixtures import path, read_colvar, read_deriv, read_xyz
+
+pytestmark = pytest.mark.gpu
+
+CV_TOL, F_TOL, W_TOL = 1e-6, 1e-5, 1e-5
+
+
+def _P():
+    import plumed2_amd as P
+    return P
+
+
+def check(v, d, w, ov, od, ow, cv_tol=CV_TOL, f_tol=F_TOL, w_tol=W_TOL, tag=""):
+    assert abs(v - ov) <= cv_tol * max(abs(ov), 1e-12), (tag, v, ov)
+    fmax = max(np.max(np.abs(od)), 1e-12)
+    assert np.max(np.abs(d - od)) <= f_tol * fmax, (tag, np.max(np.abs(d - od)) / fmax)
+    wmax = max(np.max(np.abs(ow)), 1e-12)
+    assert np.max(np.abs(w - ow)) <= w_tol * wmax, (tag, np.max(np.abs(w - ow)) / wmax)
+
+
+def uniform(n, rho=33.4, seed=0):
+    L = (n / rho) ** (1 / 3)
+    return np.random.default_rng(seed).uniform(0, L, (n, 3)), L
+
+
+def test_two_group_c1_shape():
+    """GROUPA=1000, GROUPB=1000 atoms in a 5 nm cube, no neighbour list."""
+    P = _P()
+    x = np.random.default_rng(1).uniform(0, 5, (2000, 3))
+    box = np.diag([5.0, 5, 5])
+    defn = "RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.0"
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-1000 GROUPB=1001-2000 SWITCH={{{defn}}}")
+    v, d, w = act.calculate(x, box)
+    ov, od, ow, npair = O.coordination(O.Switch(defn), box, x, 1000, 1000)
+    check(v, d, w, ov, od, ow)
+    st = act.stats()
+    assert st["path"] == 0 and st["contributing_pairs"] == npair
+
+
+@pytest.mark.parametrize("n", [2, 9, 64, 513, 4000])
+def test_single_group_sizes(n):
+    P = _P()
+    x, L = uniform(max(n, 64), seed=n)
+    x = x[:n] if n < 64 else x
+    box = np.diag([max(L, 3.0)] * 3)
+    defn = "RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2"
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{n} SWITCH={{{defn}}}")
+    v, d, w = act.calculate(x, box)
+    ov, od, ow, npair = O.coordination(O.Switch(defn), box, x, n, 0)
+    if npair == 0:
+        assert v == 0.0 and np.all(d == 0)
+    else:
+        check(v, d, w, ov, od, ow, tag=n)
+    assert act.stats()["contributing_pairs"] == npair
+
+
+def test_neighbour_list_stride_random_walk():
+    """NLISTCELLS NL_CUTOFF=1.5 NL_STRIDE=10 over a random walk: every frame
+    against the oracle driven by the reference cell-list candidate set rebuilt
+    on the same steps."""
+    P = _P()
+    n = 4000
+    x, L = uniform(n, seed=2)
+    box = np.diag([L] * 3)
+    defn = "RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2"
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{n} SWITCH={{{defn}}} NLISTCELLS NL_CUTOFF=1.5 NL_STRIDE=10")
+    sw = O.Switch(defn)
+    rng = np.random.default_rng(3)
+    pairs = None
+    for step in range(23):
+        if step % 10 == 0:
+            pairs = O.nl_cells(O.Pbc(box), 1.5, x, n, 0)
+        v, d, w = act.calculate(x, box, step=step)
+        ov, od, ow, npair = O.coordination(sw, box, x, n, 0, pairs=pairs)
+        check(v, d, w, ov, od, ow, tag=step)
+        assert act.stats()["contributing_pairs"] == npair
+        x = x + rng.normal(0, 0.005, x.shape)
+    assert act.stats()["rebuilds"] == 3
+
+
+@pytest.mark.parametrize("tilt", [(0.1, -0.05, 0.08), (-0.1, 0.1, -0.1)])
+def test_triclinic(tilt):
+    P = _P()
+    n = 3000
+    x, L = uniform(n, seed=4)
+    box = np.array([[L, 0, 0], [tilt[0] * L, L, 0], [tilt[1] * L, tilt[2] * L, L]])
+    defn = "RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2"
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{n} SWITCH={{{defn}}}")
+    v, d, w = act.calculate(x, box)
+    ov, od, ow, npair = O.coordination(O.Switch(defn), box, x, n, 0)
+    check(v, d, w, ov, od, ow)
+    assert act.stats()["path"] == 0 and act.stats()["contributing_pairs"] == npair
+
+
+@pytest.mark.parametrize("defn", ["EXP R_0=0.2 D_MAX=1.0", "GAUSSIAN R_0=0.3 D_0=0.1 D_MAX=1.0",
+                                  "RATIONAL R_0=0.3 NN=5 MM=11 D_MAX=1.0", "RATIONAL R_0=0.35 NN=8 MM=16 D_MAX=1.0",
+                                  "TANH R_0=0.3 D_MAX=1.0", "COSINUS R_0=1.0", "SMAP R_0=0.3 A=3 B=2 D_MAX=1.0",
+                                  "CUBIC D_0=0.1 D_MAX=1.0", "Q R_0=1.0 D_0=0.1 BETA=5.0 LAMBDA=1.0 REF=0.4 D_MAX=1.0",
+                                  "GAUSSIAN R_0=1.0 D_MAX=1.0"])
+def test_switch_types(defn):
+    P = _P()
+    n = 2500
+    x, L = uniform(n, seed=5)
+    box = np.diag([L] * 3)
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{n} SWITCH={{{defn}}}")
+    v, d, w = act.calculate(x, box)
+    ov, od, ow, npair = O.coordination(O.Switch(defn), box, x, n, 0)
+    check(v, d, w, ov, od, ow, tag=defn)
+
+
+def test_overlapping_groups_skip_self_pairs():
+    P = _P()
+    x, L = uniform(900, seed=6)
+    box = np.diag([L] * 3)
+    ga = np.arange(600)
+    gb = np.arange(300, 900)
+    req = np.concatenate([ga, gb])
+    defn = "RATIONAL R_0=0.3 D_MAX=1.0"
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-600 GROUPB=301-900 SWITCH={{{defn}}}")
+    v, d, w = act.calculate(x[req], box)
+    ov, od, ow, _ = O.coordination(O.Switch(defn), box, x[req], 600, 600, absidx=req)
+    check(v, d, w, ov, od, ow)
+
+
+def test_groupb_larger_than_groupa():
+    P = _P()
+    x, L = uniform(3000, seed=7)
+    box = np.diag([L] * 3)
+    defn = "RATIONAL R_0=0.35 NN=8 MM=16 D_MAX=1.0"
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-200 GROUPB=201-3000 SWITCH={{{defn}}}")
+    v, d, w = act.calculate(x, box)
+    ov, od, ow, npair = O.coordination(O.Switch(defn), box, x, 200, 2800)
+    check(v, d, w, ov, od, ow)
+    assert act.stats()["contributing_pairs"] == npair
+
+
+def test_fp64_paths():
+    """NOPBC, PAIR and unbounded D_MAX run the FP64 kernels."""
+    P = _P()
+    x, L = uniform(1200, seed=8)
+    box = np.diag([L] * 3)
+    sw = "RATIONAL R_0=0.3 D_MAX=1.0"
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-600 GROUPB=601-1200 SWITCH={{{sw}}} NOPBC")
+    v, d, w = act.calculate(x, box)
+    ov, od, ow, _ = O.coordination(O.Switch(sw), box, x, 600, 600, use_pbc=False)
+    assert act.stats()["path"] == 1
+    check(v, d, w, ov, od, ow, 1e-10, 1e-10, 1e-10)
+    act = P.Coordination("c: COORDINATION GROUPA=1-600 GROUPB=601-1200 PAIR R_0=0.3")
+    v, d, w = act.calculate(x, box)
+    ov, od, ow, _ = O.coordination(O.Switch(nn=6, mm=0, r0=0.3), box, x, 600, 600, dopair=True)
+    assert act.stats()["path"] == 2
+    check(v, d, w, ov, od, ow, 1e-10, 1e-10, 1e-10)
+    act = P.Coordination("c: COORDINATION GROUPA=1-400 SWITCH={EXP R_0=0.3}")
+    v, d, w = act.calculate(x[:400], box)
+    ov, od, ow, _ = O.coordination(O.Switch("EXP R_0=0.3"), box, x[:400], 400, 0)
+    check(v, d, w, ov, od, ow, 1e-10, 1e-10, 1e-10)
+
+
+RT20 = {
+    "rational": "RATIONAL R_0=1.3 NN=5 MM=11 D_MAX=2.6",
+    "fastrational": "RATIONAL R_0=1.3 NN=6 MM=10 D_MAX=2.6",
+    "fastrational-MMeq2NN": "RATIONAL R_0=1.3 D_MAX=2.6",
+    "exp": "EXP R_0=0.8 D_0=0.5 D_MAX=2.6",
+    "gaussian": "GAUSSIAN R_0=1.0 D_0=0.3 D_MAX=2.6",
+    "fastgaussian": "GAUSSIAN R_0=1.0 D_0=0.0 D_MAX=2.6",
+    "smap": "SMAP R_0=1.3 A=3 B=2 D_MAX=2.6",
+    "cubic": "CUBIC D_0=0 D_MAX=2.6",
+    "tanh": "TANH R_0=1.3 D_MAX=2.6",
+    "q": "Q R_0=1.0 D_0=0.3 BETA=1.0 LAMBDA=1.0 REF=1.3 D_MAX=2.6",
+}
+
+
+@pytest.mark.parametrize("name", sorted(RT20))
+def test_rt20_golden(name):
+    """regtest/basic/rt20-switch-*: the 2-atom sweep through the GPU (1-atom
+    clusters on both sides), against the golden COLVAR (%f) and derivatives (%8.4f)."""
+    P = _P()
+    frames = read_xyz(path("switchtraj", "switchtraj.xyz"))
+    names, colvar = read_colvar(path(f"rt20-switch-{name}", "COLVAR.reference"))
+    deriv = read_deriv(path(f"rt20-switch-{name}", "deriv.reference"))
+    for label, defn in (("c", RT20[name] + " NOSTRETCH"), ("cs", RT20[name])):
+        act = P.Coordination(f"{label}: COORDINATION GROUPA=1 GROUPB=2 SWITCH={{{defn}}}")
+        col = names.index(label)
+        for k, (pos, box) in enumerate(frames):
+            v, d, w = act.calculate(pos, box, step=k)
+            assert abs(v - colvar[k, col]) <= 5.01e-7 + 1e-6 * abs(colvar[k, col]), (label, k)
+            params = np.concatenate([d.reshape(-1), w.reshape(-1)])
+            np.testing.assert_allclose(params, deriv[label][k, :15], atol=5.1e-5, err_msg=f"{label} frame {k}")
+
+
+def test_rt42_golden_triclinic():
+    """regtest/basic/rt42 check_c / check_e / check_g on the triclinic 108-atom frames."""
+    P = _P()
+    frames = read_xyz(path("rt42", "trajectory.xyz"))
+    lines = {"check_c": "c: COORDINATION GROUPA=1-108 GROUPB=1-108 R_0=1",
+             "check_e": "e: COORDINATION GROUPA=1-108 GROUPB=1-108 R_0=1 NN=7 D_0=0.1",
+             "check_g": "g: COORDINATION GROUPA=1-108 GROUPB=1-108 SWITCH={EXP R_0=1}"}
+    for key, line in lines.items():
+        _, ref = read_colvar(path("rt42", key + ".reference"))
+        act = P.Coordination(line)
+        for k in range(0, len(frames), 5):
+            pos, box = frames[k]
+            v, _, _ = act.calculate(np.concatenate([pos, pos]), box, step=k)
+            assert abs(v - ref[k, 1]) <= 5.01e-5 + 1e-6 * abs(ref[k, 1]), (key, k, v, ref[k, 1])
+
+
+def test_rt42c_golden():
+    P = _P()
+    frames = read_xyz(path("rt42c", "trajectory.xyz"))
+    _, ref = read_colvar(path("rt42c", "colvar.reference"))
+    act = P.Coordination("c: COORDINATION GROUPA=1-108 R_0=1")
+    for k in range(0, len(frames), 3):
+        pos, box = frames[k]
+        v, _, _ = act.calculate(pos, box, step=k)
+        assert abs(v - ref[k, 1]) <= 5.01e-5 + 1e-6 * abs(ref[k, 1])
+
+
+def test_repeatable_value_and_virial():
+    """CV and virial are reduced in a fixed order: bitwise repeatable."""
+    P = _P()
+    x, L = uniform(6000, seed=9)
+    box = np.diag([L] * 3)
+    act = P.Coordination("c: COORDINATION GROUPA=1-6000 SWITCH={RATIONAL R_0=0.3 D_MAX=1.2}")
+    v1, d1, w1 = act.calculate(x, box, 0)
+    v2, d2, w2 = act.calculate(x, box, 1)
+    assert v1 == v2
+    np.testing.assert_array_equal(w1, w2)
+    np.testing.assert_allclose(d1, d2, rtol=0, atol=1e-12 * np.max(np.abs(d1)))
+
+
+def test_full_size_properties():
+    """50,000 atoms (the benchmark workload, frame 0): value, derivatives and
+    virial against the oracle driven by the reference cell-list candidates."""
+    P = _P()
+    n = 50000
+    x, L = uniform(n, seed=2)
+    box = np.diag([L] * 3)
+    defn = "RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2"
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{n} SWITCH={{{defn}}} NLISTCELLS NL_CUTOFF=1.5 NL_STRIDE=10")
+    v, d, w = act.calculate(x, box, 0)
+    pairs = O.nl_cells(O.Pbc(box), 1.5, x, n, 0)
+    ov, od, ow, npair = O.coordination(O.Switch(defn), box, x, n, 0, pairs=pairs)
+    check(v, d, w, ov, od, ow)
+    assert act.stats()["contributing_pairs"] == npair
+    assert np.max(np.abs(d.sum(0))) < 1e-9 * np.max(np.abs(d)) * n
+
+
+def test_device_entry_point_torch_stream():
+    import torch
+
+    P = _P()
+    n = 3000
+    x, L = uniform(n, seed=10)
+    box = np.diag([L] * 3)
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{n} SWITCH={{RATIONAL R_0=0.3 D_MAX=1.2}}")
+    xt = torch.from_numpy(x).cuda()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        val, der, vir = act.calculate_torch(xt, box, step=0)
+    torch.cuda.synchronize()
+    v, d, w = act.calculate(x, box, step=1)
+    assert float(val.item()) == v
+    np.testing.assert_allclose(der.cpu().numpy(), d, atol=1e-12 * np.max(np.abs(d)))
+    np.testing.assert_array_equal(vir.cpu().numpy(), w)
+
+
+def test_errors_are_raised():
+    P = _P()
+    with pytest.raises(P.Plumed2AmdError, match="R_0 should be explicitly specified"):
+        P.Coordination("c: COORDINATION GROUPA=1-10")
+    with pytest.raises(P.Plumed2AmdError, match="D_MAX larger than NL_CUTOFF"):
+        P.Coordination("c: COORDINATION GROUPA=1-10 SWITCH={RATIONAL R_0=1 D_MAX=3} NLIST NL_CUTOFF=2 NL_STRIDE=5")
+    act = P.Coordination("c: COORDINATION GROUPA=1-10 R_0=1")
+    with pytest.raises(ValueError):
+        act.calculate(np.zeros((9, 3)), np.diag([3.0] * 3))
