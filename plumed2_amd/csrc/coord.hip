@@ -223,22 +223,48 @@ __global__ void k_scatter(int n, const int4* __restrict__ info, const int* __res
   tmp[col_start[col] + k] = i;
 }
 
-// rank of each atom inside its column by (key, index): deterministic order
-__global__ void k_rank(int n, const int4* __restrict__ info, const double* __restrict__ key,
-                       const int* __restrict__ col_start, const int* __restrict__ col_cl, const int* __restrict__ tmp,
-                       int* __restrict__ slot_loc) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int col = info[i].w;
-  int b = col_start[col], e = col_start[col + 1];
-  double ka = key[i];
-  int rank = 0;
-  for (int t = b; t < e; t++) {
-    int o = tmp[t];
-    double kb = key[o];
-    rank += (kb < ka) || (kb == ka && o < i);
+// rank of each atom inside its column by (key, index): deterministic order.
+// One workgroup per column; the column's keys are staged in LDS when they fit.
+constexpr int kRankLds = 2048;
+__global__ __launch_bounds__(256) void k_rank(int ncol, const double* __restrict__ key,
+                                              const int* __restrict__ col_start, const int* __restrict__ col_cl,
+                                              const int* __restrict__ tmp, int* __restrict__ slot_loc) {
+  __shared__ double sk[kRankLds];
+  __shared__ int si[kRankLds];
+  const int col = blockIdx.x;
+  if (col >= ncol) return;
+  const int b = col_start[col], n = col_start[col + 1] - b;
+  const int base = col_cl[col] * CL;
+  if (n <= kRankLds) {
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      int o = tmp[b + t];
+      si[t] = o;
+      sk[t] = key[o];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      const double ka = sk[t];
+      const int ia = si[t];
+      int rank = 0;
+      for (int u = 0; u < n; u++) {
+        const double kb = sk[u];
+        rank += (kb < ka) || (kb == ka && si[u] < ia);
+      }
+      slot_loc[base + rank] = ia;
+    }
+  } else {
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      const int ia = tmp[b + t];
+      const double ka = key[ia];
+      int rank = 0;
+      for (int u = 0; u < n; u++) {
+        const int o = tmp[b + u];
+        const double kb = key[o];
+        rank += (kb < ka) || (kb == ka && o < ia);
+      }
+      slot_loc[base + rank] = ia;
+    }
   }
-  slot_loc[col_cl[col] * CL + rank] = i;
 }
 
 __global__ void k_cl_col(int ncol, const int* __restrict__ col_cl, int* __restrict__ cl_col) {
@@ -494,8 +520,13 @@ __device__ double exact_r2(const PbcArg* __restrict__ P, const double* __restric
 }
 
 // One wave per work item (i cluster, chunk of <=64 list entries).
-// Lane l evaluates the pair (i = l&7 of cluster ci, j = l>>3 of cluster cj).
-template <int P, bool GENERIC>
+//  1. lane-parallel pruning of the chunk against the current bounding boxes;
+//  2. the passing entries are compacted and each lane stages the 8 atoms of
+//     its j cluster into LDS with the image offset already added (8
+//     independent loads per lane);
+//  3. dense loop over the staged clusters: lane l evaluates the pair
+//     (i = l&7 of cluster ci, j = l>>3 of the staged cluster).
+template <int P, bool GENERIC, bool TWO>
 __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __restrict__ items,
                                                        const int* __restrict__ estart,
                                                        const Entry* __restrict__ ents,
@@ -505,9 +536,14 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __
                                                        const int* __restrict__ absI, const int* __restrict__ absJ,
                                                        const double* __restrict__ pos64,
                                                        double* __restrict__ fpart, double* __restrict__ partial) {
+  constexpr int NW = PAIR_BLOCK / WAVE;
+  __shared__ float4 sj[NW][WAVE][CL];
+  __shared__ int scode[NW][WAVE];
+  __shared__ int sabs[TWO ? NW : 1][TWO ? WAVE : 1][CL];
   const int lane = threadIdx.x & 63;
-  const int item = blockIdx.x * (PAIR_BLOCK / WAVE) + (threadIdx.x >> 6);
-  if (item >= A.nitems) return;
+  const int w = threadIdx.x >> 6;
+  const int item = blockIdx.x * NW + w;
+  if (item >= A.nitems) return;  // uniform per wave; no workgroup barrier below
   const int2 it = items[item];
   const int ci = it.x;
   const int e0 = estart[ci] + it.y * 64;
@@ -515,9 +551,8 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __
   const int ii = lane & 7, jj = lane >> 3;
   const float4 pi = posI[ci * CL + ii];
   const int reqi = __float_as_int(pi.w);
-  const int ai = A.twogroup ? absI[ci * CL + ii] : 0;
+  const int ai = TWO ? absI[ci * CL + ii] : 0;
 
-  // lane-parallel pruning of the chunk against the current bounding boxes
   Entry my = {0, 0.f, 0.f, 0.f};
   bool pass = false;
   {
@@ -533,25 +568,37 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __
       pass = gx * gx + gy * gy + gz * gz <= A.prune2;
     }
   }
-  unsigned long long mask = __ballot(pass);
+  const unsigned long long mask = __ballot(pass);
+  const int npass = __popcll(mask);
+  if (pass) {
+    const int k = __popcll(mask & ((1ull << lane) - 1));
+    const int cj = my.code & kCjMask;
+    float4 q[CL];
+#pragma unroll
+    for (int t = 0; t < CL; t++) q[t] = posJ[cj * CL + t];
+#pragma unroll
+    for (int t = 0; t < CL; t++) sj[w][k][t] = make_float4(q[t].x + my.dx, q[t].y + my.dy, q[t].z + my.dz, q[t].w);
+    scode[w][k] = my.code;
+    if constexpr (TWO) {
+#pragma unroll
+      for (int t = 0; t < CL; t++) sabs[w][k][t] = absJ[cj * CL + t];
+    }
+  }
+  // the staging above is read by other lanes of this wave only
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 
   float cv = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
   float vxx = 0.f, vxy = 0.f, vxz = 0.f, vyy = 0.f, vyz = 0.f, vzz = 0.f;
   int npair = 0;
   const float band = 4.0e-6f * A.dmax2;
 
-  while (mask) {
-    const int k = __ffsll((long long)mask) - 1;
-    mask &= mask - 1;
-    const int code = __builtin_amdgcn_readlane(my.code, k);
-    const float Dx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my.dx), k));
-    const float Dy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my.dy), k));
-    const float Dz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my.dz), k));
-    const int cj = code & kCjMask;
-    const float4 pj = posJ[cj * CL + jj];
-    const float dx = (pj.x + Dx) - pi.x;
-    const float dy = (pj.y + Dy) - pi.y;
-    const float dz = (pj.z + Dz) - pi.z;
+  for (int k = 0; k < npass; k++) {
+    const int code = scode[w][k];
+    const float4 pj = sj[w][k][jj];
+    const float dx = pj.x - pi.x;
+    const float dy = pj.y - pi.y;
+    const float dz = pj.z - pi.z;
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     bool cut = r2 <= A.dmax2;
     if (__builtin_expect(__ballot(fabsf(r2 - A.dmax2) < band) != 0ull, 0)) {
@@ -563,9 +610,9 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __
     }
     const bool diag = code < 0;
     if (diag) cut = cut && (jj != ii);  // an atom is not paired with itself
-    if (A.twogroup) cut = cut && (absJ[cj * CL + jj] != ai);
+    if constexpr (TWO) cut = cut && (sabs[w][k][jj] != ai);
     float s, df;
-    if (GENERIC) {
+    if constexpr (GENERIC) {
       double dfd;
       double sd = switch_calc_sqr(A.sw, (double)r2, dfd);
       s = (float)sd;
@@ -593,10 +640,10 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __
       vzz = fmaf(gz, dz, vzz);
     } else if (diag) {
       const bool c2 = cut && (jj > ii);
-      const float w = c2 ? 1.f : 0.f;
+      const float wgt = c2 ? 1.f : 0.f;
       npair += __popcll(__ballot(c2));
-      cv = fmaf(w, s, cv);
-      const float hx = w * gx, hy = w * gy, hz = w * gz;
+      cv = fmaf(wgt, s, cv);
+      const float hx = wgt * gx, hy = wgt * gy, hz = wgt * gz;
       vxx = fmaf(hx, dx, vxx);
       vxy = fmaf(hx, dy, vxy);
       vxz = fmaf(hx, dz, vxz);
@@ -641,21 +688,42 @@ __global__ void k_gather(int ncl, const float4* __restrict__ p, const int* __res
   deriv[3 * (size_t)req + r % 3] = sum;
 }
 
-// deterministic sum of the per-item partials -> value, virial, pair count
-__global__ void k_finalize(int nitems, const double* __restrict__ partial, double* __restrict__ value,
-                           double* __restrict__ virial, double* __restrict__ stats_out) {
-  __shared__ double s[8][512];
+// deterministic sum of the per-item partials -> value, virial, pair count.
+// Stage 1: 128 blocks reduce contiguous ranges; stage 2: one block.
+constexpr int kRedBlocks = 128;
+__global__ __launch_bounds__(256) void k_reduce1(int nitems, const double* __restrict__ partial,
+                                                 double* __restrict__ out) {
+  __shared__ double s[8][256];
   const int t = threadIdx.x;
+  const int per = (nitems + gridDim.x - 1) / gridDim.x;
+  const int b = blockIdx.x * per, e = min(nitems, b + per);
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const double4* p4 = reinterpret_cast<const double4*>(partial);
-  for (int i = t; i < nitems; i += 512) {
-    double4 a = p4[2 * (size_t)i], b = p4[2 * (size_t)i + 1];
-    acc[0] += a.x; acc[1] += a.y; acc[2] += a.z; acc[3] += a.w;
-    acc[4] += b.x; acc[5] += b.y; acc[6] += b.z; acc[7] += b.w;
+  for (int i = b + t; i < e; i += 256) {
+    double4 x = p4[2 * (size_t)i], y = p4[2 * (size_t)i + 1];
+    acc[0] += x.x; acc[1] += x.y; acc[2] += x.z; acc[3] += x.w;
+    acc[4] += y.x; acc[5] += y.y; acc[6] += y.z; acc[7] += y.w;
   }
   for (int q = 0; q < 8; q++) s[q][t] = acc[q];
   __syncthreads();
-  for (int o = 256; o > 0; o >>= 1) {
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o)
+      for (int q = 0; q < 8; q++) s[q][t] += s[q][t + o];
+    __syncthreads();
+  }
+  if (t < 8) out[(size_t)blockIdx.x * 8 + t] = s[t][0];
+}
+
+__global__ void k_finalize(int nitems, const double* __restrict__ partial, double* __restrict__ value,
+                           double* __restrict__ virial, double* __restrict__ stats_out) {
+  __shared__ double s[8][128];
+  const int t = threadIdx.x;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = t; i < nitems; i += 128)
+    for (int q = 0; q < 8; q++) acc[q] += partial[(size_t)i * 8 + q];
+  for (int q = 0; q < 8; q++) s[q][t] = acc[q];
+  __syncthreads();
+  for (int o = 64; o > 0; o >>= 1) {
     if (t < o)
       for (int q = 0; q < 8; q++) s[q][t] += s[q][t + o];
     __syncthreads();
@@ -671,6 +739,17 @@ __global__ void k_finalize(int nitems, const double* __restrict__ partial, doubl
     }
     stats_out[0] = s[7][0];
     stats_out[1] += s[7][0];  // running total since the last reset
+  }
+}
+
+// sum of nitems partials into value / virial / stats in two stages
+static void reduce_partials(hipStream_t s, int nitems, const double* partial, double* scratch, double* value,
+                            double* virial, double* stats) {
+  if (nitems > 4 * kRedBlocks) {
+    hipLaunchKernelGGL(k_reduce1, dim3(kRedBlocks), dim3(256), 0, s, nitems, partial, scratch);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(128), 0, s, kRedBlocks, scratch, value, virial, stats);
+  } else {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(128), 0, s, nitems, partial, value, virial, stats);
   }
 }
 
@@ -837,7 +916,7 @@ struct p2a_coord {
   int ndir = 0;
   int nitems = 0;
   int64_t nentries = 0;
-  p2a::DBuf<double> partial, stats, pos_buf, deriv_buf, out_buf;
+  p2a::DBuf<double> partial, red, stats, pos_buf, deriv_buf, out_buf;
   p2a::DBuf<long long> abs_dev;
   int64_t rebuilds = 0;
   bool profiling = false;
@@ -939,8 +1018,8 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     if (G.n > 0) {
       hipLaunchKernelGGL(k_scatter, dim3(grid1(G.n, 256)), dim3(256), 0, s, G.n, G.info.p, G.col_start.p, G.fill.p,
                          G.tmp.p);
-      hipLaunchKernelGGL(k_rank, dim3(grid1(G.n, 256)), dim3(256), 0, s, G.n, G.info.p, G.key.p, G.col_start.p,
-                         G.col_cl.p, G.tmp.p, G.slot_loc.p);
+      hipLaunchKernelGGL(k_rank, dim3(G.ncol), dim3(256), 0, s, G.ncol, G.key.p, G.col_start.p, G.col_cl.p,
+                         G.tmp.p, G.slot_loc.p);
     }
     hipLaunchKernelGGL(k_cl_col, dim3(grid1(G.ncol, 256)), dim3(256), 0, s, G.ncol, G.col_cl.p, G.cl_col.p);
     if ((rc = launch_check("rebuild: binning"))) return rc;
@@ -1121,9 +1200,15 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     A.nitems = D.nitems;
     dim3 grid(grid1(D.nitems, PAIR_BLOCK / WAVE)), block(PAIR_BLOCK);
     double* part = c->partial.p + 8 * (size_t)D.item_offset;
-#define P2A_PAIRS(P, GEN)                                                                                       \
-  hipLaunchKernelGGL((k_pairs<P, GEN>), grid, block, 0, s, A, D.items.p, D.estart.p, D.ents.p, GI.p.p, GJ.p.p, \
-                     GI.bb.p, GJ.bb.p, GI.abs_slot.p, GJ.abs_slot.p, pos_dev, D.fpart.p, part)
+#define P2A_PAIRS(P, GEN)                                                                                        \
+  do {                                                                                                            \
+    if (A.twogroup)                                                                                               \
+      hipLaunchKernelGGL((k_pairs<P, GEN, true>), grid, block, 0, s, A, D.items.p, D.estart.p, D.ents.p, GI.p.p, \
+                         GJ.p.p, GI.bb.p, GJ.bb.p, GI.abs_slot.p, GJ.abs_slot.p, pos_dev, D.fpart.p, part);      \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_pairs<P, GEN, false>), grid, block, 0, s, A, D.items.p, D.estart.p, D.ents.p, GI.p.p,\
+                         GJ.p.p, GI.bb.p, GJ.bb.p, GI.abs_slot.p, GJ.abs_slot.p, pos_dev, D.fpart.p, part);      \
+  } while (0)
     if (sw.type == P2A_SW_RATIONAL_FIXED) {
       switch (sw.nn / 2) {
         case 1: P2A_PAIRS(1, false); break;
@@ -1151,8 +1236,8 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     hipLaunchKernelGGL(k_gather, dim3(grid1((size_t)GI.ncl * CL * 3, 256)), dim3(256), 0, s, GI.ncl, GI.p.p,
                        D.istart.p, D.fpart.p, deriv_dev);
   }
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(512), 0, s, c->nitems, c->partial.p, value_dev, virial_dev,
-                     c->stats.p);
+  if ((rc = c->red.reserve(8 * kRedBlocks))) return rc;
+  reduce_partials(s, c->nitems, c->partial.p, c->red.p, value_dev, virial_dev, c->stats.p);
   return launch_check("finalize");
 }
 
@@ -1177,8 +1262,8 @@ static int run_fallback(p2a_coord* c, const double* box9, const double* pos_dev,
                        pos_dev, c->abs_dev.p, deriv_dev, c->partial.p);
   }
   if ((rc = launch_check("fp64 pair kernel"))) return rc;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(512), 0, s, std::max(nblocks, 1), c->partial.p, value_dev, virial_dev,
-                     c->stats.p);
+  if ((rc = c->red.reserve(8 * kRedBlocks))) return rc;
+  reduce_partials(s, std::max(nblocks, 1), c->partial.p, c->red.p, value_dev, virial_dev, c->stats.p);
   return launch_check("finalize");
 }
 
