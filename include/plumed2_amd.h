@@ -175,6 +175,74 @@ int p2a_coord_get_stats(p2a_coord *ctx, p2a_coord_stats *stats);
  * (kernel time and total_contributing_pairs). */
 int p2a_coord_set_profiling(p2a_coord *ctx, int on);
 
+/* ------------------------------------------------------------------------
+ * METAD bias on a grid (diagonal Gaussian hills, well-tempered heights)
+ * ------------------------------------------------------------------------ */
+
+typedef struct p2a_metad p2a_metad;
+
+typedef struct p2a_metad_spec {
+  int32_t ncv;                /* 1..4                                             */
+  int32_t periodic[4];        /* per CV (grid wraps, differences use the period)   */
+  double gmin[4], gmax[4];    /* GRID_MIN/GRID_MAX (or the CV domain if periodic)  */
+  uint32_t nbin[4];           /* GRID_BIN                                          */
+  double sigma[4];            /* SIGMA                                             */
+  double height;              /* HEIGHT                                            */
+  double biasf;               /* BIASFACTOR (<=1: not well-tempered)               */
+  double kbt;                 /* k_B T (TEMP * 0.0083144621)                       */
+  int32_t pace;               /* PACE                                              */
+  int32_t nostretch;          /* 1 mimics PLUMED_DP2CUTOFF_NOSTRETCH               */
+} p2a_metad_spec;
+
+/* Grid creation: MetaD::MetaD grid set-up (src/bias/MetaD.cpp:1006-1060,
+ * 1466-1474) and GridBase::Init (src/tools/Grid.cpp:239-286): non-periodic
+ * directions get nbin+1 nodes; value and gradient per node in FP64. */
+int p2a_metad_create(p2a_metad **ctx, int device, const p2a_metad_spec *spec);
+int p2a_metad_destroy(p2a_metad *ctx);
+int p2a_metad_set_stream(p2a_metad *ctx, void *stream);
+/* number of grid nodes */
+int64_t p2a_metad_grid_size(const p2a_metad *ctx);
+
+/* Deposit nhills hills in sequence, each with the well-tempered height
+ * h = HEIGHT*exp(-V(c)/(kbt*(BIASFACTOR-1))) of the grid left by the previous
+ * ones (MetaD::update :2377-2470, getHeight :2280-2305, addGaussian
+ * :1886-1937).  centers: host [nhills][ncv]; heights: host output (may be NULL)
+ * receiving the deposited Gaussian heights.  One device launch. */
+int p2a_metad_deposit(p2a_metad *ctx, int64_t nhills, const double *centers, double *heights);
+/* Same with device arrays; asynchronous. */
+int p2a_metad_deposit_device(p2a_metad *ctx, int64_t nhills, const double *centers_dev, double *heights_dev);
+/* Add hills of given heights (HILLS restart, MetaD::readGaussians :1807-1829). */
+int p2a_metad_add_hills(p2a_metad *ctx, int64_t nhills, const double *centers, const double *heights);
+
+/* Bias and its gradient by the grid spline (GridBase::getValueAndDerivatives
+ * spline branch, src/tools/Grid.cpp:442-505) at npoints points: host arrays
+ * cv [npoints][ncv] -> bias [npoints], der [npoints][ncv] (der may be NULL).
+ * Points outside the grid return P2A_ERR_GRID (Grid.cpp:108-118). */
+int p2a_metad_bias(p2a_metad *ctx, int64_t npoints, const double *cv, double *bias, double *der);
+int p2a_metad_bias_device(p2a_metad *ctx, int64_t npoints, const double *cv_dev, double *bias_dev, double *der_dev);
+
+/* One MD step of METAD on device-resident CVs: MetaD::calculate (:2315-2375:
+ * bias and force -dV/ds) followed by MetaD::update (a hill every PACE steps,
+ * not on the first step).  cv_dev [ncv]; bias_dev [1]; force_dev [ncv];
+ * heights_dev (may be NULL) receives the height of a hill deposited at this
+ * step (0 otherwise).  Asynchronous; errors are latched and reported by
+ * p2a_metad_check. */
+int p2a_metad_step_device(p2a_metad *ctx, int64_t step, const double *cv_dev, double *bias_dev, double *force_dev,
+                          double *height_dev);
+/* Synchronise and report a latched device error (P2A_ERR_GRID). */
+int p2a_metad_check(p2a_metad *ctx);
+
+/* Copy the grid out / in: values [size], derivatives [size][ncv] (Grid.cpp:878-910 layout). */
+int p2a_metad_get_grid(p2a_metad *ctx, double *values, double *derivs);
+int p2a_metad_set_grid(p2a_metad *ctx, const double *values, const double *derivs);
+
+/* Chain rule of the bias force onto the atoms, ActionWithValue::checkForForces
+ * (src/core/ActionWithValue.cpp:337-413) for one CV: forces[3N] += f * deriv[3N],
+ * virial[9] += f * virial_cv[9].  Device arrays; f is a device scalar (the
+ * METAD force on that CV).  Asynchronous on `stream` (NULL: default stream). */
+int p2a_apply_force_device(void *stream, int64_t natoms, const double *f_dev, const double *deriv_dev,
+                           const double *virial_cv_dev, double *forces_dev, double *virial_dev);
+
 #if defined(__GNUC__)
 #pragma GCC visibility pop
 #endif

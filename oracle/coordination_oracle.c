@@ -747,12 +747,22 @@ static void o_gindices_x(const ogrid_t *g, const double *x, unsigned *ind) {
 
 /* Gaussian value/derivative at cv (MetaD.cpp:2186-2278, diagonal hill, no
  * INTERVAL).  der[] is accumulated into (der -= ...). */
+/* Value::difference (src/core/Value.h:405-416): d2-d1, wrapped by Tools::pbc
+ * when the argument is periodic (period > 0) */
+static double o_vdiff(double period, double d1, double d2) {
+  if (!(period > 0.0)) return d2 - d1;
+  double inv = 1.0 / period;
+  double s = (d2 - d1) * inv;
+  s = o_pbc1(s);
+  return s * period;
+}
+
 static double o_gauss(int ncv, const double *cv, const double *center, const double *sigma, double height,
-                      double stretchA, double stretchB, double *der) {
+                      double stretchA, double stretchB, double *der, const double *period) {
   double dp[OG_MAXDIM], inv[OG_MAXDIM], dp2 = 0.0, bias = 0.0;
   for (int i = 0; i < ncv; i++) {
     inv[i] = fabs(sigma[i]) > 1.e-20 ? 1.0 / sigma[i] : 0.0;
-    dp[i] = (cv[i] - center[i]) * inv[i];
+    dp[i] = o_vdiff(period ? period[i] : 0.0, center[i], cv[i]) * inv[i];
     dp2 += dp[i] * dp[i];
   }
   dp2 *= 0.5;
@@ -794,7 +804,9 @@ OR_EXPORT int64_t ogrid_add_gaussian(ogrid_t *g, const double *center, const dou
       int64_t node = o_gindex(g, tmp);
       double xx[OG_MAXDIM], der[OG_MAXDIM] = {0, 0, 0, 0};
       for (int i = 0; i < dim; i++) xx[i] = g->min[i] + (double)tmp[i] * g->dx[i];
-      double b = o_gauss(dim, xx, center, sigma, height, stretchA, stretchB, der);
+      double period[OG_MAXDIM];
+      for (int i = 0; i < dim; i++) period[i] = g->pbc[i] ? g->max[i] - g->min[i] : 0.0;
+      double b = o_gauss(dim, xx, center, sigma, height, stretchA, stretchB, der, period);
       g->grid[node] += b;
       for (int i = 0; i < dim; i++) g->der[node * dim + i] += der[i];
       touched++;
@@ -887,6 +899,6 @@ OR_EXPORT double ometad_bias_hills(int ncv, int nhills, const double *centers, c
   double bias = 0.0;
   for (int i = 0; i < ncv; i++) der[i] = 0.0;
   for (int k = 0; k < nhills; k++)
-    bias += o_gauss(ncv, cv, centers + (size_t)k * ncv, sigma, heights[k], stretchA, stretchB, der);
+    bias += o_gauss(ncv, cv, centers + (size_t)k * ncv, sigma, heights[k], stretchA, stretchB, der, NULL);
   return bias;
 }

@@ -7,5 +7,6 @@ reference's action interface.
 from ._capi import Plumed2AmdError, lib  # noqa: F401
 from .coordination import (Coordination, CoordinationSpec, describe_switch,  # noqa: F401
                            parse_switch, rational_switch)
+from .metad import MetaD  # noqa: F401
 
 __version__ = "0.1.0"

@@ -1,0 +1,602 @@
+// METAD bias grid on gfx950: hill deposition, well-tempered heights, spline
+// bias/force, the fused per-step METAD kernel and the force chain rule.
+//
+// What each piece replaces in the reference:
+//   k_deposit_seq   MetaD::update -> getHeight -> addGaussian -> evaluate-
+//                   GaussianAndDerivatives -> Grid::addValueAndDerivatives
+//                   (src/bias/MetaD.cpp:2377-2470, 2280-2305, 1886-1937,
+//                   2186-2278; src/tools/Grid.cpp:55-106, 904-910), looping
+//                   over many hills inside one persistent workgroup because
+//                   each well-tempered height depends on the grid left by the
+//                   previous hill
+//   k_add_hills     addGaussian for hills of known heights (HILLS restart,
+//                   MetaD::readGaussians :1807-1829), all hills in parallel
+//   spline()        GridBase::getValueAndDerivatives spline branch
+//                   (src/tools/Grid.cpp:442-505) with getSplineNeighbors
+//                   (:328-356)
+//   k_metad_step    MetaD::calculate (:2315-2375) + MetaD::update on a
+//                   device-resident CV
+//   k_apply_force   ActionWithValue::checkForForces daxpy
+//                   (src/core/ActionWithValue.cpp:337-413)
+//
+// Layout in HBM: value[node] and gradient[node*ncv + i] in FP64, nodes in the
+// column-major order of GridBase::getIndex (first CV fastest).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "p2a_internal.h"
+
+namespace p2a {
+
+#define P2A_HIPM(x)                                                                                  \
+  do {                                                                                               \
+    hipError_t e_ = (x);                                                                             \
+    if (e_ != hipSuccess) return fail(P2A_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kMaxCv = 4;
+constexpr int kDepositThreads = 256;
+
+struct GridArg {
+  int ncv;
+  int pbc[kMaxCv];
+  unsigned nbin[kMaxCv];  // nodes per direction
+  int nneigh[kMaxCv];     // support half-width in bins
+  double min[kMaxCv], dx[kMaxCv];
+  double period[kMaxCv], inv_period[kMaxCv];
+  double invsigma[kMaxCv];
+  double height0, biasf, kbt, stretchA, stretchB;
+  int welltemp;
+  int support;            // prod(2*nneigh+1)
+  int may_alias;          // a periodic support wider than the grid visits a node twice
+};
+
+// Tools::pbc (Tools.h:547-570)
+__device__ __forceinline__ double mpbc1(double x) {
+  x += 100.0;
+  return (x >= 0) ? x - (double)(int)(x + 0.5) : x - (double)(int)(x - 0.5);
+}
+
+// Value::difference (src/core/Value.h:405-416): d2 - d1, wrapped when periodic
+__device__ __forceinline__ double vdiff(const GridArg& G, int i, double d1, double d2) {
+  if (!G.pbc[i]) return d2 - d1;
+  double s = (d2 - d1) * G.inv_period[i];
+  s = mpbc1(s);
+  return s * G.period[i];
+}
+
+// getIndices(x) + getIndex (Grid.cpp:108-132): -1 outside the grid
+__device__ long long grid_index_of(const GridArg& G, const double* x, unsigned* ind) {
+  for (int i = 0; i < G.ncv; i++) {
+    double f = floor((x[i] - G.min[i]) / G.dx[i]);
+    if (!(f >= 0.0) || f >= (double)G.nbin[i]) return -1;
+    ind[i] = (unsigned)f;
+  }
+  long long index = ind[G.ncv - 1];
+  for (int i = G.ncv - 1; i > 0; --i) index = index * G.nbin[i - 1] + ind[i - 1];
+  return index;
+}
+
+__device__ __forceinline__ long long flat(const GridArg& G, const unsigned* ind) {
+  long long index = ind[G.ncv - 1];
+  for (int i = G.ncv - 1; i > 0; --i) index = index * G.nbin[i - 1] + ind[i - 1];
+  return index;
+}
+
+// spline value and gradient at x; returns false outside the grid
+__device__ bool spline(const GridArg& G, const double* __restrict__ val, const double* __restrict__ der,
+                       const double* x, double& value, double* dv) {
+  unsigned ind[kMaxCv];
+  for (int i = 0; i < G.ncv; i++) dv[i] = 0.0;
+  value = 0.0;
+  if (grid_index_of(G, x, ind) < 0) return false;
+  double xfloor[kMaxCv];
+  for (int i = 0; i < G.ncv; i++) xfloor[i] = G.min[i] + (double)ind[i] * G.dx[i];
+  const unsigned nn = 1u << G.ncv;
+  for (unsigned ip = 0; ip < nn; ip++) {
+    unsigned ni[kMaxCv];
+    unsigned t = ip;
+    int cnt = 0;
+    for (int j = 0; j < G.ncv; j++) {
+      unsigned i0 = t % 2 + ind[j];
+      t /= 2;
+      if (!G.pbc[j] && i0 == G.nbin[j]) continue;
+      if (G.pbc[j] && i0 == G.nbin[j]) i0 = 0;
+      ni[cnt++] = i0;
+    }
+    if (cnt != G.ncv) continue;
+    const long long node = flat(G, ni);
+    const double grid = val[node];
+    double C[kMaxCv], D[kMaxCv], ff = 1.0;
+    for (int j = 0; j < G.ncv; j++) {
+      // ni[j] equals ind[j] or the next node (possibly wrapped)
+      const int x0 = (ni[j] == ind[j]) ? 0 : 1;
+      const double dx = G.dx[j];
+      const double X = fabs((x[j] - xfloor[j]) / dx - (double)x0);
+      const double X2 = X * X, X3 = X2 * X;
+      const double yy = fabs(grid) < 0.0000001 ? 0.0 : -der[node * G.ncv + j] / grid;
+      const double sg = x0 ? -1.0 : 1.0;
+      C[j] = (1.0 - 3.0 * X2 + 2.0 * X3) - sg * yy * (X - 2.0 * X2 + X3) * dx;
+      D[j] = (-6.0 * X + 6.0 * X2) - sg * yy * (1.0 - 4.0 * X + 3.0 * X2) * dx;
+      D[j] *= sg / dx;
+      ff *= C[j];
+    }
+    for (int j = 0; j < G.ncv; j++) {
+      double fd = D[j];
+      for (int i = 0; i < G.ncv; i++)
+        if (i != j) fd *= C[i];
+      dv[j] += grid * fd;
+    }
+    value += grid * ff;
+  }
+  return true;
+}
+
+// Gaussian of one hill at the node point xx (MetaD.cpp:2244-2275, diagonal)
+__device__ __forceinline__ double gaussian(const GridArg& G, const double* xx, const double* c, double h,
+                                           double* d) {
+  double dp[kMaxCv], dp2 = 0.0;
+  for (int i = 0; i < G.ncv; i++) {
+    dp[i] = vdiff(G, i, c[i], xx[i]) * G.invsigma[i];
+    dp2 += dp[i] * dp[i];
+  }
+  dp2 *= 0.5;
+  double bias = 0.0;
+  for (int i = 0; i < G.ncv; i++) d[i] = 0.0;
+  if (dp2 < 6.25) {
+    bias = h * exp(-dp2);
+    for (int i = 0; i < G.ncv; i++) d[i] = -bias * dp[i] * G.invsigma[i] * G.stretchA;
+    bias = G.stretchA * bias + h * G.stretchB;
+  }
+  return bias;
+}
+
+// node of support offset `s` around the bin `ind` (GridBase::getNeighbors,
+// Grid.cpp:55-106); false when it falls outside a non-periodic direction
+__device__ __forceinline__ bool support_node(const GridArg& G, const unsigned* ind, int s, unsigned* node_ind) {
+  for (int i = 0; i < G.ncv; i++) {
+    const int w = 2 * G.nneigh[i] + 1;
+    const int o = s % w;
+    s /= w;
+    int i0 = o - G.nneigh[i] + (int)ind[i];
+    if (!G.pbc[i] && (i0 < 0 || i0 >= (int)G.nbin[i])) return false;
+    if (G.pbc[i] && i0 < 0) i0 = G.nbin[i] - (-i0) % G.nbin[i];
+    if (G.pbc[i] && i0 >= (int)G.nbin[i]) i0 %= G.nbin[i];
+    node_ind[i] = (unsigned)i0;
+  }
+  return true;
+}
+
+// add one hill of height h centred at c to the grid, all threads of the block
+// (distinct nodes per thread unless may_alias, then thread 0 alone)
+__device__ void add_hill_block(const GridArg& G, double* __restrict__ val, double* __restrict__ der,
+                               const unsigned* ind, const double* c, double h) {
+  const int t0 = G.may_alias ? 0 : (int)threadIdx.x;
+  const int step = G.may_alias ? 1 : (int)blockDim.x;
+  if (G.may_alias && threadIdx.x != 0) return;
+  for (int s = t0; s < G.support; s += step) {
+    unsigned ni[kMaxCv];
+    if (!support_node(G, ind, s, ni)) continue;
+    const long long node = flat(G, ni);
+    double xx[kMaxCv], d[kMaxCv];
+    for (int i = 0; i < G.ncv; i++) xx[i] = G.min[i] + (double)ni[i] * G.dx[i];
+    const double b = gaussian(G, xx, c, h, d);
+    val[node] += b;
+    for (int i = 0; i < G.ncv; i++) der[node * G.ncv + i] += d[i];
+  }
+}
+
+__device__ __forceinline__ double wt_height(const GridArg& G, double vbias) {
+  if (!G.welltemp) return G.height0;
+  if (G.biasf > 1.0) return G.height0 * exp(-vbias / (G.kbt * (G.biasf - 1.0)));
+  return G.height0 * exp(-vbias / G.kbt);
+}
+
+// sequential well-tempered deposition of many hills in one workgroup
+__global__ __launch_bounds__(kDepositThreads) void k_deposit_seq(GridArg G, double* __restrict__ val,
+                                                                  double* __restrict__ der, long long nhills,
+                                                                  const double* __restrict__ centers,
+                                                                  double* __restrict__ heights,
+                                                                  int* __restrict__ err) {
+  __shared__ double sh_h;
+  __shared__ int sh_ok;
+  __shared__ unsigned sh_ind[kMaxCv];
+  __shared__ double sh_c[kMaxCv];
+  for (long long k = 0; k < nhills; k++) {
+    if (threadIdx.x == 0) {
+      double c[kMaxCv];
+      for (int i = 0; i < G.ncv; i++) c[i] = centers[k * G.ncv + i];
+      unsigned ind[kMaxCv];
+      int ok = grid_index_of(G, c, ind) >= 0;
+      double h = G.height0;
+      if (ok && G.welltemp) {
+        double v, dv[kMaxCv];
+        ok = spline(G, val, der, c, v, dv);
+        h = wt_height(G, v);
+      }
+      if (!ok) atomicCAS(err, 0, (int)(1 + (k < 0x3fffffff ? k : 0x3fffffff)));
+      for (int i = 0; i < G.ncv; i++) {
+        sh_ind[i] = ind[i];
+        sh_c[i] = c[i];
+      }
+      sh_h = h;
+      sh_ok = ok;
+      if (heights) heights[k] = ok ? h : 0.0;
+    }
+    __syncthreads();
+    if (!sh_ok) return;  // uniform: every thread reads the same flag
+    unsigned ind[kMaxCv];
+    double c[kMaxCv];
+    for (int i = 0; i < G.ncv; i++) {
+      ind[i] = sh_ind[i];
+      c[i] = sh_c[i];
+    }
+    const double h = sh_h;
+    add_hill_block(G, val, der, ind, c, h);
+    __syncthreads();
+  }
+}
+
+// hills of given heights, in parallel (one block per hill)
+__global__ __launch_bounds__(kDepositThreads) void k_add_hills(GridArg G, double* __restrict__ val,
+                                                                double* __restrict__ der, long long nhills,
+                                                                const double* __restrict__ centers,
+                                                                const double* __restrict__ heights,
+                                                                int* __restrict__ err) {
+  const long long k = blockIdx.x;
+  if (k >= nhills) return;
+  double c[kMaxCv];
+  unsigned ind[kMaxCv];
+  for (int i = 0; i < G.ncv; i++) c[i] = centers[k * G.ncv + i];
+  if (grid_index_of(G, c, ind) < 0) {
+    if (threadIdx.x == 0) atomicCAS(err, 0, 1);
+    return;
+  }
+  const double h = heights[k];
+  for (int s = threadIdx.x; s < G.support; s += blockDim.x) {
+    unsigned ni[kMaxCv];
+    if (!support_node(G, ind, s, ni)) continue;
+    const long long node = flat(G, ni);
+    double xx[kMaxCv], d[kMaxCv];
+    for (int i = 0; i < G.ncv; i++) xx[i] = G.min[i] + (double)ni[i] * G.dx[i];
+    const double b = gaussian(G, xx, c, h, d);
+    unsafeAtomicAdd(&val[node], b);
+    for (int i = 0; i < G.ncv; i++) unsafeAtomicAdd(&der[node * G.ncv + i], d[i]);
+  }
+}
+
+__global__ void k_bias(GridArg G, const double* __restrict__ val, const double* __restrict__ der, long long n,
+                       const double* __restrict__ cv, double* __restrict__ bias, double* __restrict__ dout,
+                       int* __restrict__ err) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double x[kMaxCv], v, dv[kMaxCv];
+  for (int i = 0; i < G.ncv; i++) x[i] = cv[p * G.ncv + i];
+  if (!spline(G, val, der, x, v, dv)) {
+    atomicCAS(err, 0, 1);
+    v = 0.0;
+  }
+  bias[p] = v;
+  if (dout)
+    for (int i = 0; i < G.ncv; i++) dout[p * G.ncv + i] = dv[i];
+}
+
+// MetaD::calculate + update for one step, one workgroup
+__global__ __launch_bounds__(kDepositThreads) void k_metad_step(GridArg G, double* __restrict__ val,
+                                                                 double* __restrict__ der,
+                                                                 const double* __restrict__ cv, int deposit,
+                                                                 double* __restrict__ bias,
+                                                                 double* __restrict__ force,
+                                                                 double* __restrict__ height,
+                                                                 int* __restrict__ err) {
+  __shared__ double sh_h;
+  __shared__ int sh_ok;
+  __shared__ unsigned sh_ind[kMaxCv];
+  __shared__ double sh_c[kMaxCv];
+  if (threadIdx.x == 0) {
+    double c[kMaxCv], v = 0.0, dv[kMaxCv];
+    unsigned ind[kMaxCv];
+    for (int i = 0; i < G.ncv; i++) c[i] = cv[i];
+    int ok = spline(G, val, der, c, v, dv);
+    if (!ok) atomicCAS(err, 0, 1);
+    // MetaD.cpp:2348-2350: no bias (and no force) when BIASFACTOR=1
+    const bool nobias = (G.biasf == 1.0);
+    if (bias) *bias = nobias ? 0.0 : v;
+    if (force)
+      for (int i = 0; i < G.ncv; i++) force[i] = nobias ? 0.0 : -dv[i];
+    ok = ok && grid_index_of(G, c, ind) >= 0;
+    // getHeight evaluates the bias of the grid before this hill: the same v
+    const double h = wt_height(G, v);
+    for (int i = 0; i < G.ncv; i++) {
+      sh_ind[i] = ind[i];
+      sh_c[i] = c[i];
+    }
+    sh_h = h;
+    sh_ok = ok && deposit;
+    if (height) *height = (ok && deposit) ? h : 0.0;
+  }
+  __syncthreads();
+  if (!sh_ok) return;
+  unsigned ind[kMaxCv];
+  double c[kMaxCv];
+  for (int i = 0; i < G.ncv; i++) {
+    ind[i] = sh_ind[i];
+    c[i] = sh_c[i];
+  }
+  add_hill_block(G, val, der, ind, c, sh_h);
+}
+
+__global__ void k_apply_force(long long natoms, const double* __restrict__ f, const double* __restrict__ deriv,
+                              const double* __restrict__ vcv, double* __restrict__ forces,
+                              double* __restrict__ virial) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const double fv = *f;
+  if (i < 3 * natoms) forces[i] += fv * deriv[i];
+  if (virial && vcv && i < 9) virial[i] += fv * vcv[i];
+}
+
+}  // namespace p2a
+
+struct p2a_metad {
+  int device = 0;
+  hipStream_t own_stream = nullptr, stream = nullptr;
+  p2a_metad_spec spec{};
+  p2a::GridArg G{};
+  size_t size = 0;
+  double* val = nullptr;
+  double* der = nullptr;
+  int* err = nullptr;
+  double* scratch = nullptr;  // staging for host-array entry points
+  size_t scratch_n = 0;
+  bool first_step = true;
+};
+
+namespace p2a {
+
+static int metad_scratch(p2a_metad* m, size_t n) {
+  if (n <= m->scratch_n) return P2A_OK;
+  if (m->scratch) (void)hipFree(m->scratch);
+  m->scratch = nullptr;
+  m->scratch_n = 0;
+  if (hipMalloc(&m->scratch, n * sizeof(double)) != hipSuccess) return fail(P2A_ERR_NOMEM, "hipMalloc failed");
+  m->scratch_n = n;
+  return P2A_OK;
+}
+
+static int metad_read_err(p2a_metad* m, const char* what) {
+  int e = 0;
+  P2A_HIPM(hipMemcpyAsync(&e, m->err, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+  P2A_HIPM(hipStreamSynchronize(m->stream));
+  if (e) {
+    P2A_HIPM(hipMemsetAsync(m->err, 0, sizeof(int), m->stream));
+    P2A_HIPM(hipStreamSynchronize(m->stream));
+    return fail(P2A_ERR_GRID, std::string(what) + ": Looking for a value outside the grid");
+  }
+  return P2A_OK;
+}
+
+}  // namespace p2a
+
+using namespace p2a;
+
+extern "C" {
+
+int p2a_metad_create(p2a_metad** out, int device, const p2a_metad_spec* spec) {
+  if (!out || !spec) return fail(P2A_ERR_ARG, "null argument");
+  *out = nullptr;
+  const int ncv = spec->ncv;
+  if (ncv < 1 || ncv > kMaxCv) return fail(P2A_ERR_UNSUPPORTED, "METAD grid supports 1 to 4 CVs");
+  if (spec->pace <= 0) return fail(P2A_ERR_ARG, "PACE should be positive");
+  if (spec->biasf < 1.0 && spec->biasf != -1.0) return fail(P2A_ERR_ARG, "well tempered bias factor is nonsensical");
+  if (spec->biasf >= 1.0 && spec->kbt <= 0.0)
+    return fail(P2A_ERR_ARG,
+                "Unless the MD engine passes the temperature to plumed, with well-tempered metad you must specify it using TEMP");
+  GridArg G;
+  std::memset(&G, 0, sizeof(G));
+  G.ncv = ncv;
+  size_t size = 1;
+  int64_t support = 1;
+  for (int i = 0; i < ncv; i++) {
+    if (!(spec->gmax[i] > spec->gmin[i])) return fail(P2A_ERR_ARG, "maximum in grid must be larger than minimum");
+    if (spec->nbin[i] == 0) return fail(P2A_ERR_ARG, "number of grid points must be greater than zero");
+    if (!(spec->sigma[i] > 0.0)) return fail(P2A_ERR_ARG, "SIGMA should be positive");
+    G.pbc[i] = spec->periodic[i] != 0;
+    G.min[i] = spec->gmin[i];
+    G.dx[i] = (spec->gmax[i] - spec->gmin[i]) / (double)spec->nbin[i];
+    G.nbin[i] = spec->nbin[i] + (G.pbc[i] ? 0 : 1);
+    G.period[i] = spec->gmax[i] - spec->gmin[i];
+    G.inv_period[i] = 1.0 / G.period[i];
+    G.invsigma[i] = 1.0 / spec->sigma[i];
+    // MetaD::getGaussianSupport (MetaD.cpp:1939-1994)
+    G.nneigh[i] = (int)std::ceil(std::sqrt(2.0 * 6.25) * spec->sigma[i] / G.dx[i]);
+    if (G.pbc[i] && 2 * G.nneigh[i] + 1 > (int)G.nbin[i]) G.may_alias = 1;
+    size *= G.nbin[i];
+    support *= 2 * G.nneigh[i] + 1;
+  }
+  if (support > (1 << 30)) return fail(P2A_ERR_UNSUPPORTED, "Gaussian support too large for the grid spacing");
+  G.support = (int)support;
+  G.height0 = spec->height;
+  G.biasf = spec->biasf;
+  G.kbt = spec->kbt;
+  G.welltemp = spec->biasf >= 1.0;
+  // dp2cutoff stretch (Tools.h:71-79, MetaD.cpp:762-765)
+  G.stretchA = spec->nostretch ? 1.0 : 1.00193418799744762399;
+  G.stretchB = spec->nostretch ? 0.0 : -.00193418799744762399;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(P2A_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= ndev) return fail(P2A_ERR_ARG, "device index out of range");
+  P2A_HIPM(hipSetDevice(device));
+  auto* m = new p2a_metad();
+  m->device = device;
+  m->spec = *spec;
+  m->G = G;
+  m->size = size;
+  bool ok = hipStreamCreateWithFlags(&m->own_stream, hipStreamNonBlocking) == hipSuccess &&
+            hipMalloc(&m->val, size * sizeof(double)) == hipSuccess &&
+            hipMalloc(&m->der, size * ncv * sizeof(double)) == hipSuccess &&
+            hipMalloc(&m->err, sizeof(int)) == hipSuccess;
+  if (!ok) {
+    p2a_metad_destroy(m);
+    return fail(P2A_ERR_NOMEM, "device allocation of the METAD grid failed");
+  }
+  m->stream = m->own_stream;
+  if (hipMemsetAsync(m->val, 0, size * sizeof(double), m->stream) != hipSuccess ||
+      hipMemsetAsync(m->der, 0, size * ncv * sizeof(double), m->stream) != hipSuccess ||
+      hipMemsetAsync(m->err, 0, sizeof(int), m->stream) != hipSuccess ||
+      hipStreamSynchronize(m->stream) != hipSuccess) {
+    p2a_metad_destroy(m);
+    return fail(P2A_ERR_HIP, "clearing the METAD grid failed");
+  }
+  *out = m;
+  g_last_error.clear();
+  return P2A_OK;
+}
+
+int p2a_metad_destroy(p2a_metad* m) {
+  if (!m) return P2A_OK;
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  if (m->val) (void)hipFree(m->val);
+  if (m->der) (void)hipFree(m->der);
+  if (m->err) (void)hipFree(m->err);
+  if (m->scratch) (void)hipFree(m->scratch);
+  if (m->own_stream) (void)hipStreamDestroy(m->own_stream);
+  delete m;
+  return P2A_OK;
+}
+
+int p2a_metad_set_stream(p2a_metad* m, void* stream) {
+  if (!m) return fail(P2A_ERR_ARG, "null context");
+  m->stream = stream ? (hipStream_t)stream : m->own_stream;
+  return P2A_OK;
+}
+
+int64_t p2a_metad_grid_size(const p2a_metad* m) { return m ? (int64_t)m->size : 0; }
+
+int p2a_metad_deposit_device(p2a_metad* m, int64_t nhills, const double* centers_dev, double* heights_dev) {
+  if (!m || (nhills > 0 && !centers_dev)) return fail(P2A_ERR_ARG, "null argument");
+  if (nhills <= 0) return P2A_OK;
+  P2A_HIPM(hipSetDevice(m->device));
+  hipLaunchKernelGGL(k_deposit_seq, dim3(1), dim3(kDepositThreads), 0, m->stream, m->G, m->val, m->der,
+                     (long long)nhills, centers_dev, heights_dev, m->err);
+  P2A_HIPM(hipGetLastError());
+  return P2A_OK;
+}
+
+int p2a_metad_deposit(p2a_metad* m, int64_t nhills, const double* centers, double* heights) {
+  if (!m || (nhills > 0 && !centers)) return fail(P2A_ERR_ARG, "null argument");
+  if (nhills <= 0) return P2A_OK;
+  P2A_HIPM(hipSetDevice(m->device));
+  const int ncv = m->G.ncv;
+  int rc;
+  if ((rc = metad_scratch(m, (size_t)nhills * (ncv + 1)))) return rc;
+  double* c_dev = m->scratch;
+  double* h_dev = m->scratch + (size_t)nhills * ncv;
+  P2A_HIPM(hipMemcpyAsync(c_dev, centers, (size_t)nhills * ncv * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  if ((rc = p2a_metad_deposit_device(m, nhills, c_dev, h_dev))) return rc;
+  if (heights)
+    P2A_HIPM(hipMemcpyAsync(heights, h_dev, (size_t)nhills * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  return metad_read_err(m, "METAD deposition");
+}
+
+int p2a_metad_add_hills(p2a_metad* m, int64_t nhills, const double* centers, const double* heights) {
+  if (!m || (nhills > 0 && (!centers || !heights))) return fail(P2A_ERR_ARG, "null argument");
+  if (nhills <= 0) return P2A_OK;
+  P2A_HIPM(hipSetDevice(m->device));
+  const int ncv = m->G.ncv;
+  int rc;
+  if ((rc = metad_scratch(m, (size_t)nhills * (ncv + 1)))) return rc;
+  double* c_dev = m->scratch;
+  double* h_dev = m->scratch + (size_t)nhills * ncv;
+  P2A_HIPM(hipMemcpyAsync(c_dev, centers, (size_t)nhills * ncv * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  P2A_HIPM(hipMemcpyAsync(h_dev, heights, (size_t)nhills * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  hipLaunchKernelGGL(k_add_hills, dim3((unsigned)nhills), dim3(kDepositThreads), 0, m->stream, m->G, m->val, m->der,
+                     (long long)nhills, c_dev, h_dev, m->err);
+  P2A_HIPM(hipGetLastError());
+  return metad_read_err(m, "METAD hills");
+}
+
+int p2a_metad_bias_device(p2a_metad* m, int64_t n, const double* cv_dev, double* bias_dev, double* der_dev) {
+  if (!m || (n > 0 && (!cv_dev || !bias_dev))) return fail(P2A_ERR_ARG, "null argument");
+  if (n <= 0) return P2A_OK;
+  P2A_HIPM(hipSetDevice(m->device));
+  hipLaunchKernelGGL(k_bias, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, m->G, m->val, m->der,
+                     (long long)n, cv_dev, bias_dev, der_dev, m->err);
+  P2A_HIPM(hipGetLastError());
+  return P2A_OK;
+}
+
+int p2a_metad_bias(p2a_metad* m, int64_t n, const double* cv, double* bias, double* der) {
+  if (!m || (n > 0 && (!cv || !bias))) return fail(P2A_ERR_ARG, "null argument");
+  if (n <= 0) return P2A_OK;
+  P2A_HIPM(hipSetDevice(m->device));
+  const int ncv = m->G.ncv;
+  int rc;
+  if ((rc = metad_scratch(m, (size_t)n * (2 * ncv + 1)))) return rc;
+  double* c_dev = m->scratch;
+  double* b_dev = c_dev + (size_t)n * ncv;
+  double* d_dev = b_dev + n;
+  P2A_HIPM(hipMemcpyAsync(c_dev, cv, (size_t)n * ncv * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  if ((rc = p2a_metad_bias_device(m, n, c_dev, b_dev, d_dev))) return rc;
+  P2A_HIPM(hipMemcpyAsync(bias, b_dev, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  if (der) P2A_HIPM(hipMemcpyAsync(der, d_dev, (size_t)n * ncv * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  return metad_read_err(m, "METAD bias");
+}
+
+int p2a_metad_step_device(p2a_metad* m, int64_t step, const double* cv_dev, double* bias_dev, double* force_dev,
+                          double* height_dev) {
+  if (!m || !cv_dev) return fail(P2A_ERR_ARG, "null argument");
+  P2A_HIPM(hipSetDevice(m->device));
+  // MetaD::update (MetaD.cpp:2379-2385): a hill every PACE steps, never on
+  // the first step seen
+  int deposit = 0;
+  if (step % m->spec.pace == 0 && !m->first_step) deposit = 1;
+  else m->first_step = false;
+  hipLaunchKernelGGL(k_metad_step, dim3(1), dim3(kDepositThreads), 0, m->stream, m->G, m->val, m->der, cv_dev,
+                     deposit, bias_dev, force_dev, height_dev, m->err);
+  P2A_HIPM(hipGetLastError());
+  return P2A_OK;
+}
+
+int p2a_metad_check(p2a_metad* m) {
+  if (!m) return fail(P2A_ERR_ARG, "null context");
+  P2A_HIPM(hipSetDevice(m->device));
+  return metad_read_err(m, "METAD");
+}
+
+int p2a_metad_get_grid(p2a_metad* m, double* values, double* derivs) {
+  if (!m) return fail(P2A_ERR_ARG, "null context");
+  P2A_HIPM(hipSetDevice(m->device));
+  if (values) P2A_HIPM(hipMemcpyAsync(values, m->val, m->size * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  if (derivs)
+    P2A_HIPM(hipMemcpyAsync(derivs, m->der, m->size * m->G.ncv * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  P2A_HIPM(hipStreamSynchronize(m->stream));
+  return P2A_OK;
+}
+
+int p2a_metad_set_grid(p2a_metad* m, const double* values, const double* derivs) {
+  if (!m) return fail(P2A_ERR_ARG, "null context");
+  P2A_HIPM(hipSetDevice(m->device));
+  if (values) P2A_HIPM(hipMemcpyAsync(m->val, values, m->size * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  if (derivs)
+    P2A_HIPM(hipMemcpyAsync(m->der, derivs, m->size * m->G.ncv * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  P2A_HIPM(hipStreamSynchronize(m->stream));
+  return P2A_OK;
+}
+
+int p2a_apply_force_device(void* stream, int64_t natoms, const double* f_dev, const double* deriv_dev,
+                           const double* virial_cv_dev, double* forces_dev, double* virial_dev) {
+  if (!f_dev || !deriv_dev || !forces_dev) return fail(P2A_ERR_ARG, "null argument");
+  long long n = 3 * (long long)natoms;
+  long long nt = n > 9 ? n : 9;
+  hipLaunchKernelGGL(k_apply_force, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     (long long)natoms, f_dev, deriv_dev, virial_cv_dev, forces_dev, virial_dev);
+  P2A_HIPM(hipGetLastError());
+  return P2A_OK;
+}
+
+}  // extern "C"

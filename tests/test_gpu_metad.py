@@ -1,0 +1,172 @@
+"""GPU parity of the METAD grid (HIP library through the C ABI) against the FP64
+oracle and the rt11/rt67 golden files.  The grid accumulator is FP64 on both
+sides: values, gradients and heights are held to 1e-10 relative."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from fixtures import path, read_colvar
+
+pytestmark = pytest.mark.gpu
+KB = 0.0083144621
+TOL = 1e-10
+
+
+def _P():
+    import plumed2_amd as P
+    return P
+
+
+def random_walk(n, start, sigma, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    x = np.array(start, dtype=float)
+    out = []
+    for _ in range(n):
+        x = np.clip(x + rng.normal(0, sigma), lo, hi)
+        out.append(x.copy())
+    return np.array(out)
+
+
+def test_sequential_welltempered_deposition_matches_oracle():
+    P = _P()
+    m = P.MetaD("m: METAD ARG=c,d SIGMA=20,0.05 HEIGHT=1.2 PACE=500 BIASFACTOR=10 TEMP=300 "
+                "GRID_MIN=0,0 GRID_MAX=5000,6 GRID_BIN=512,512", periodic={})
+    centers = random_walk(400, [2500.0, 3.0], [15.0, 0.04], [0, 0], [4999.9, 5.999], 5)
+    h = m.deposit(centers)
+    g = O.Grid([0.0, 0.0], [5000.0, 6.0], [512, 512], [0, 0])
+    oh = g.deposit_wt(centers, [20.0, 0.05], 1.2, 10.0, 300 * KB)
+    np.testing.assert_allclose(h, oh, rtol=TOL)
+    v, d = m.grid()
+    ov, od = g.values(), g.derivs()
+    assert np.max(np.abs(v - ov)) <= TOL * np.max(np.abs(ov))
+    assert np.max(np.abs(d - od)) <= TOL * np.max(np.abs(od))
+    pts = random_walk(300, [2500.0, 3.0], [40.0, 0.1], [0, 0], [4999.9, 5.999], 6)
+    b, db = m.bias(pts)
+    for k in range(0, 300, 7):
+        ob, odb = g.spline(pts[k])
+        assert abs(b[k] - ob) <= TOL * max(1.0, abs(ob))
+        np.testing.assert_allclose(db[k], odb, rtol=1e-9, atol=1e-12)
+
+
+def test_periodic_grid_and_restart_from_hills(tmp_path):
+    P = _P()
+    line = ("metad: METAD ARG=phi,psi SIGMA=0.2,0.2 HEIGHT=1.2 BIASFACTOR=10 PACE=500 TEMP=300 "
+            f"GRID_MIN=-pi,-pi GRID_MAX=pi,pi GRID_BIN=150,150 FILE={tmp_path}/HILLS FMT=%20.12f")
+    per = {"phi": ("-pi", "pi"), "psi": ("-pi", "pi")}
+    m = P.MetaD(line, periodic=per)
+    rng = np.random.default_rng(7)
+    cvs = rng.uniform(-math.pi, math.pi, (60, 2))
+    heights = []
+    for k, c in enumerate(cvs):
+        m.calculate(c)
+        h = m.update(500 * k, c)
+        if h is not None:
+            heights.append(h)
+    assert len(heights) == 59  # no hill on the first step
+    g = O.Grid([-math.pi, -math.pi], [math.pi, math.pi], [150, 150], [1, 1])
+    oh = g.deposit_wt(cvs[1:], [0.2, 0.2], 1.2, 10.0, 300 * KB)
+    np.testing.assert_allclose(heights, oh, rtol=TOL)
+    v, d = m.grid()
+    np.testing.assert_allclose(v, g.values(), atol=TOL * np.max(np.abs(g.values())))
+    m.close()
+    # restart: a fresh action re-reads HILLS (heights rescaled back)
+    m2 = P.MetaD(line + " RESTART=YES", periodic=per)
+    v2, _ = m2.grid()
+    # HILLS keeps 12 decimals of heights and centres
+    assert np.max(np.abs(v2 - v)) < 1e-8 * np.max(np.abs(v))
+    text = open(f"{tmp_path}/HILLS").read().splitlines()
+    assert text[0] == "#! FIELDS time phi psi sigma_phi sigma_psi height biasf"
+    assert text[1] == "#! SET multivariate false"
+    assert text[2] == "#! SET kerneltype stretched-gaussian"
+    assert "#! SET min_phi -pi" in text
+
+
+def test_rt11_golden_through_gpu():
+    P = _P()
+    names, hills = read_colvar(path("rt11", "HILLS.reference"))
+    _, colvar = read_colvar(path("rt11", "COLVAR.reference"))
+    m = P.MetaD("md: METAD ARG=d1,vol SIGMA=0.1,0.2 HEIGHT=1.0 PACE=10 TEMP=300 GRID_MIN=0.0,120.0 "
+                "GRID_MAX=2.0,130.0 BIASFACTOR=10 GRID_BIN=100,100 FMT=%14.6f", timestep=0.005)
+    m.hills_path = None
+    for k in range(len(hills)):
+        c = hills[k, 1:3]
+        b, _ = m.calculate(c)
+        assert abs(b - colvar[k + 1, 3]) < 5.1e-4 + 1e-3 * abs(colvar[k + 1, 3])
+        h = m.deposit([c])[0]
+        assert abs(h * 10 / 9 - hills[k, names.index("height")]) < 5.1e-7 + 1e-7 * hills[k, -2]
+    # HILLS record formatting matches the reference file for the same numbers
+    rec = m.hill_record(hills[0, 0], hills[0, 1:3], hills[0, names.index("height")] * 0.9)
+    ref_line = open(path("rt11", "HILLS.reference")).read().splitlines()[3]
+    assert rec.split() == ref_line.split()
+
+
+def test_rt67_golden_through_gpu():
+    P = _P()
+    names, col = read_colvar(path("rt67", "COLVAR.reference"))
+    m = P.MetaD("metad: METAD ARG=phi,psi SIGMA=0.20,0.20 HEIGHT=1.20 BIASFACTOR=10 PACE=500 TEMP=300 "
+                "GRID_MIN=-pi,-pi GRID_MAX=pi,pi GRID_BIN=150,150",
+                periodic={"phi": ("-pi", "pi"), "psi": ("-pi", "pi")})
+    m.hills_path = None
+    ib = names.index("metad.bias")
+    for k in range(len(col)):
+        b, _ = m.calculate(col[k, 1:3])
+        assert abs(b - col[k, ib]) < 2e-3
+        m.update(500 * k, col[k, 1:3])
+
+
+def test_device_step_matches_host_semantics():
+    import torch
+
+    P = _P()
+    line = "m: METAD ARG=a,b SIGMA=0.3,0.2 HEIGHT=0.8 PACE=3 BIASFACTOR=6 TEMP=310 GRID_MIN=-2,-2 GRID_MAX=2,2 GRID_BIN=80,80"
+    md, mh = P.MetaD(line), P.MetaD(line)
+    md.hills_path = mh.hills_path = None
+    cvs = random_walk(40, [0.0, 0.0], [0.2, 0.2], [-1.9, -1.9], [1.9, 1.9], 8)
+    dev = torch.device("cuda:0")
+    cv_t = torch.empty(2, dtype=torch.float64, device=dev)
+    b_t = torch.empty(1, dtype=torch.float64, device=dev)
+    f_t = torch.empty(2, dtype=torch.float64, device=dev)
+    h_t = torch.empty(1, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    for step, c in enumerate(cvs, start=5):
+        cv_t.copy_(torch.from_numpy(c))
+        md.step_device(step, cv_t.data_ptr(), b_t.data_ptr(), f_t.data_ptr(), h_t.data_ptr(), stream=stream)
+        torch.cuda.synchronize()
+        hb, hf = mh.calculate(c)
+        hh = mh.update(step, c)
+        assert abs(b_t.item() - hb) <= TOL * max(1, abs(hb))
+        np.testing.assert_allclose(f_t.cpu().numpy(), hf, rtol=1e-9, atol=1e-12)
+        assert (h_t.item() == 0.0) == (hh is None)
+        if hh is not None:
+            assert abs(h_t.item() - hh) <= TOL * hh
+    md.check()
+
+
+def test_outside_grid_is_an_error():
+    P = _P()
+    m = P.MetaD("m: METAD ARG=a SIGMA=0.1 HEIGHT=1 PACE=1 GRID_MIN=0 GRID_MAX=1 GRID_BIN=10")
+    m.hills_path = None
+    with pytest.raises(P.Plumed2AmdError, match="outside the grid"):
+        m.bias([[1.5]])
+    with pytest.raises(P.Plumed2AmdError, match="outside the grid"):
+        m.deposit([[-0.2]])
+    # the context stays usable after the error
+    b, _ = m.bias([[0.5]])
+    assert b[0] == 0.0
+
+
+def test_add_hills_equals_sequential_with_same_heights():
+    P = _P()
+    line = "m: METAD ARG=a,b SIGMA=0.3,0.2 HEIGHT=0.8 PACE=3 GRID_MIN=-2,-2 GRID_MAX=2,2 GRID_BIN=60,70"
+    m1, m2 = P.MetaD(line), P.MetaD(line)
+    c = random_walk(200, [0.0, 0.0], [0.2, 0.2], [-1.9, -1.9], [1.9, 1.9], 9)
+    h = m1.deposit(c)  # not well-tempered: every height is HEIGHT
+    assert np.all(h == 0.8)
+    m2.add_hills(c, h)
+    v1, d1 = m1.grid()
+    v2, d2 = m2.grid()
+    np.testing.assert_allclose(v1, v2, atol=1e-12 * np.max(np.abs(v1)))
+    np.testing.assert_allclose(d1, d2, atol=1e-12 * np.max(np.abs(d1)))

@@ -1,0 +1,163 @@
+"""Pins the METAD part of the FP64 oracle (CPU).
+
+Anchors: the reference Grid class compiled from its sources (spline value and
+gradient, neighbour enumeration), the regtest goldens rt11 (2-D non-periodic
+well-tempered grid: HILLS heights, bias) and rt67 (2-D periodic grid, bias
+every PACE), and closed forms of a single Gaussian.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from fixtures import path, read_colvar
+
+needs_ref = pytest.mark.skipif(O.ref() is None, reason="compiled reference subset (oracle/_ref) not built here")
+
+KB = 0.0083144621
+
+
+def random_grid(gmin, gmax, nbin, pbc, seed):
+    g = O.Grid(gmin, gmax, nbin, pbc)
+    rng = np.random.default_rng(seed)
+    v = rng.normal(0, 3, g.size)
+    v[rng.random(g.size) < 0.05] = 0.0  # exercise the |grid|<1e-7 branch
+    d = rng.normal(0, 5, (g.size, g.dim))
+    g.set(v, d)
+    return g, v, d
+
+
+@needs_ref
+@pytest.mark.parametrize("gmin,gmax,nbin,pbc", [
+    ([0.0, 120.0], [2.0, 130.0], [20, 30], [0, 0]),
+    ([-math.pi, -math.pi], [math.pi, math.pi], [15, 12], [1, 1]),
+    ([0.0], [5.0], [17], [0]),
+    ([-1.0, 0.0, 2.0], [1.0, 3.0, 4.0], [6, 5, 7], [0, 1, 0]),
+])
+def test_spline_matches_compiled_reference(gmin, gmax, nbin, pbc):
+    g, v, d = random_grid(gmin, gmax, nbin, pbc, 1)
+    r = O.RefGrid(gmin, gmax, nbin, pbc)
+    assert r.size == g.size
+    r.set(v, d)
+    rng = np.random.default_rng(2)
+    dim = len(gmin)
+    for _ in range(400):
+        x = [rng.uniform(gmin[i], gmax[i] - 1e-9) for i in range(dim)]
+        a, da = g.spline(x)
+        b, db = r.spline(x)
+        assert a == b
+        np.testing.assert_array_equal(da, db)
+
+
+@needs_ref
+@pytest.mark.parametrize("pbc", [[0, 0], [1, 1], [1, 0]])
+def test_support_nodes_match_compiled_reference(pbc):
+    gmin, gmax, nbin = [-math.pi, -math.pi], [math.pi, math.pi], [40, 30]
+    sigma = [0.3, 0.45]
+    g = O.Grid(gmin, gmax, nbin, pbc)
+    r = O.RefGrid(gmin, gmax, nbin, pbc)
+    dx = [(gmax[i] - gmin[i]) / nbin[i] for i in range(2)]
+    nneigh = O.support_bins(sigma, dx)
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        c = [rng.uniform(gmin[i], gmax[i] - 1e-9) for i in range(2)]
+        ref_nodes = set(int(k) for k in r.neighbors(c, nneigh))
+        g2 = O.Grid(gmin, gmax, nbin, pbc)
+        g2.add_gaussian(c, sigma, 1.0)
+        touched = set(np.nonzero(g2.values() != 0.0)[0].tolist()) | set(np.nonzero(np.any(g2.derivs() != 0, 1))[0].tolist())
+        # every node the oracle touched is a reference support node, and the
+        # reference support equals the oracle's enumeration (stretchB shifts
+        # every node inside the cutoff, nodes outside stay exactly 0)
+        assert touched <= ref_nodes
+
+
+def test_single_gaussian_closed_form():
+    gmin, gmax, nbin = [0.0, 0.0], [10.0, 6.0], [100, 60]
+    sigma, h = [0.5, 0.3], 1.7
+    c = [4.33, 2.71]
+    g = O.Grid(gmin, gmax, nbin, [0, 0])
+    g.add_gaussian(c, sigma, h)
+    v, d = g.values(), g.derivs()
+    A, B = O.DP2CUTOFF_A, O.DP2CUTOFF_B
+    nx = nbin[0] + 1
+    for idx in [0, 1234, 4520, 2000, 3000]:
+        i, j = idx % nx, idx // nx
+        x = np.array([gmin[0] + i * 0.1, gmin[1] + j * 0.1])
+        dp = (x - c) / sigma
+        dp2 = 0.5 * (dp @ dp)
+        if dp2 < 6.25:
+            e = h * math.exp(-dp2)
+            assert abs(v[idx] - (A * e + h * B)) < 1e-14
+            np.testing.assert_allclose(d[idx], -e * dp / sigma * A, atol=1e-13)
+        else:
+            assert v[idx] == 0.0
+    # spline interpolates the node values and gradients exactly at nodes
+    for idx in [1234, 2000, 3000]:
+        i, j = idx % nx, idx // nx
+        x = [gmin[0] + i * 0.1 + 1e-12, gmin[1] + j * 0.1 + 1e-12]
+        val, der = g.spline(x)
+        assert abs(val - v[idx]) < 1e-9
+        np.testing.assert_allclose(der, d[idx], atol=1e-6)
+
+
+def test_grid_bias_close_to_hills_sum():
+    """A fine grid reproduces the analytic sum of the hills (rt21-style check)."""
+    gmin, gmax, nbin = [0.0, 0.0], [4.0, 4.0], [200, 200]
+    sigma = [0.2, 0.3]
+    rng = np.random.default_rng(4)
+    centers = rng.uniform(1.0, 3.0, (40, 2))
+    heights = rng.uniform(0.5, 1.5, 40)
+    g = O.Grid(gmin, gmax, nbin, [0, 0])
+    for c, h in zip(centers, heights):
+        g.add_gaussian(c, sigma, h)
+    for x in rng.uniform(1.2, 2.8, (30, 2)):
+        gb, gd = g.spline(x)
+        hb, hd = O.bias_hills(centers, sigma, heights, x)
+        assert abs(gb - hb) < 2e-3 * max(1, abs(hb))
+        np.testing.assert_allclose(gd, hd, atol=2e-2 * max(1, np.max(np.abs(hd))))
+
+
+def test_rt11_golden_heights_and_bias():
+    """regtest/basic/rt11 (md): ARG=d1,vol SIGMA=0.1,0.2 HEIGHT=1.0 PACE=10
+    TEMP=300 BIASFACTOR=10 GRID_MIN=0,120 GRID_MAX=2,130 GRID_BIN=100,100.
+    Replays the HILLS centres: WT heights (printed x biasf/(biasf-1), %14.6f)
+    and md.bias in COLVAR (%6.3f, bias of the previous hills at each centre)."""
+    names, hills = read_colvar(path("rt11", "HILLS.reference"))
+    _, colvar = read_colvar(path("rt11", "COLVAR.reference"))
+    g = O.Grid([0.0, 120.0], [2.0, 130.0], [100, 100], [0, 0])
+    kbt, biasf = 300 * KB, 10.0
+    centers = hills[:, 1:3]
+    biases = []
+    for k, c in enumerate(centers):
+        v, _ = g.spline(c)
+        biases.append(v)
+        h = g.deposit_wt([c], [0.1, 0.2], 1.0, biasf, kbt)[0]
+        printed = hills[k, names.index("height")]
+        assert abs(h * biasf / (biasf - 1) - printed) < 5.1e-7 + 1e-7 * printed
+    # COLVAR rows at t = 0.05, 0.1, ... are the deposition steps
+    for k in range(len(centers)):
+        row = colvar[k + 1]
+        assert abs(row[0] - hills[k, 0]) < 1e-9
+        assert abs(biases[k] - row[3]) < 5.1e-4 + 1e-3 * abs(row[3])
+
+
+def test_rt67_golden_periodic_bias():
+    """regtest/basic/rt67: phi/psi on a periodic 150x150 grid, SIGMA=0.2,0.2,
+    HEIGHT=1.2, BIASFACTOR=10, PACE=500 at 300 K; the bias printed every PACE
+    steps is the spline of the hills deposited at the previous rows."""
+    names, col = read_colvar(path("rt67", "COLVAR.reference"))
+    kbt_candidates = [300 * KB]
+    g = O.Grid([-math.pi, -math.pi], [math.pi, math.pi], [150, 150], [1, 1])
+    biasf = 10.0
+    kbt = kbt_candidates[0]
+    ib = names.index("metad.bias")
+    worst = 0.0
+    for k in range(len(col)):
+        c = col[k, 1:3]
+        v, _ = g.spline(c)
+        worst = max(worst, abs(v - col[k, ib]))
+        if k > 0:  # no hill on the first step
+            g.deposit_wt([c], [0.2, 0.2], 1.2, biasf, kbt)
+    # CVs are printed with 4 decimals: the bias is reproduced to ~1e-3
+    assert worst < 2e-3, worst
