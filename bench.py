@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -90,6 +91,134 @@ def cpu_baseline(x0, L, nthreads, budget_s=25.0):
             "cv_frame0": val}
 
 
+def _events(torch):
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def bench_autograd(P, torch, steps=10, warm=2):
+    """Differentiable COORDINATION: GROUPA=10,000, GROUPB=490,000 at 100 /nm^3,
+    RATIONAL R_0=0.35 NN=8 MM=16 D_MAX=1.0, float32 positions; one step = the
+    value-only forward plus the recomputing backward."""
+    from plumed2_amd.autograd import coordination
+
+    na, nb = 10_000, 490_000
+    n = na + nb
+    L = (n / 100.0) ** (1.0 / 3.0)
+    box = np.diag([L] * 3)
+    x = np.random.default_rng(3).uniform(0.0, L, (n, 3))
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{na} GROUPB={na + 1}-{n} "
+                         "SWITCH={RATIONAL R_0=0.35 NN=8 MM=16 D_MAX=1.0}")
+    xt = torch.tensor(x, dtype=torch.float32, device="cuda", requires_grad=True)
+    for s in range(warm):
+        coordination(xt, act, box, s).backward()
+    torch.cuda.synchronize()
+    st = act.stats()
+    pairs = st["contributing_pairs"]
+    a, b = _events(torch)
+    a.record()
+    for s in range(warm, warm + steps):
+        xt.grad = None
+        coordination(xt, act, box, s).backward()
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    act.close()
+    return {"workload": f"GROUPA={na} GROUPB={nb}, rho=100/nm^3, L={L:.4f} nm, RATIONAL R_0=0.35 NN=8 MM=16 "
+                        "D_MAX=1.0, float32 positions, torch.autograd forward+backward",
+            "ms_per_step": ms, "contributing_pairs_per_step": pairs,
+            "pair_evals_per_s": pairs / (ms / 1e3)}
+
+
+def bench_metad(P, torch, nhills=10_000, steps=2000):
+    """METAD on 2 CVs, 512x512 grid over [0,5000]x[0,6], SIGMA=20,0.05,
+    HEIGHT=1.2, BIASFACTOR=10, TEMP=300: 10,000 well-tempered hills at
+    random-walk centres (seed 5) deposited in one launch, then the per-step
+    bias+force+deposit kernel (PACE=500) on a device-resident CV."""
+    import oracle
+
+    line = ("m: METAD ARG=c,d SIGMA=20,0.05 HEIGHT=1.2 PACE=500 BIASFACTOR=10 TEMP=300 "
+            "GRID_MIN=0,0 GRID_MAX=5000,6 GRID_BIN=512,512")
+    rng = np.random.default_rng(5)
+    c = np.array([2500.0, 3.0])
+    centers = np.empty((nhills, 2))
+    for k in range(nhills):
+        c = np.clip(c + rng.normal(0.0, [20.0, 0.05]), [0.0, 0.0], [4999.999, 5.999999])
+        centers[k] = c
+    m = P.MetaD(line)
+    m.hills_path = None
+    cdev = torch.tensor(centers, dtype=torch.float64, device="cuda")
+    hdev = torch.empty(nhills, dtype=torch.float64, device="cuda")
+    import ctypes as C
+    from plumed2_amd import _capi as A
+    m.step_device  # noqa: B018  (binds the METAD symbols)
+    A.check(A.lib().p2a_metad_set_stream(m._ctx, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    a, b = _events(torch)
+    torch.cuda.synchronize()
+    a.record()
+    A.check(A.lib().p2a_metad_deposit_device(m._ctx, nhills, C.c_void_p(cdev.data_ptr()), C.c_void_p(hdev.data_ptr())))
+    b.record()
+    torch.cuda.synchronize()
+    m.check()
+    dep_ms = a.elapsed_time(b)
+    support = int(np.prod([2 * math.ceil(math.sqrt(12.5) * s / d) + 1 for s, d in ((20.0, 5000 / 512), (0.05, 6 / 512))]))
+    # per-step bias + force (+ a hill every PACE) on a device CV
+    bias = torch.empty(1, dtype=torch.float64, device="cuda")
+    force = torch.empty(2, dtype=torch.float64, device="cuda")
+    walk = torch.tensor(centers[np.arange(steps) % nhills], dtype=torch.float64, device="cuda")
+    a.record()
+    for s in range(steps):
+        m.step_device(s, walk[s].data_ptr(), bias.data_ptr(), force.data_ptr())
+    b.record()
+    torch.cuda.synchronize()
+    m.check()
+    step_us = a.elapsed_time(b) * 1e3 / steps
+    # FP64 host port of the same deposition, bounded sample
+    g = oracle.Grid([0.0, 0.0], [5000.0, 6.0], [512, 512], [0, 0])
+    t0 = time.perf_counter()
+    nref = 1000
+    g.deposit_wt(centers[:nref], [20.0, 0.05], 1.2, 10.0, 300 * 0.0083144621)
+    cpu_s = (time.perf_counter() - t0) / nref
+    m.close()
+    return {"workload": "METAD 2 CVs, 512x512 grid on [0,5000]x[0,6], SIGMA=20,0.05, HEIGHT=1.2, BIASFACTOR=10, "
+                        f"{nhills} well-tempered hills (random walk, seed 5)",
+            "deposit_ms_total": dep_ms, "hills_per_s": nhills / (dep_ms / 1e3),
+            "grid_node_updates_per_s": nhills * support / (dep_ms / 1e3), "support_nodes_per_hill": support,
+            "step_us_bias_force_deposit": step_us,
+            "cpu_port_hills_per_s": 1.0 / cpu_s, "cpu_port_sample": f"{nref} hills, FP64 oracle, 1 core"}
+
+
+def bench_coord_100k(P, torch, steps=200, warm=20):
+    """The COORDINATION CV of the METAD workload: 100,000 atoms as the 50k case."""
+    n = 100_000
+    L = (n / DENSITY) ** (1.0 / 3.0)
+    box = np.diag([L] * 3)
+    x0 = np.random.default_rng(5).uniform(0.0, L, (n, 3))
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    frames = torch.from_numpy(x0).to(dev) + torch.cumsum(
+        torch.randn((warm + steps, n, 3), dtype=torch.float64, device=dev, generator=g) * SIGMA, dim=0)
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{n} SWITCH={{{SWITCH}}} NLISTCELLS NL_CUTOFF={NL_CUTOFF} "
+                         f"NL_STRIDE={NL_STRIDE}")
+    out = (torch.empty(1, dtype=torch.float64, device=dev), torch.empty((n, 3), dtype=torch.float64, device=dev),
+           torch.empty((3, 3), dtype=torch.float64, device=dev))
+    for s in range(warm):
+        act.calculate_torch(frames[s].contiguous(), box, s, out)
+    torch.cuda.synchronize()
+    act.set_profiling(True)
+    a, b = _events(torch)
+    a.record()
+    for s in range(warm, warm + steps):
+        act.calculate_torch(frames[s].contiguous(), box, s, out)
+    b.record()
+    torch.cuda.synchronize()
+    st = act.stats()
+    ms = a.elapsed_time(b) / steps
+    act.close()
+    return {"workload": f"single-group COORDINATION, {n} atoms, rho={DENSITY}, L={L:.4f} nm, as the 50k case",
+            "ms_per_step": ms, "pair_evals_per_s": st["total_contributing_pairs"] / (ms * steps / 1e3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +226,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    ap.add_argument("--no-extras", action="store_true", help="skip the autograd / METAD / 100k measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
@@ -182,11 +312,17 @@ def main():
             "flop_per_pair": FLOP_PER_PAIR,
         },
     }
+    act.close()
+    del frames
+    torch.cuda.empty_cache()
+    if not args.no_extras:
+        result["extras"] = {"autograd_two_group_500k": bench_autograd(P, torch),
+                            "coordination_100k": bench_coord_100k(P, torch),
+                            "metad_grid": bench_metad(P, torch)}
     if not args.no_cpu:
         nthreads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         result["cpu_baseline"] = cpu_baseline(x0, L, nthreads)
     print(json.dumps(result))
-    act.close()
 
 
 if __name__ == "__main__":

@@ -137,8 +137,9 @@ typedef struct p2a_coord p2a_coord;
 int p2a_coord_create(p2a_coord **ctx, int device, const p2a_coord_spec *spec);
 int p2a_coord_destroy(p2a_coord *ctx);
 
-/* Use the caller's HIP stream (hipStream_t passed as void*); NULL restores the
- * context's own stream. */
+/* Run the context's work on the caller's HIP stream (hipStream_t passed as
+ * void*); NULL selects the default (null) stream.  Until this is called the
+ * context uses a non-blocking stream of its own. */
 int p2a_coord_set_stream(p2a_coord *ctx, void *stream);
 
 /* One step of CoordinationBase::prepare + calculate
@@ -151,9 +152,20 @@ int p2a_coord_calculate(p2a_coord *ctx, int64_t step, const double *box9, const 
 
 /* Same with device-resident positions and outputs (value: 1 double, deriv:
  * 3*natoms doubles, virial: 9 doubles, all device pointers; any may be NULL).
- * box9 is a host array.  Asynchronous on the context's stream. */
+ * With deriv_dev = virial_dev = NULL only the value is evaluated (a cheaper
+ * kernel that visits each pair once).  box9 is a host array.  Asynchronous on
+ * the context's stream. */
 int p2a_coord_calculate_device(p2a_coord *ctx, int64_t step, const double *box9, const double *pos_dev,
                                double *value_dev, double *deriv_dev, double *virial_dev);
+
+/* Backward pass: derivatives (and the virial if virial_dev is not NULL) of
+ * the value of the last calculate_device call, recomputing the pair terms on
+ * the neighbour structure that call built (nothing is stored per pair).
+ * Positions must be those of that call.  Replaces the derivative part of
+ * CoordinationBase::calculate (CoordinationBase.cpp:196-206) when the forward
+ * asked for the value only (deriv_dev = virial_dev = NULL).  Asynchronous. */
+int p2a_coord_backward_device(p2a_coord *ctx, const double *box9, const double *pos_dev, double *deriv_dev,
+                              double *virial_dev);
 
 typedef struct p2a_coord_stats {
   int64_t natoms;

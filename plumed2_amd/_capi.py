@@ -66,6 +66,7 @@ _SIGS = {
                             C.c_int),
     "p2a_coord_calculate_device": ([C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p], C.c_int),
+    "p2a_coord_backward_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "p2a_coord_get_stats": ([C.c_void_p, C.POINTER(p2a_coord_stats)], C.c_int),
     "p2a_coord_set_profiling": ([C.c_void_p, C.c_int], C.c_int),
 }

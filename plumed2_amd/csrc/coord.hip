@@ -526,7 +526,18 @@ __device__ double exact_r2(const PbcArg* __restrict__ P, const double* __restric
 //     independent loads per lane);
 //  3. dense loop over the staged clusters: lane l evaluates the pair
 //     (i = l&7 of cluster ci, j = l>>3 of the staged cluster).
-template <int P, bool GENERIC, bool TWO>
+template <int P>
+__device__ __forceinline__ float sw_fixed_value(const SwF& f, float r2) {
+  float x = r2 * f.invr0_2;
+  float xp = 1.0f;
+#pragma unroll
+  for (int k = 1; k < P; k++) xp *= x;
+  return fmaf(__builtin_amdgcn_rcpf(fmaf(xp, x, 1.0f)), f.stretch, f.shift);
+}
+
+// VO (value only): no derivatives and no virial; entries that do not add to
+// the value are skipped, so the loop visits each unordered pair once.
+template <int P, bool GENERIC, bool TWO, bool VO>
 __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __restrict__ items,
                                                        const int* __restrict__ estart,
                                                        const Entry* __restrict__ ents,
@@ -595,6 +606,9 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __
 
   for (int k = 0; k < npass; k++) {
     const int code = scode[w][k];
+    if constexpr (VO) {
+      if (!(code & kCountBit) && code >= 0) continue;  // uniform
+    }
     const float4 pj = sj[w][k][jj];
     const float dx = pj.x - pi.x;
     const float dy = pj.y - pi.y;
@@ -611,6 +625,19 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __
     const bool diag = code < 0;
     if (diag) cut = cut && (jj != ii);  // an atom is not paired with itself
     if constexpr (TWO) cut = cut && (sabs[w][k][jj] != ai);
+    if constexpr (VO) {
+      float s;
+      if constexpr (GENERIC) {
+        double dfd;
+        s = (float)switch_calc_sqr(A.sw, (double)r2, dfd);
+      } else {
+        s = sw_fixed_value<P>(A.f, r2);
+      }
+      const bool c2 = diag ? (cut && (jj > ii)) : cut;
+      npair += __popcll(__ballot(c2));
+      cv += c2 ? s : 0.f;
+      continue;
+    }
     float s, df;
     if constexpr (GENERIC) {
       double dfd;
@@ -660,7 +687,7 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __
     dfy += __shfl_xor(dfy, o, 64);
     dfz += __shfl_xor(dfz, o, 64);
   }
-  if (jj == 0) {
+  if (!VO && jj == 0) {
     double* fp = fpart + (size_t)item * (CL * 3) + ii * 3;
     fp[0] = -dfx;
     fp[1] = -dfy;
@@ -1149,7 +1176,7 @@ static int upload_pbc(p2a_coord* c, const double* box9) {
 }
 
 static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, double* value_dev, double* deriv_dev,
-                     double* virial_dev) {
+                     double* virial_dev, bool value_only) {
   int rc;
   hipStream_t s = c->stream;
   const int ngroups = c->spec.nb > 0 ? 2 : 1;
@@ -1192,7 +1219,9 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     c->ev_pool.pop_back();
     P2A_HIP(hipEventRecord(e0, s));
   }
-  for (int d = 0; d < c->ndir; d++) {
+  // value only: the counting direction (the first) holds every counted pair
+  const int ndir_run = value_only ? 1 : c->ndir;
+  for (int d = 0; d < ndir_run; d++) {
     Dir& D = c->dir[d];
     if (D.nitems == 0) continue;
     Group& GI = c->g[D.I];
@@ -1200,14 +1229,18 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     A.nitems = D.nitems;
     dim3 grid(grid1(D.nitems, PAIR_BLOCK / WAVE)), block(PAIR_BLOCK);
     double* part = c->partial.p + 8 * (size_t)D.item_offset;
-#define P2A_PAIRS(P, GEN)                                                                                        \
-  do {                                                                                                            \
-    if (A.twogroup)                                                                                               \
-      hipLaunchKernelGGL((k_pairs<P, GEN, true>), grid, block, 0, s, A, D.items.p, D.estart.p, D.ents.p, GI.p.p, \
-                         GJ.p.p, GI.bb.p, GJ.bb.p, GI.abs_slot.p, GJ.abs_slot.p, pos_dev, D.fpart.p, part);      \
-    else                                                                                                          \
-      hipLaunchKernelGGL((k_pairs<P, GEN, false>), grid, block, 0, s, A, D.items.p, D.estart.p, D.ents.p, GI.p.p,\
-                         GJ.p.p, GI.bb.p, GJ.bb.p, GI.abs_slot.p, GJ.abs_slot.p, pos_dev, D.fpart.p, part);      \
+#define P2A_PAIRS_VO(P, GEN, TWO, VO)                                                                          \
+  hipLaunchKernelGGL((k_pairs<P, GEN, TWO, VO>), grid, block, 0, s, A, D.items.p, D.estart.p, D.ents.p, GI.p.p, \
+                     GJ.p.p, GI.bb.p, GJ.bb.p, GI.abs_slot.p, GJ.abs_slot.p, pos_dev, D.fpart.p, part)
+#define P2A_PAIRS(P, GEN)                                    \
+  do {                                                       \
+    if (A.twogroup) {                                        \
+      if (value_only) P2A_PAIRS_VO(P, GEN, true, true);      \
+      else P2A_PAIRS_VO(P, GEN, true, false);                \
+    } else {                                                 \
+      if (value_only) P2A_PAIRS_VO(P, GEN, false, true);     \
+      else P2A_PAIRS_VO(P, GEN, false, false);               \
+    }                                                        \
   } while (0)
     if (sw.type == P2A_SW_RATIONAL_FIXED) {
       switch (sw.nn / 2) {
@@ -1223,13 +1256,14 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
       P2A_PAIRS(1, true);
     }
 #undef P2A_PAIRS
+#undef P2A_PAIRS_VO
   }
   if (c->profiling) {
     P2A_HIP(hipEventRecord(e1, s));
     c->pending.emplace_back(e0, e1);
   }
   if ((rc = launch_check("pair kernel"))) return rc;
-  for (int d = 0; d < c->ndir; d++) {
+  for (int d = 0; d < c->ndir && !value_only; d++) {
     Dir& D = c->dir[d];
     Group& GI = c->g[D.I];
     if (GI.ncl == 0) continue;
@@ -1237,7 +1271,8 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
                        D.istart.p, D.fpart.p, deriv_dev);
   }
   if ((rc = c->red.reserve(8 * kRedBlocks))) return rc;
-  reduce_partials(s, c->nitems, c->partial.p, c->red.p, value_dev, virial_dev, c->stats.p);
+  reduce_partials(s, value_only ? c->dir[0].nitems : c->nitems, c->partial.p, c->red.p, value_dev, virial_dev,
+                  c->stats.p);
   return launch_check("finalize");
 }
 
@@ -1275,6 +1310,8 @@ static int calc_device(p2a_coord* c, int64_t step, const double* box9, const dou
     if ((rc = c->stats.reserve(8))) return rc;
     hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(64), 0, c->stream, c->stats.p, (size_t)8);
   }
+  // no derivative and no virial requested: the cheaper value-only kernel
+  const bool value_only = (deriv_dev == nullptr && virial_dev == nullptr);
   double* vdev = value_dev;
   double* wdev = virial_dev;
   double* ddev = deriv_dev;
@@ -1300,7 +1337,7 @@ static int calc_device(p2a_coord* c, int64_t step, const double* box9, const dou
   if (!need && std::memcmp(c->box_at_build, box9, sizeof(c->box_at_build)) != 0) need = true;
   c->firsttime = false;
   if (need && (rc = rebuild(c, box9, pos_dev))) return rc;
-  return run_pairs(c, box9, pos_dev, vdev, ddev, wdev);
+  return run_pairs(c, box9, pos_dev, vdev, ddev, wdev, value_only);
 }
 
 static void collect_profile(p2a_coord* c) {
@@ -1396,7 +1433,8 @@ int p2a_coord_destroy(p2a_coord* c) {
 
 int p2a_coord_set_stream(p2a_coord* c, void* stream) {
   if (!c) return fail(P2A_ERR_ARG, "null context");
-  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  // NULL selects the default (null) stream, which orders with every blocking stream
+  c->stream = (hipStream_t)stream;
   return P2A_OK;
 }
 
@@ -1407,6 +1445,19 @@ int p2a_coord_calculate_device(p2a_coord* c, int64_t step, const double* box9, c
   return calc_device(c, step, box9, pos_dev, value_dev, deriv_dev, virial_dev);
 }
 
+int p2a_coord_backward_device(p2a_coord* c, const double* box9, const double* pos_dev, double* deriv_dev,
+                              double* virial_dev) {
+  if (!c || !pos_dev || !box9 || !deriv_dev) return fail(P2A_ERR_ARG, "null argument");
+  P2A_HIP(hipSetDevice(c->device));
+  int rc;
+  if ((rc = c->out_buf.reserve(16))) return rc;
+  double* wdev = virial_dev ? virial_dev : c->out_buf.p + 1;
+  if (c->path != 0 || choose_path(c, box9) != 0) return run_fallback(c, box9, pos_dev, c->out_buf.p, deriv_dev, wdev);
+  if (!c->built || std::memcmp(c->box_at_build, box9, sizeof(c->box_at_build)) != 0)
+    return fail(P2A_ERR_ARG, "backward needs the neighbour structure of a forward call on the same box");
+  return run_pairs(c, box9, pos_dev, c->out_buf.p, deriv_dev, wdev, false);
+}
+
 int p2a_coord_calculate(p2a_coord* c, int64_t step, const double* box9, const double* pos, double* value,
                         double* deriv, double* virial) {
   if (!c || !pos) return fail(P2A_ERR_ARG, "null argument");
@@ -1415,7 +1466,9 @@ int p2a_coord_calculate(p2a_coord* c, int64_t step, const double* box9, const do
   const size_t n3 = 3 * (size_t)c->natoms;
   if ((rc = c->pos_buf.reserve(n3)) || (rc = c->deriv_buf.reserve(n3)) || (rc = c->out_buf.reserve(16))) return rc;
   P2A_HIP(hipMemcpyAsync(c->pos_buf.p, pos, n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  if ((rc = calc_device(c, step, box9, c->pos_buf.p, c->out_buf.p, c->deriv_buf.p, c->out_buf.p + 1))) return rc;
+  if ((rc = calc_device(c, step, box9, c->pos_buf.p, c->out_buf.p, deriv ? c->deriv_buf.p : nullptr,
+                        (deriv || virial) ? c->out_buf.p + 1 : nullptr)))
+    return rc;
   double out[10];
   P2A_HIP(hipMemcpyAsync(out, c->out_buf.p, 10 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if (deriv) P2A_HIP(hipMemcpyAsync(deriv, c->deriv_buf.p, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));

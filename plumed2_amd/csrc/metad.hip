@@ -471,7 +471,8 @@ int p2a_metad_destroy(p2a_metad* m) {
 
 int p2a_metad_set_stream(p2a_metad* m, void* stream) {
   if (!m) return fail(P2A_ERR_ARG, "null context");
-  m->stream = stream ? (hipStream_t)stream : m->own_stream;
+  // NULL selects the default (null) stream, which orders with every blocking stream
+  m->stream = (hipStream_t)stream;
   return P2A_OK;
 }
 
