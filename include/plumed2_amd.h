@@ -187,6 +187,65 @@ int p2a_coord_get_stats(p2a_coord *ctx, p2a_coord_stats *stats);
  * (kernel time and total_contributing_pairs). */
 int p2a_coord_set_profiling(p2a_coord *ctx, int on);
 
+/* Work decomposition of the pair kernel: `chunk` list entries per wave (32 or
+ * 64) and a persistent grid of `blocks_per_cu` workgroups per compute unit
+ * (0 = one wave per work item).  Results do not depend on it.  No reference
+ * counterpart (the reference's OpenMP loop has no such knob). */
+int p2a_coord_set_tuning(p2a_coord *ctx, int chunk, int blocks_per_cu);
+
+/* ------------------------------------------------------------------------
+ * Batched COORDINATION: many independent single-group systems (frames or
+ * walkers, each with its own atom count and periodic cell) sharing one
+ * switching function and list radius.  Every kernel of a step covers all
+ * systems at once, the neighbour structure is rebuilt at every call
+ * (NL_STRIDE=1), and the step can be replayed as one captured HIP graph.
+ * Replaces running one COORDINATION action per system
+ * (CoordinationBase::prepare + calculate, src/colvar/CoordinationBase.cpp:
+ * 137-232, with NLISTCELLS, NL_CUTOFF=r_list, NL_STRIDE=1) and summing the
+ * per-system outputs.
+ *
+ * Layouts: positions are the systems' AoS FP64 arrays concatenated in system
+ * order ([sum natoms][3]); derivatives likewise; values[nsys]; virials
+ * [nsys][9]; boxes [nsys][9] (rows = lattice vectors).
+ * ------------------------------------------------------------------------ */
+typedef struct p2a_batch p2a_batch;
+
+typedef struct p2a_batch_stats {
+  int64_t nsys, natoms;
+  int64_t clusters, list_entries, items; /* of the last step                       */
+  int64_t contributing_pairs;            /* sum over systems, last step            */
+  int32_t chunk, blocks_per_cu;          /* pair-kernel decomposition in use       */
+  int32_t graph;                         /* 1 when the last step ran as a graph    */
+  int32_t regrowths;                     /* list arrays grown after an overflow    */
+  double last_step_ms;                   /* device time of the last step           */
+} p2a_batch_stats;
+
+/* natoms[nsys] > 0 atoms per system; sw = the switch (p2a_switch_parse);
+ * rlist = neighbour-list radius (>= D_MAX; NL_CUTOFF of the reference). */
+int p2a_batch_create(p2a_batch **b, int device, int nsys, const int64_t *natoms, const p2a_switch *sw,
+                     double rlist);
+int p2a_batch_destroy(p2a_batch *b);
+/* Use the caller's stream (NULL = the default stream; graphs then stay off). */
+int p2a_batch_set_stream(p2a_batch *b, void *stream);
+/* Periodic cells of all systems (host, [nsys][9]).  Each cell must be wider
+ * than 2*rlist in every direction (P2A_ERR_UNSUPPORTED names the system). */
+int p2a_batch_set_boxes(p2a_batch *b, const double *boxes);
+/* One step over every system; device pointers.  deriv and virials may be NULL
+ * together (value only is not specialised: they are computed and dropped).
+ * Returns when the step has finished. */
+int p2a_batch_calculate_device(p2a_batch *b, const double *pos_dev, double *values_dev, double *deriv_dev,
+                               double *virials_dev);
+/* Contributing pairs per system of the last step (host int64[nsys]). */
+int p2a_batch_get_pair_counts(p2a_batch *b, int64_t *out);
+/* Capture the step into a HIP graph on the first call and replay it after
+ * (on != 0); re-captured when pointers, tuning or array sizes change. */
+int p2a_batch_set_graph(p2a_batch *b, int on);
+int p2a_batch_set_tuning(p2a_batch *b, int chunk, int blocks_per_cu);
+/* Time `reps` steps of every candidate decomposition on these positions,
+ * keep the fastest; *chunk / *blocks_per_cu receive the choice (may be NULL). */
+int p2a_batch_autotune(p2a_batch *b, const double *pos_dev, int reps, int *chunk, int *blocks_per_cu);
+int p2a_batch_get_stats(p2a_batch *b, p2a_batch_stats *stats);
+
 /* ------------------------------------------------------------------------
  * METAD bias on a grid (diagonal Gaussian hills, well-tempered heights)
  * ------------------------------------------------------------------------ */

@@ -7,6 +7,7 @@ reference's action interface.
 from ._capi import Plumed2AmdError, lib  # noqa: F401
 from .coordination import (Coordination, CoordinationSpec, describe_switch,  # noqa: F401
                            parse_switch, rational_switch)
+from .batch import CoordinationBatch  # noqa: F401
 from .metad import MetaD  # noqa: F401
 
 __version__ = "0.1.0"

@@ -48,6 +48,13 @@ class p2a_coord_stats(C.Structure):
                 ("pair_kernel_launches", C.c_int64), ("total_contributing_pairs", C.c_int64)]
 
 
+class p2a_batch_stats(C.Structure):
+    _fields_ = [("nsys", C.c_int64), ("natoms", C.c_int64), ("clusters", C.c_int64), ("list_entries", C.c_int64),
+                ("items", C.c_int64), ("contributing_pairs", C.c_int64), ("chunk", C.c_int32),
+                ("blocks_per_cu", C.c_int32), ("graph", C.c_int32), ("regrowths", C.c_int32),
+                ("last_step_ms", C.c_double)]
+
+
 _lib = None
 
 _SIGS = {
@@ -69,6 +76,18 @@ _SIGS = {
     "p2a_coord_backward_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "p2a_coord_get_stats": ([C.c_void_p, C.POINTER(p2a_coord_stats)], C.c_int),
     "p2a_coord_set_profiling": ([C.c_void_p, C.c_int], C.c_int),
+    "p2a_coord_set_tuning": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    "p2a_batch_create": ([C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_void_p, C.POINTER(p2a_switch), C.c_double],
+                         C.c_int),
+    "p2a_batch_destroy": ([C.c_void_p], C.c_int),
+    "p2a_batch_set_stream": ([C.c_void_p, C.c_void_p], C.c_int),
+    "p2a_batch_set_boxes": ([C.c_void_p, C.c_void_p], C.c_int),
+    "p2a_batch_calculate_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "p2a_batch_get_pair_counts": ([C.c_void_p, C.c_void_p], C.c_int),
+    "p2a_batch_set_graph": ([C.c_void_p, C.c_int], C.c_int),
+    "p2a_batch_set_tuning": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    "p2a_batch_autotune": ([C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+    "p2a_batch_get_stats": ([C.c_void_p, C.c_void_p], C.c_int),
 }
 
 # symbols declared in the header that later sections bind lazily (METAD, batch)

@@ -147,3 +147,7 @@ class Coordination:
 
     def set_profiling(self, on: bool):
         A.check(A.lib().p2a_coord_set_profiling(self._ctx, int(bool(on))))
+
+    def set_tuning(self, chunk: int = 64, blocks_per_cu: int = 0):
+        """Entries per wave (32/64) and persistent grid of the pair kernel."""
+        A.check(A.lib().p2a_coord_set_tuning(self._ctx, int(chunk), int(blocks_per_cu)))

@@ -1,0 +1,1228 @@
+// Device code of the COORDINATION path: neighbour-structure build, the
+// cluster-pair kernel, reductions and the FP64 kernels for the inputs the
+// cluster path does not cover.  Shared by the single-system context
+// (coord.hip) and the multi-system batch (batch.hip); every kernel has
+// internal linkage, so each translation unit instantiates what it launches.
+//
+// What each piece replaces in the reference:
+//   k_bin .. k_list   NeighborList::update cells branch + LinkCells
+//                     (NeighborList.cpp:168-236, LinkCells.cpp:99-239); the
+//                     O(N^2) pair array is never materialised, the list is one
+//                     of 8x8-atom cluster pairs
+//   k_pack            ActionAtomistic::retrieveAtoms gather + the image
+//                     bookkeeping of Pbc::distance
+//   k_pairs           CoordinationBase::calculate pair loop
+//                     (CoordinationBase.cpp:178-207) with
+//                     fixedRational<N>::calculateSqr (SwitchingFunction.cpp:
+//                     203-215) inlined
+//   k_finalize,       the OpenMP critical merge + comm.Sum
+//   k_sys_reduce      (CoordinationBase.cpp:209-224)
+//   k_allpairs,       the same loop in FP64 for NOPBC, PAIR, unbounded D_MAX,
+//   k_pairmode        cells narrower than 2*cutoff
+//
+// Several systems (frames, walkers) can share one set of arrays: atoms,
+// columns and clusters of system s are contiguous, SysArg[s] holds its cell
+// and its column grid, and per-atom / per-cluster system indices select it.
+// A single system is the case nsys = 1 with null system-index arrays.
+//
+// Layout in HBM:
+//   slots      8 consecutive slots = one cluster; clusters are runs of atoms
+//              in columns of the fractional (a,b) plane sorted along c;
+//              float4 {x,y,z local to the cluster origin, requested index}
+//   cl_origin  double4 per cluster, FP64 origin fixed at the rebuild
+//   cl_bb      2 x float4 per cluster (centre, half extents), every step
+//   cl_kr      float2 per cluster: range of fractional c of its atoms
+//   list       CSR over i clusters of Entry{code, delta}: code = j cluster |
+//              count << 30 | diagonal << 31; delta = o_j + T - o_i (FP32)
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "p2a_internal.h"
+#include "switch_eval.h"
+
+namespace p2a {
+
+#define P2A_HIP(x)                                                                           \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess) return fail(P2A_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int CL = 8;           // atoms per cluster
+constexpr int WAVE = 64;
+constexpr int PAIR_BLOCK = 256; // 4 waves per workgroup
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  int reserve(size_t m) {
+    if (m <= n && p) return P2A_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    size_t want = std::max<size_t>(m, 1);
+    if (hipMalloc(&p, want * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      return fail(P2A_ERR_NOMEM, "hipMalloc of " + std::to_string(want * sizeof(T)) + " bytes failed");
+    }
+    n = want;
+    return P2A_OK;
+  }
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+struct Entry {
+  int code;
+  float dx, dy, dz;
+};
+
+// cell and column grid of one system
+struct SysArg {
+  double a[9];    // lattice vectors as rows
+  double inv[9];  // inverse of a
+  float af[9];    // lattice vectors in FP32 (list search)
+  float rb[3];    // list radius as a fraction of the cell along a, b, c (r * |reciprocal row|)
+  int ncx, ncy;   // columns along a and b
+  int colbase;    // first column of this system in the shared column arrays
+  int pad_;
+};
+
+struct SwF {  // FP32 constants of fixedRational<N>
+  float dmax2, invr0_2, stretch, shift, cder;  // cder = -N*invr0_2*stretch
+};
+
+struct PbcArg {
+  int type;
+  double box[9], inv[9], red[9], ired[9];
+  int nshift[8];
+  double shifts[8][kMaxShift][3];
+};
+
+// Entry code bits: j cluster (bits 0-29), "count" (bit 30: this entry adds
+// to the value and the virial; each unordered pair of atoms is counted by
+// exactly one entry), "diagonal" (bit 31: i cluster paired with itself
+// without image, pairs with j == i are skipped and counted when j > i).
+constexpr int kCjMask = 0x3FFFFFFF;
+constexpr int kCountBit = 0x40000000;
+
+namespace {
+
+// ------------------------------------------------------------------------
+// small device helpers
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// exclusive prefix sum over the 64 lanes; *total = sum of all lanes
+__device__ __forceinline__ int wave_excl_scan(int v, int lane, int& total) {
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  total = __shfl(x, 63, 64);
+  return x - v;
+}
+
+__device__ __forceinline__ int floordiv(int a, int n) { return (a >= 0) ? a / n : -((-a + n - 1) / n); }
+
+__device__ __forceinline__ int nsys_of(const int* idx, int i) { return idx ? idx[i] : 0; }
+
+// ------------------------------------------------------------------------
+// rebuild kernels
+// ------------------------------------------------------------------------
+
+// fractional coordinates, image and column of every atom (atom i of the
+// arrays is requested atom off + i)
+__global__ void k_bin(int n, const double* __restrict__ pos, int off, const SysArg* __restrict__ sys,
+                      const int* __restrict__ atom_sys, int4* __restrict__ info, double* __restrict__ key,
+                      int* __restrict__ col_count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SysArg& S = sys[nsys_of(atom_sys, i)];
+  const double* x = pos + 3 * (size_t)(off + i);
+  double s[3];
+  int m[3];
+  for (int j = 0; j < 3; j++) {
+    s[j] = x[0] * S.inv[0 * 3 + j] + x[1] * S.inv[1 * 3 + j] + x[2] * S.inv[2 * 3 + j];
+    double fl = floor(s[j]);
+    m[j] = (int)fl;
+    s[j] -= fl;
+  }
+  int cx = min((int)(s[0] * S.ncx), S.ncx - 1);
+  int cy = min((int)(s[1] * S.ncy), S.ncy - 1);
+  cx = max(cx, 0);
+  cy = max(cy, 0);
+  int col = S.colbase + cx + S.ncx * cy;
+  info[i] = make_int4(m[0], m[1], m[2], col);
+  key[i] = s[2];
+  atomicAdd(&col_count[col], 1);
+}
+
+// exclusive scans of column counts (atoms) and of ceil(count/8) (clusters),
+// one workgroup
+__global__ void k_scan_cols(int ncol, const int* __restrict__ cnt, int* __restrict__ col_start,
+                            int* __restrict__ col_cl, int* __restrict__ totals) {
+  __shared__ int sa[1024], sc[1024];
+  const int t = threadIdx.x, T = blockDim.x;
+  int chunk = (ncol + T - 1) / T;
+  int b = t * chunk, e = min(ncol, b + chunk);
+  int la = 0, lc = 0;
+  for (int i = b; i < e; i++) {
+    la += cnt[i];
+    lc += (cnt[i] + CL - 1) / CL;
+  }
+  sa[t] = la;
+  sc[t] = lc;
+  __syncthreads();
+  for (int o = 1; o < T; o <<= 1) {
+    int va = (t >= o) ? sa[t - o] : 0, vc = (t >= o) ? sc[t - o] : 0;
+    __syncthreads();
+    sa[t] += va;
+    sc[t] += vc;
+    __syncthreads();
+  }
+  int ra = sa[t] - la, rc = sc[t] - lc;
+  for (int i = b; i < e; i++) {
+    col_start[i] = ra;
+    col_cl[i] = rc;
+    ra += cnt[i];
+    rc += (cnt[i] + CL - 1) / CL;
+  }
+  if (t == T - 1) {
+    col_start[ncol] = sa[t];
+    col_cl[ncol] = sc[t];
+    totals[0] = sa[t];
+    totals[1] = sc[t];
+  }
+}
+
+// exclusive scan of n ints (one workgroup; n read from n_dev when given);
+// chunk > 0 scans ceil(in/chunk) instead of in; totals[slot] = sum
+__global__ void k_scan(int n, const int* __restrict__ n_dev, const int* __restrict__ in, int* __restrict__ out,
+                       int* __restrict__ totals, int slot, int chunk) {
+  __shared__ int s[1024];
+  if (n_dev) n = *n_dev;
+  const int t = threadIdx.x, T = blockDim.x;
+  int per = (n + T - 1) / T;
+  int b = t * per, e = min(n, b + per);
+  int l = 0;
+  for (int i = b; i < e; i++) l += chunk ? (in[i] + chunk - 1) / chunk : in[i];
+  s[t] = l;
+  __syncthreads();
+  for (int o = 1; o < T; o <<= 1) {
+    int v = (t >= o) ? s[t - o] : 0;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  int r = s[t] - l;
+  for (int i = b; i < e; i++) {
+    out[i] = r;
+    r += chunk ? (in[i] + chunk - 1) / chunk : in[i];
+  }
+  if (t == T - 1) {
+    out[n] = s[t];
+    totals[slot] = s[t];
+  }
+}
+
+__global__ void k_fill_int(int* p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_double(double* p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = 0.0;
+}
+
+__global__ void k_scatter(int n, const int4* __restrict__ info, const int* __restrict__ col_start,
+                          int* __restrict__ fill, int* __restrict__ tmp) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int col = info[i].w;
+  int k = atomicAdd(&fill[col], 1);
+  tmp[col_start[col] + k] = i;
+}
+
+// rank of each atom inside its column by (key, index): deterministic order.
+// One workgroup per column; the column's keys are staged in LDS when they fit.
+constexpr int kRankLds = 2048;
+__global__ __launch_bounds__(256) void k_rank(int ncol, const double* __restrict__ key,
+                                              const int* __restrict__ col_start, const int* __restrict__ col_cl,
+                                              const int* __restrict__ tmp, int* __restrict__ slot_loc) {
+  __shared__ double sk[kRankLds];
+  __shared__ int si[kRankLds];
+  const int col = blockIdx.x;
+  if (col >= ncol) return;
+  const int b = col_start[col], n = col_start[col + 1] - b;
+  const int base = col_cl[col] * CL;
+  if (n <= kRankLds) {
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      int o = tmp[b + t];
+      si[t] = o;
+      sk[t] = key[o];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      const double ka = sk[t];
+      const int ia = si[t];
+      int rank = 0;
+      for (int u = 0; u < n; u++) {
+        const double kb = sk[u];
+        rank += (kb < ka) || (kb == ka && si[u] < ia);
+      }
+      slot_loc[base + rank] = ia;
+    }
+  } else {
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      const int ia = tmp[b + t];
+      const double ka = key[ia];
+      int rank = 0;
+      for (int u = 0; u < n; u++) {
+        const int o = tmp[b + u];
+        const double kb = key[o];
+        rank += (kb < ka) || (kb == ka && o < ia);
+      }
+      slot_loc[base + rank] = ia;
+    }
+  }
+}
+
+__global__ void k_cl_col(int ncol, const int* __restrict__ col_cl, int* __restrict__ cl_col) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  for (int k = col_cl[c]; k < col_cl[c + 1]; k++) cl_col[k] = c;
+}
+
+// FP64 origin (bounding-box centre at the rebuild), rebuild bounding box,
+// range of fractional c and system of every cluster
+__global__ void k_origin(int ncl, const int* __restrict__ ncl_dev, const int* __restrict__ slot_loc,
+                         const int4* __restrict__ info, const double* __restrict__ key,
+                         const double* __restrict__ pos, int off, const SysArg* __restrict__ sys,
+                         const int* __restrict__ atom_sys, double4* __restrict__ origin, float4* __restrict__ bb0,
+                         float2* __restrict__ kr, int* __restrict__ cl_sys) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ncl_dev) ncl = *ncl_dev;
+  if (c >= ncl) return;
+  // the first slot of a cluster always holds an atom (padding comes last)
+  const int s = nsys_of(atom_sys, slot_loc[c * CL]);
+  const SysArg& S = sys[s];
+  if (cl_sys) cl_sys[c] = s;
+  double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  double klo = DBL_MAX, khi = -DBL_MAX;
+  for (int k = 0; k < CL; k++) {
+    int a = slot_loc[c * CL + k];
+    if (a < 0) continue;
+    int4 m = info[a];
+    const double* x = pos + 3 * (size_t)(off + a);
+    for (int d = 0; d < 3; d++) {
+      double w = x[d] - (m.x * S.a[0 * 3 + d] + m.y * S.a[1 * 3 + d] + m.z * S.a[2 * 3 + d]);
+      lo[d] = fmin(lo[d], w);
+      hi[d] = fmax(hi[d], w);
+    }
+    klo = fmin(klo, key[a]);
+    khi = fmax(khi, key[a]);
+  }
+  double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
+  origin[c] = make_double4(cx, cy, cz, 0.0);
+  bb0[2 * c] = make_float4((float)cx, (float)cy, (float)cz, 0.f);
+  bb0[2 * c + 1] = make_float4((float)(0.5 * (hi[0] - lo[0])), (float)(0.5 * (hi[1] - lo[1])),
+                               (float)(0.5 * (hi[2] - lo[2])), 0.f);
+  // widened outwards so that the FP32 range always contains the FP64 keys
+  kr[c] = make_float2((float)klo - 1.0e-6f, (float)khi + 1.0e-6f);
+}
+
+struct ListArgs {
+  int nclI;
+  const int* nclI_dev;  // device count of i clusters (null: nclI)
+  int mode;             // 0 one group, 1 two groups (counting side), 2 two groups (other side)
+  int ent_cap;          // capacity of the entry array (fill pass)
+  float rlist2;
+};
+
+// Cluster-pair search: one wave per i cluster.  FILL=0 counts, FILL=1 writes.
+// Candidates are the clusters of the (column, image) window around the i
+// cluster whose range of fractional c overlaps the i cluster's range widened
+// by the list radius: a pair closer than r differs by at most r*|c*| in
+// fractional c, so the window is exact, and the clusters of a column are
+// sorted along c, so each window is one binary-searched run of clusters.
+template <int FILL>
+__global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict__ cl_colI,
+                                              const int* __restrict__ cl_sysI, const SysArg* __restrict__ sysI,
+                                              const SysArg* __restrict__ sysJ, const float4* __restrict__ bbI,
+                                              const float4* __restrict__ bbJ, const float2* __restrict__ krI,
+                                              const float2* __restrict__ krJ, const int* __restrict__ col_clJ,
+                                              const double4* __restrict__ orgI, const double4* __restrict__ orgJ,
+                                              int* __restrict__ count, const int* __restrict__ start,
+                                              Entry* __restrict__ out) {
+  __shared__ int s_first[4][WAVE], s_off[4][WAVE], s_img[4][WAVE];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x / WAVE;
+  const int ci = blockIdx.x * (blockDim.x / WAVE) + wv;
+  const int nclI = L.nclI_dev ? *L.nclI_dev : L.nclI;
+  if (ci >= nclI) return;  // uniform per wave; no workgroup barrier below
+  const int s = nsys_of(cl_sysI, ci);
+  const SysArg& SI = sysI[s];
+  const SysArg& SJ = sysJ[s];
+  const int colI = cl_colI[ci] - SI.colbase;
+  const int cxi = colI % SI.ncx, cyi = colI / SI.ncx;
+  const float4 ic = bbI[2 * ci], ih = bbI[2 * ci + 1];
+  const float2 ki = krI[ci];
+  const int x0 = (int)floorf(((float)cxi / SI.ncx - SJ.rb[0]) * SJ.ncx) - 1;
+  const int x1 = (int)floorf(((float)(cxi + 1) / SI.ncx + SJ.rb[0]) * SJ.ncx) + 1;
+  const int y0 = (int)floorf(((float)cyi / SI.ncy - SJ.rb[1]) * SJ.ncy) - 1;
+  const int y1 = (int)floorf(((float)(cyi + 1) / SI.ncy + SJ.rb[1]) * SJ.ncy) + 1;
+  // images -1, 0, +1 along a and b cover every candidate when the cell is
+  // wider than 2*rlist; each unwrapped column is a distinct (column, image)
+  const int xa = max(x0, -SJ.ncx), xb = min(x1, 2 * SJ.ncx - 1);
+  const int ya = max(y0, -SJ.ncy), yb = min(y1, 2 * SJ.ncy - 1);
+  const int nx = xb - xa + 1, ny = yb - ya + 1;
+  const int ntup = nx * ny * 3;
+  const float zlo = ki.x - SJ.rb[2] - 1.0e-6f, zhi = ki.y + SJ.rb[2] + 1.0e-6f;
+  const double4 oi = orgI[ci];
+  int n = 0;
+  const int base = FILL ? start[ci] : 0;
+  for (int t0 = 0; t0 < ntup; t0 += WAVE) {
+    const int t = t0 + lane;
+    int first = 0, cnt = 0, img = 0;
+    if (t < ntup) {
+      const int mzi = t % 3, r = t / 3;
+      const int cxu = xa + r % nx, cyu = ya + r / nx;
+      const int mx = floordiv(cxu, SJ.ncx), cxw = cxu - mx * SJ.ncx;
+      const int my = floordiv(cyu, SJ.ncy), cyw = cyu - my * SJ.ncy;
+      const int col = SJ.colbase + cxw + SJ.ncx * cyw;
+      const int c0 = col_clJ[col], c1 = col_clJ[col + 1];
+      const float lo = zlo - (float)(mzi - 1), hi = zhi - (float)(mzi - 1);
+      if (hi >= 0.f && lo < 1.f && c1 > c0) {
+        int a = c0, b = c1;
+        while (a < b) {  // first cluster whose upper key reaches lo
+          const int m = (a + b) >> 1;
+          if (krJ[m].y < lo) a = m + 1;
+          else b = m;
+        }
+        first = a;
+        b = c1;
+        while (a < b) {  // first cluster whose lower key exceeds hi
+          const int m = (a + b) >> 1;
+          if (krJ[m].x <= hi) a = m + 1;
+          else b = m;
+        }
+        cnt = a - first;
+      }
+      img = (mx + 1) | ((my + 1) << 2) | (mzi << 4);
+    }
+    int total;
+    const int off = wave_excl_scan(cnt, lane, total);
+    s_first[wv][lane] = first;
+    s_off[wv][lane] = off;
+    s_img[wv][lane] = img;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int u0 = 0; u0 < total; u0 += WAVE) {
+      const int u = u0 + lane;
+      bool pass = false;
+      int cj = 0, mx = 0, my = 0, mz = 0;
+      if (u < total) {
+        int a = 0, b = WAVE - 1;  // last tuple whose offset is <= u
+        while (a < b) {
+          const int m = (a + b + 1) >> 1;
+          if (s_off[wv][m] <= u) a = m;
+          else b = m - 1;
+        }
+        cj = s_first[wv][a] + (u - s_off[wv][a]);
+        const int im = s_img[wv][a];
+        mx = (im & 3) - 1;
+        my = ((im >> 2) & 3) - 1;
+        mz = ((im >> 4) & 3) - 1;
+        const float Tx = mx * SJ.af[0] + my * SJ.af[3] + mz * SJ.af[6];
+        const float Ty = mx * SJ.af[1] + my * SJ.af[4] + mz * SJ.af[7];
+        const float Tz = mx * SJ.af[2] + my * SJ.af[5] + mz * SJ.af[8];
+        const float4 jc = bbJ[2 * cj], jh = bbJ[2 * cj + 1];
+        const float gx = fmaxf(fabsf(jc.x + Tx - ic.x) - ih.x - jh.x, 0.f);
+        const float gy = fmaxf(fabsf(jc.y + Ty - ic.y) - ih.y - jh.y, 0.f);
+        const float gz = fmaxf(fabsf(jc.z + Tz - ic.z) - ih.z - jh.z, 0.f);
+        pass = gx * gx + gy * gy + gz * gz < L.rlist2;
+      }
+      const unsigned long long bal = __ballot(pass);
+      if (FILL && pass) {
+        const int k = __popcll(bal & ((1ull << lane) - 1));
+        const int idx = base + n + k;
+        if (idx < L.ent_cap) {
+          const double4 oj = orgJ[cj];
+          const double Tx = mx * SJ.a[0] + my * SJ.a[3] + mz * SJ.a[6];
+          const double Ty = mx * SJ.a[1] + my * SJ.a[4] + mz * SJ.a[7];
+          const double Tz = mx * SJ.a[2] + my * SJ.a[5] + mz * SJ.a[8];
+          const bool zero = (mx == 0 && my == 0 && mz == 0);
+          const bool diag = L.mode == 0 && cj == ci && zero;
+          const bool positive = (mx > 0) || (mx == 0 && (my > 0 || (my == 0 && mz > 0)));
+          const bool counted = L.mode == 1 || (L.mode == 0 && (cj > ci || (cj == ci && positive)));
+          Entry en;
+          en.code = cj | (counted ? kCountBit : 0) | (diag ? (int)0x80000000u : 0);
+          en.dx = (float)(oj.x + Tx - oi.x);
+          en.dy = (float)(oj.y + Ty - oi.y);
+          en.dz = (float)(oj.z + Tz - oi.z);
+          out[idx] = en;
+        }
+      }
+      n += __popcll(bal);
+    }
+    // the next tuple chunk overwrites the tables read above
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (!FILL && lane == 0) count[ci] = n;
+}
+
+// work items (i cluster, chunk of `chunk` entries); writes past items_cap are dropped
+__global__ void k_items(int ncl, const int* __restrict__ ncl_dev, const int* __restrict__ count,
+                        const int* __restrict__ item_start, int2* __restrict__ items, int chunk, int items_cap) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ncl_dev) ncl = *ncl_dev;
+  if (c >= ncl) return;
+  int nch = (count[c] + chunk - 1) / chunk;
+  for (int k = 0; k < nch; k++) {
+    const int it = item_start[c] + k;
+    if (it < items_cap) items[it] = make_int2(c, k);
+  }
+}
+
+// ------------------------------------------------------------------------
+// per-step kernels
+// ------------------------------------------------------------------------
+
+// local FP32 coordinates of every slot; padding slots far away and distinct
+__global__ void k_pack(int nslot, const int* __restrict__ ncl_dev, const int* __restrict__ slot_loc,
+                       const int4* __restrict__ info, const double* __restrict__ pos, int off,
+                       const SysArg* __restrict__ sys, const int* __restrict__ atom_sys,
+                       const double4* __restrict__ origin, float4* __restrict__ out) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ncl_dev) nslot = *ncl_dev * CL;
+  if (s >= nslot) return;
+  int a = slot_loc[s];
+  if (a < 0) {
+    out[s] = make_float4(1.0e5f + 1.0e3f * (float)(s & 1023), 1.0e5f + 1.0e3f * (float)((s >> 10) & 1023),
+                         1.0e5f + 1.0e3f * (float)(s >> 20), __int_as_float(-1));
+    return;
+  }
+  const SysArg& S = sys[nsys_of(atom_sys, a)];
+  int4 m = info[a];
+  double4 o = origin[s / CL];
+  const double* x = pos + 3 * (size_t)(off + a);
+  double w[3];
+  for (int d = 0; d < 3; d++) w[d] = x[d] - (m.x * S.a[0 * 3 + d] + m.y * S.a[1 * 3 + d] + m.z * S.a[2 * 3 + d]);
+  out[s] = make_float4((float)(w[0] - o.x), (float)(w[1] - o.y), (float)(w[2] - o.z), __int_as_float(off + a));
+}
+
+// current bounding box of each cluster in its local frame
+__global__ void k_bbox(int ncl, const int* __restrict__ ncl_dev, const float4* __restrict__ p,
+                       float4* __restrict__ bb) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ncl_dev) ncl = *ncl_dev;
+  if (c >= ncl) return;
+  float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (int k = 0; k < CL; k++) {
+    float4 q = p[c * CL + k];
+    if (__float_as_int(q.w) < 0) continue;
+    lo[0] = fminf(lo[0], q.x);
+    hi[0] = fmaxf(hi[0], q.x);
+    lo[1] = fminf(lo[1], q.y);
+    hi[1] = fmaxf(hi[1], q.y);
+    lo[2] = fminf(lo[2], q.z);
+    hi[2] = fmaxf(hi[2], q.z);
+  }
+  bb[2 * c] = make_float4(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]), 0.f);
+  bb[2 * c + 1] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]), 0.5f * (hi[2] - lo[2]), 0.f);
+}
+
+// Tools::pbc (Tools.h:547-570)
+__device__ __forceinline__ double pbc1(double x) {
+  x += 100.0;
+  return (x >= 0) ? x - (double)(int)(x + 0.5) : x - (double)(int)(x - 0.5);
+}
+
+// Pbc::distance (Pbc.cpp:362-415)
+__device__ void pbc_distance(const PbcArg& P, const double* a, const double* b, double* d) {
+  d[0] = b[0] - a[0];
+  d[1] = b[1] - a[1];
+  d[2] = b[2] - a[2];
+  if (P.type == 1) {
+    for (int i = 0; i < 3; i++) d[i] = pbc1(d[i] * P.inv[4 * i]) * P.box[4 * i];
+  } else if (P.type == 2) {
+    double s[3];
+    for (int j = 0; j < 3; j++) s[j] = d[0] * P.ired[j] + d[1] * P.ired[3 + j] + d[2] * P.ired[6 + j];
+    for (int j = 0; j < 3; j++) s[j] = pbc1(s[j]);
+    for (int j = 0; j < 3; j++) d[j] = s[0] * P.red[j] + s[1] * P.red[3 + j] + s[2] * P.red[6 + j];
+    if (fabs(s[0]) + fabs(s[1]) + fabs(s[2]) > 0.5) {
+      int oct = (s[0] > 0 ? 4 : 0) + (s[1] > 0 ? 2 : 0) + (s[2] > 0 ? 1 : 0);
+      double best[3] = {d[0], d[1], d[2]};
+      double lbest = best[0] * best[0] + best[1] * best[1] + best[2] * best[2];
+      for (int k = 0; k < P.nshift[oct]; k++) {
+        double t0 = d[0] + P.shifts[oct][k][0], t1 = d[1] + P.shifts[oct][k][1], t2 = d[2] + P.shifts[oct][k][2];
+        double lt = t0 * t0 + t1 * t1 + t2 * t2;
+        if (lt < lbest) {
+          lbest = lt;
+          best[0] = t0;
+          best[1] = t1;
+          best[2] = t2;
+        }
+      }
+      d[0] = best[0];
+      d[1] = best[1];
+      d[2] = best[2];
+    }
+  }
+}
+
+// s and dfunc of fixedRational<2P> on r^2 in FP32
+template <int P>
+__device__ __forceinline__ void sw_fixed(const SwF& f, float r2, float& s, float& df) {
+  float x = r2 * f.invr0_2;
+  float xp = 1.0f;
+#pragma unroll
+  for (int k = 1; k < P; k++) xp *= x;  // x^(P-1)
+  float res = __builtin_amdgcn_rcpf(fmaf(xp, x, 1.0f));
+  s = fmaf(res, f.stretch, f.shift);
+  df = f.cder * xp * res * res;
+}
+
+template <int P>
+__device__ __forceinline__ float sw_fixed_value(const SwF& f, float r2) {
+  float x = r2 * f.invr0_2;
+  float xp = 1.0f;
+#pragma unroll
+  for (int k = 1; k < P; k++) xp *= x;
+  return fmaf(__builtin_amdgcn_rcpf(fmaf(xp, x, 1.0f)), f.stretch, f.shift);
+}
+
+struct PairArgs {
+  int nitems;
+  const int* nitems_dev;  // device item count (null: nitems)
+  int items_cap, ent_cap; // capacities of the item and entry arrays
+  int twogroup;
+  float dmax2, prune2;
+  SwF f;
+  p2a_switch sw;          // FP64 parameters (generic switch and the cut band)
+  const PbcArg* pbc;      // device copies of the prepared cells (FP64 cut decisions)
+  const int* cl_sys;      // system of each i cluster (null: one system)
+};
+
+// FP64 re-evaluation of the squared distance of one pair, used only when the
+// FP32 value lies within a narrow band around D_MAX^2: the inclusive cut of
+// the reference is then decided on the distance Pbc::distance itself would
+// produce from the raw positions (including the rounding of Tools::pbc).
+// The pair is oriented as the reference lists it (GROUPA atom first; lower
+// index first inside one group) because the rounding of Tools::pbc is not
+// symmetric under d -> -d.
+__device__ double exact_r2(const PbcArg* __restrict__ P, const double* __restrict__ pos, int reqi, int reqj) {
+  double d[3];
+  const int a = min(reqi, reqj), b = max(reqi, reqj);
+  pbc_distance(*P, pos + 3 * (size_t)a, pos + 3 * (size_t)b, d);
+  return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+}
+
+struct PairAcc {
+  float cv, fx, fy, fz, vxx, vxy, vxz, vyy, vyz, vzz;
+  int npair;
+};
+
+// Evaluate one group of 8 staged j atoms against the 8 i atoms of the item:
+// lane (ii, jj) takes i = ii, j = staged atom g + jj.  KIND is uniform over
+// the group: 0 = the i cluster with itself (slot order), 1 = entries that
+// count the pair (value, virial, derivative), 2 = derivative only.
+template <int P, bool GENERIC, bool TWO, bool VO, int KIND>
+__device__ __forceinline__ void eval_group(const PairArgs& A, const float4* __restrict__ sjw,
+                                           const int* __restrict__ scw, const int* __restrict__ saw, int g,
+                                           int ii, int jj, const float4& pi, int reqi, int ai,
+                                           const float4* __restrict__ posJ, const PbcArg* __restrict__ pbc,
+                                           const double* __restrict__ pos64, float band, PairAcc& acc) {
+  const float4 pj = sjw[g + jj];
+  const float dx = pj.x - pi.x;
+  const float dy = pj.y - pi.y;
+  const float dz = pj.z - pi.z;
+  const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+  bool cut = r2 <= A.dmax2;
+  if (__builtin_expect(__ballot(fabsf(r2 - A.dmax2) < band) != 0ull, 0)) {
+    const int meta = __float_as_int(pj.w);  // (entry << 3 | slot), -1 for padding
+    if (fabsf(r2 - A.dmax2) < band && reqi >= 0 && meta >= 0) {
+      const int cj = scw[meta >> 3] & kCjMask;
+      const int reqj = __float_as_int(posJ[cj * CL + (meta & 7)].w);
+      if (reqj >= 0) cut = switch_inside(A.sw, exact_r2(pbc, pos64, reqi, reqj));
+    }
+  }
+  if (KIND == 0) cut = cut && (jj != ii);  // an atom is not paired with itself
+  if constexpr (TWO) cut = cut && (saw[g + jj] != ai);
+  if constexpr (VO) {
+    float s;
+    if constexpr (GENERIC) {
+      double dfd;
+      s = (float)switch_calc_sqr(A.sw, (double)r2, dfd);
+    } else {
+      s = sw_fixed_value<P>(A.f, r2);
+    }
+    const bool c2 = KIND == 0 ? (cut && (jj > ii)) : cut;
+    acc.npair += __popcll(__ballot(c2));
+    acc.cv += c2 ? s : 0.f;
+    return;
+  }
+  float s, df;
+  if constexpr (GENERIC) {
+    double dfd;
+    double sd = switch_calc_sqr(A.sw, (double)r2, dfd);
+    s = (float)sd;
+    df = (float)dfd;
+  } else {
+    sw_fixed<P>(A.f, r2, s, df);
+  }
+  df = cut ? df : 0.f;
+  const float gx = df * dx, gy = df * dy, gz = df * dz;
+  // derivative of atom i over every partner (the full list holds both
+  // orientations of each cluster pair)
+  acc.fx += gx;
+  acc.fy += gy;
+  acc.fz += gz;
+  if (KIND == 1) {
+    // value and virial once per unordered pair
+    acc.npair += __popcll(__ballot(cut));
+    acc.cv += cut ? s : 0.f;
+    acc.vxx = fmaf(gx, dx, acc.vxx);
+    acc.vxy = fmaf(gx, dy, acc.vxy);
+    acc.vxz = fmaf(gx, dz, acc.vxz);
+    acc.vyy = fmaf(gy, dy, acc.vyy);
+    acc.vyz = fmaf(gy, dz, acc.vyz);
+    acc.vzz = fmaf(gz, dz, acc.vzz);
+  } else if (KIND == 0) {
+    const bool c2 = cut && (jj > ii);
+    const float wgt = c2 ? 1.f : 0.f;
+    acc.npair += __popcll(__ballot(c2));
+    acc.cv = fmaf(wgt, s, acc.cv);
+    const float hx = wgt * gx, hy = wgt * gy, hz = wgt * gz;
+    acc.vxx = fmaf(hx, dx, acc.vxx);
+    acc.vxy = fmaf(hx, dy, acc.vxy);
+    acc.vxz = fmaf(hx, dz, acc.vxz);
+    acc.vyy = fmaf(hy, dy, acc.vyy);
+    acc.vyz = fmaf(hy, dz, acc.vyz);
+    acc.vzz = fmaf(hz, dz, acc.vzz);
+  }
+}
+
+// One wave per work item (i cluster, chunk of <= CH list entries); a grid
+// smaller than the item count makes the waves loop over items.
+//  1. lane-parallel pruning of the chunk's entries against the current
+//     bounding boxes (D_MAX);
+//  2. each passing lane loads the 8 atoms of its j cluster, adds the image
+//     offset and keeps the atoms within D_MAX of the i cluster's box; the
+//     kept atoms are staged in LDS, compacted and grouped by entry kind
+//     (self / counting / derivative-only), each kind padded to a multiple
+//     of 8 with far-away atoms, so that every group of 8 has one kind;
+//  3. dense loop over the groups: lane l evaluates the pair
+//     (i = l&7 of cluster ci, j = staged atom 8g + (l>>3)).
+// VO (value only): no derivatives and no virial; derivative-only entries
+// are skipped, so the loop visits each unordered pair once.
+template <int P, bool GENERIC, bool TWO, bool VO, int CH>
+__global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __restrict__ items,
+                                                       const int* __restrict__ estart,
+                                                       const Entry* __restrict__ ents,
+                                                       const float4* __restrict__ posI,
+                                                       const float4* __restrict__ posJ,
+                                                       const float4* __restrict__ bbI, const float4* __restrict__ bbJ,
+                                                       const int* __restrict__ absI, const int* __restrict__ absJ,
+                                                       const double* __restrict__ pos64,
+                                                       double* __restrict__ fpart, double* __restrict__ partial) {
+  constexpr int NW = PAIR_BLOCK / WAVE;
+  constexpr int CAP = CH * CL + 3 * CL;
+  __shared__ float4 sj[NW][CAP];
+  __shared__ int scode[NW][CH];
+  __shared__ int sabs[TWO ? NW : 1][TWO ? CAP : 1];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int ii = lane & 7, jj = lane >> 3;
+  const int nitems = A.nitems_dev ? min(*A.nitems_dev, A.items_cap) : A.nitems;
+  const float band = 4.0e-6f * A.dmax2;
+  float4* sjw = sj[w];
+  int* scw = scode[w];
+  int* saw = TWO ? sabs[w] : nullptr;
+
+  for (int item = blockIdx.x * NW + w; item < nitems; item += gridDim.x * NW) {
+    const int2 it = items[item];
+    const int ci = it.x;
+    const int e0 = estart[ci] + it.y * CH;
+    const int e1 = min(min(estart[ci + 1], e0 + CH), A.ent_cap);
+    const float4 pi = posI[ci * CL + ii];
+    const int reqi = __float_as_int(pi.w);
+    const int ai = TWO ? absI[ci * CL + ii] : 0;
+    const float4 ic = bbI[2 * ci], ih = bbI[2 * ci + 1];
+    const PbcArg* pbc = A.pbc + nsys_of(A.cl_sys, ci);
+
+    // 1. prune the chunk's entries
+    Entry my = {0, 0.f, 0.f, 0.f};
+    bool pass = false;
+    {
+      const int e = e0 + lane;
+      if (lane < CH && e < e1) {
+        my = ents[e];
+        if (!VO || (my.code & kCountBit) || my.code < 0) {
+          const int cj = my.code & kCjMask;
+          const float4 jc = bbJ[2 * cj], jh = bbJ[2 * cj + 1];
+          const float gx = fmaxf(fabsf(jc.x + my.dx - ic.x) - ih.x - jh.x, 0.f);
+          const float gy = fmaxf(fabsf(jc.y + my.dy - ic.y) - ih.y - jh.y, 0.f);
+          const float gz = fmaxf(fabsf(jc.z + my.dz - ic.z) - ih.z - jh.z, 0.f);
+          pass = gx * gx + gy * gy + gz * gz <= A.prune2;
+        }
+      }
+    }
+    // 2. keep the j atoms within reach of the i box, stage them by kind
+    float4 q[CL];
+    int m8 = 0, kind = 3;
+    if (pass) {
+      const int cj = my.code & kCjMask;
+#pragma unroll
+      for (int t = 0; t < CL; t++) {
+        const float4 v = posJ[cj * CL + t];
+        q[t] = make_float4(v.x + my.dx, v.y + my.dy, v.z + my.dz, 0.f);
+      }
+      if (my.code < 0) {
+        kind = 0;
+        m8 = 0xFF;  // the i cluster with itself keeps slot order
+      } else {
+        kind = (my.code & kCountBit) ? 1 : 2;
+#pragma unroll
+        for (int t = 0; t < CL; t++) {
+          const float gx = fmaxf(fabsf(q[t].x - ic.x) - ih.x, 0.f);
+          const float gy = fmaxf(fabsf(q[t].y - ic.y) - ih.y, 0.f);
+          const float gz = fmaxf(fabsf(q[t].z - ic.z) - ih.z, 0.f);
+          if (gx * gx + gy * gy + gz * gz <= A.prune2) m8 |= 1 << t;
+        }
+      }
+    }
+    const int cnt = __popc(m8);
+    const int n0 = __ballot(kind == 0) ? CL : 0;  // at most one self entry per item
+    int tot1, tot2 = 0;
+    const int o1 = wave_excl_scan(kind == 1 ? cnt : 0, lane, tot1);
+    int o2 = 0;
+    if constexpr (!VO) o2 = wave_excl_scan(kind == 2 ? cnt : 0, lane, tot2);
+    const int n1 = (tot1 + CL - 1) & ~(CL - 1), n2 = (tot2 + CL - 1) & ~(CL - 1);
+    const int b1 = n0, b2 = n0 + n1;
+    if (pass) {
+      scw[lane] = my.code;
+      const int cj = my.code & kCjMask;
+      int p = kind == 0 ? 0 : (kind == 1 ? b1 + o1 : b2 + o2);
+#pragma unroll
+      for (int t = 0; t < CL; t++) {
+        if ((m8 >> t) & 1) {
+          sjw[p] = make_float4(q[t].x, q[t].y, q[t].z, __int_as_float((lane << 3) | t));
+          if constexpr (TWO) saw[p] = absJ[cj * CL + t];
+          p++;
+        }
+      }
+    }
+    // padding of each kind up to a multiple of 8: far away, never within D_MAX
+    if (lane < CL) {
+      const float4 far = make_float4(1.0e5f, 1.0e5f, 1.0e5f, __int_as_float(-1));
+      if (tot1 + lane < n1) {
+        sjw[b1 + tot1 + lane] = far;
+        if constexpr (TWO) saw[b1 + tot1 + lane] = -1;
+      }
+      if (tot2 + lane < n2) {
+        sjw[b2 + tot2 + lane] = far;
+        if constexpr (TWO) saw[b2 + tot2 + lane] = -1;
+      }
+    }
+    // the staging above is read by other lanes of this wave only
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+
+    PairAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0};
+    if (n0)
+      eval_group<P, GENERIC, TWO, VO, 0>(A, sjw, scw, saw, 0, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band, acc);
+    for (int g = b1; g < b1 + n1; g += CL)
+      eval_group<P, GENERIC, TWO, VO, 1>(A, sjw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band, acc);
+    if constexpr (!VO) {
+      for (int g = b2; g < b2 + n2; g += CL)
+        eval_group<P, GENERIC, TWO, VO, 2>(A, sjw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band, acc);
+    }
+    // derivative of atom i: -sum over the 8 lanes sharing i (lanes ii + 8k);
+    // chunks of one cluster are combined by k_gather in a fixed order
+    if constexpr (!VO) {
+      double dfx = acc.fx, dfy = acc.fy, dfz = acc.fz;
+      for (int o = 8; o < 64; o <<= 1) {
+        dfx += __shfl_xor(dfx, o, 64);
+        dfy += __shfl_xor(dfy, o, 64);
+        dfz += __shfl_xor(dfz, o, 64);
+      }
+      if (jj == 0) {
+        double* fp = fpart + (size_t)item * (CL * 3) + ii * 3;
+        fp[0] = -dfx;
+        fp[1] = -dfy;
+        fp[2] = -dfz;
+      }
+    }
+    double r[7] = {acc.cv, acc.vxx, acc.vxy, acc.vxz, acc.vyy, acc.vyz, acc.vzz};
+    for (int q2 = 0; q2 < 7; q2++) r[q2] = wave_sum(r[q2]);
+    if (lane == 0) {
+      double* out = partial + (size_t)item * 8;
+      for (int q2 = 0; q2 < 7; q2++) out[q2] = r[q2];
+      out[7] = (double)acc.npair;
+    }
+    // the next item overwrites the staging read above
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// derivatives of the atoms of every i cluster: sum of its chunks' partials
+__global__ void k_gather(int ncl, const int* __restrict__ ncl_dev, const float4* __restrict__ p,
+                         const int* __restrict__ item_start, int items_cap, const double* __restrict__ fpart,
+                         double* __restrict__ deriv) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ncl_dev) ncl = *ncl_dev;
+  if (t >= ncl * CL * 3) return;
+  const int c = t / (CL * 3), r = t % (CL * 3);
+  const int req = __float_as_int(p[c * CL + r / 3].w);
+  if (req < 0) return;
+  double sum = 0.0;
+  const int i1 = min(item_start[c + 1], items_cap);
+  for (int it = item_start[c]; it < i1; it++) sum += fpart[(size_t)it * (CL * 3) + r];
+  deriv[3 * (size_t)req + r % 3] = sum;
+}
+
+// deterministic sum of the per-item partials -> value, virial, pair count.
+// Stage 1: 128 blocks reduce contiguous ranges; stage 2: one block.
+constexpr int kRedBlocks = 128;
+__global__ __launch_bounds__(256) void k_reduce1(int nitems, const double* __restrict__ partial,
+                                                 double* __restrict__ out) {
+  __shared__ double s[8][256];
+  const int t = threadIdx.x;
+  const int per = (nitems + gridDim.x - 1) / gridDim.x;
+  const int b = blockIdx.x * per, e = min(nitems, b + per);
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const double4* p4 = reinterpret_cast<const double4*>(partial);
+  for (int i = b + t; i < e; i += 256) {
+    double4 x = p4[2 * (size_t)i], y = p4[2 * (size_t)i + 1];
+    acc[0] += x.x; acc[1] += x.y; acc[2] += x.z; acc[3] += x.w;
+    acc[4] += y.x; acc[5] += y.y; acc[6] += y.z; acc[7] += y.w;
+  }
+  for (int q = 0; q < 8; q++) s[q][t] = acc[q];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o)
+      for (int q = 0; q < 8; q++) s[q][t] += s[q][t + o];
+    __syncthreads();
+  }
+  if (t < 8) out[(size_t)blockIdx.x * 8 + t] = s[t][0];
+}
+
+// virial = -sum dd (x) d (CoordinationBase.cpp:200-206), symmetric
+__device__ __forceinline__ void write_virial(double* virial, const double* s6) {
+  double xx = -s6[0], xy = -s6[1], xz = -s6[2], yy = -s6[3], yz = -s6[4], zz = -s6[5];
+  virial[0] = xx; virial[1] = xy; virial[2] = xz;
+  virial[3] = xy; virial[4] = yy; virial[5] = yz;
+  virial[6] = xz; virial[7] = yz; virial[8] = zz;
+}
+
+__global__ void k_finalize(int nitems, const double* __restrict__ partial, double* __restrict__ value,
+                           double* __restrict__ virial, double* __restrict__ stats_out) {
+  __shared__ double s[8][128];
+  const int t = threadIdx.x;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = t; i < nitems; i += 128)
+    for (int q = 0; q < 8; q++) acc[q] += partial[(size_t)i * 8 + q];
+  for (int q = 0; q < 8; q++) s[q][t] = acc[q];
+  __syncthreads();
+  for (int o = 64; o > 0; o >>= 1) {
+    if (t < o)
+      for (int q = 0; q < 8; q++) s[q][t] += s[q][t + o];
+    __syncthreads();
+  }
+  if (t == 0) {
+    if (value) *value = s[0][0];
+    if (virial) {
+      double v6[6] = {s[1][0], s[2][0], s[3][0], s[4][0], s[5][0], s[6][0]};
+      write_virial(virial, v6);
+    }
+    stats_out[0] = s[7][0];
+    stats_out[1] += s[7][0];  // running total since the last reset
+  }
+}
+
+// per-system sums of the item partials: one workgroup per system; the items
+// of system s are those of its clusters, contiguous after its columns
+__global__ __launch_bounds__(256) void k_sys_reduce(int nsys, const int* __restrict__ sys_colbase,
+                                                    const int* __restrict__ col_cl,
+                                                    const int* __restrict__ item_start, int items_cap,
+                                                    const double* __restrict__ partial, double* __restrict__ values,
+                                                    double* __restrict__ virials, double* __restrict__ npairs) {
+  __shared__ double s[8][256];
+  const int sy = blockIdx.x;
+  if (sy >= nsys) return;
+  const int t = threadIdx.x;
+  const int c0 = col_cl[sys_colbase[sy]], c1 = col_cl[sys_colbase[sy + 1]];
+  const int i0 = min(item_start[c0], items_cap), i1 = min(item_start[c1], items_cap);
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = i0 + t; i < i1; i += 256)
+    for (int q = 0; q < 8; q++) acc[q] += partial[(size_t)i * 8 + q];
+  for (int q = 0; q < 8; q++) s[q][t] = acc[q];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o)
+      for (int q = 0; q < 8; q++) s[q][t] += s[q][t + o];
+    __syncthreads();
+  }
+  if (t == 0) {
+    values[sy] = s[0][0];
+    double v6[6] = {s[1][0], s[2][0], s[3][0], s[4][0], s[5][0], s[6][0]};
+    write_virial(virials + 9 * (size_t)sy, v6);
+    npairs[sy] = s[7][0];
+  }
+}
+
+// ------------------------------------------------------------------------
+// FP64 fallback kernels
+// ------------------------------------------------------------------------
+// Every requested atom accumulates its own derivative from all its partners
+// (no atomics); the value and the virial are taken once per pair.
+__global__ void k_allpairs(int nA, int nB, int use_pbc, PbcArg P, p2a_switch sw, const double* __restrict__ pos,
+                           const long long* __restrict__ absidx, double* __restrict__ deriv,
+                           double* __restrict__ partial) {
+  const int natoms = nA + nB;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double cv = 0, v[6] = {0, 0, 0, 0, 0, 0}, f[3] = {0, 0, 0};
+  double np = 0;
+  if (i < natoms) {
+    const bool two = nB > 0;
+    int j0, j1;
+    if (two) {
+      if (i < nA) { j0 = nA; j1 = natoms; } else { j0 = 0; j1 = nA; }
+    } else {
+      j0 = 0;
+      j1 = nA;
+    }
+    const double* xi = pos + 3 * (size_t)i;
+    for (int j = j0; j < j1; j++) {
+      if (j == i || absidx[i] == absidx[j]) continue;
+      const double* xj = pos + 3 * (size_t)j;
+      // pair (p,q) is oriented p -> q with p in GROUPA (two groups) or p<q
+      bool own = two ? (i < nA) : (i < j);
+      double d[3];
+      if (use_pbc) pbc_distance(P, xi, xj, d);
+      else { d[0] = xj[0] - xi[0]; d[1] = xj[1] - xi[1]; d[2] = xj[2] - xi[2]; }
+      double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      double df;
+      double s = switch_calc_sqr(sw, r2, df);
+      // deriv[i] -= df*d for d = x_j - x_i, the same for either orientation
+      f[0] -= df * d[0];
+      f[1] -= df * d[1];
+      f[2] -= df * d[2];
+      if (own) {
+        cv += s;
+        if (switch_inside(sw, r2)) np += 1;
+        v[0] -= df * d[0] * d[0];
+        v[1] -= df * d[0] * d[1];
+        v[2] -= df * d[0] * d[2];
+        v[3] -= df * d[1] * d[1];
+        v[4] -= df * d[1] * d[2];
+        v[5] -= df * d[2] * d[2];
+      }
+    }
+    deriv[3 * (size_t)i + 0] = f[0];
+    deriv[3 * (size_t)i + 1] = f[1];
+    deriv[3 * (size_t)i + 2] = f[2];
+  }
+  // block partials: cv, -virial terms are stored as +sum g (x) d for k_finalize
+  double r[8] = {cv, -v[0], -v[1], -v[2], -v[3], -v[4], -v[5], np};
+  const int lane = threadIdx.x & 63;
+  for (int q = 0; q < 8; q++) r[q] = wave_sum(r[q]);
+  __shared__ double red[4][8];
+  if (lane == 0)
+    for (int q = 0; q < 8; q++) red[threadIdx.x >> 6][q] = r[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 8; q++) {
+      double t = 0;
+      for (int w = 0; w < (int)(blockDim.x / 64); w++) t += red[w][q];
+      partial[(size_t)blockIdx.x * 8 + q] = t;
+    }
+  }
+}
+
+// PAIR: the k-th atom of GROUPA with the k-th of GROUPB
+__global__ void k_pairmode(int nA, int use_pbc, PbcArg P, p2a_switch sw, const double* __restrict__ pos,
+                           const long long* __restrict__ absidx, double* __restrict__ deriv,
+                           double* __restrict__ partial) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (i < nA) {
+    const int j = nA + i;
+    double* di = deriv + 3 * (size_t)i;
+    double* dj = deriv + 3 * (size_t)j;
+    di[0] = di[1] = di[2] = 0.0;
+    dj[0] = dj[1] = dj[2] = 0.0;
+    if (absidx[i] != absidx[j]) {
+      const double* xi = pos + 3 * (size_t)i;
+      const double* xj = pos + 3 * (size_t)j;
+      double d[3];
+      if (use_pbc) pbc_distance(P, xi, xj, d);
+      else { d[0] = xj[0] - xi[0]; d[1] = xj[1] - xi[1]; d[2] = xj[2] - xi[2]; }
+      double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      double df;
+      double s = switch_calc_sqr(sw, r2, df);
+      for (int k = 0; k < 3; k++) {
+        di[k] = -df * d[k];
+        dj[k] = df * d[k];
+      }
+      r[0] = s;
+      r[1] = df * d[0] * d[0];
+      r[2] = df * d[0] * d[1];
+      r[3] = df * d[0] * d[2];
+      r[4] = df * d[1] * d[1];
+      r[5] = df * d[1] * d[2];
+      r[6] = df * d[2] * d[2];
+      r[7] = switch_inside(sw, r2) ? 1.0 : 0.0;
+    }
+  }
+  const int lane = threadIdx.x & 63;
+  for (int q = 0; q < 8; q++) r[q] = wave_sum(r[q]);
+  __shared__ double red[4][8];
+  if (lane == 0)
+    for (int q = 0; q < 8; q++) red[threadIdx.x >> 6][q] = r[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < 8; q++) {
+      double t = 0;
+      for (int w = 0; w < (int)(blockDim.x / 64); w++) t += red[w][q];
+      partial[(size_t)blockIdx.x * 8 + q] = t;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// host helpers shared by the single-system and batch contexts
+// ------------------------------------------------------------------------
+inline int grid1(size_t n, int b) { return (int)((n + b - 1) / b); }
+
+inline int launch_check(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(P2A_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return P2A_OK;
+}
+
+inline void fill_pbc_arg(const PbcHost& h, PbcArg& P) {
+  std::memset(&P, 0, sizeof(P));
+  P.type = h.type;
+  std::memcpy(P.box, h.box, sizeof(P.box));
+  std::memcpy(P.inv, h.inv, sizeof(P.inv));
+  std::memcpy(P.red, h.reduced, sizeof(P.red));
+  std::memcpy(P.ired, h.invReduced, sizeof(P.ired));
+  std::memcpy(P.nshift, h.nshift, sizeof(P.nshift));
+  std::memcpy(P.shifts, h.shifts, sizeof(P.shifts));
+}
+
+// cell part of a SysArg (column grid and list radius are set by the caller)
+inline void sys_cell(const double* box9, double rlist, SysArg& S) {
+  PbcHost tmp;
+  pbc_setup(box9, tmp);
+  std::memcpy(S.a, box9, sizeof(S.a));
+  std::memcpy(S.inv, tmp.inv, sizeof(S.inv));
+  for (int k = 0; k < 9; k++) S.af[k] = (float)box9[k];
+  double w[3];
+  pbc_widths(box9, w);
+  for (int k = 0; k < 3; k++) S.rb[k] = (float)(rlist / w[k]);
+}
+
+// columns of a group of n atoms in a cell: side ~ cbrt(8/rho), bounded count
+inline void sys_columns(const double* box9, int n, SysArg& S) {
+  double w[3];
+  pbc_widths(box9, w);
+  double vol = std::fabs(w[0] * w[1] * w[2]);
+  double rho = std::max(n, 1) / vol;
+  double side = std::cbrt((double)CL / rho);
+  S.ncx = std::max(1, (int)std::floor(w[0] / side));
+  S.ncy = std::max(1, (int)std::floor(w[1] / side));
+  // keep the column count bounded for very dilute groups
+  while ((int64_t)S.ncx * S.ncy > 4 * (int64_t)n + 16) {
+    S.ncx = std::max(1, S.ncx / 2);
+    S.ncy = std::max(1, S.ncy / 2);
+  }
+}
+
+// the chunk size (entries per work item) and the grid of the pair kernel
+struct PairTuning {
+  int chunk = 64;        // 32 or 64
+  int blocks_per_cu = 0; // 0: one wave per item; > 0: persistent grid of that many blocks per CU
+};
+
+// launch of k_pairs for a switch, group mode and tuning
+inline void launch_pairs(const PairArgs& A, int grid, hipStream_t s, bool value_only, int chunk,
+                         const int2* items, const int* estart, const Entry* ents, const float4* posI,
+                         const float4* posJ, const float4* bbI, const float4* bbJ, const int* absI, const int* absJ,
+                         const double* pos64, double* fpart, double* partial) {
+  const p2a_switch& sw = A.sw;
+#define P2A_K(P, GEN, TWO, VO, CH)                                                                              \
+  hipLaunchKernelGGL((k_pairs<P, GEN, TWO, VO, CH>), dim3(grid), dim3(PAIR_BLOCK), 0, s, A, items, estart, ents, \
+                     posI, posJ, bbI, bbJ, absI, absJ, pos64, fpart, partial)
+#define P2A_KCH(P, GEN, TWO, VO)          \
+  do {                                    \
+    if (chunk == 32) P2A_K(P, GEN, TWO, VO, 32); \
+    else P2A_K(P, GEN, TWO, VO, 64);      \
+  } while (0)
+#define P2A_KSW(P, GEN)                                     \
+  do {                                                      \
+    if (A.twogroup) {                                       \
+      if (value_only) P2A_KCH(P, GEN, true, true);          \
+      else P2A_KCH(P, GEN, true, false);                    \
+    } else {                                                \
+      if (value_only) P2A_KCH(P, GEN, false, true);         \
+      else P2A_KCH(P, GEN, false, false);                   \
+    }                                                       \
+  } while (0)
+  if (sw.type == P2A_SW_RATIONAL_FIXED) {
+    switch (sw.nn / 2) {
+      case 1: P2A_KSW(1, false); break;
+      case 2: P2A_KSW(2, false); break;
+      case 3: P2A_KSW(3, false); break;
+      case 4: P2A_KSW(4, false); break;
+      case 5: P2A_KSW(5, false); break;
+      case 6: P2A_KSW(6, false); break;
+      default: P2A_KSW(1, true); break;
+    }
+  } else {
+    P2A_KSW(1, true);
+  }
+#undef P2A_KSW
+#undef P2A_KCH
+#undef P2A_K
+}
+
+// FP32 switch constants and cut parameters of the pair kernel
+inline void pair_args_switch(const p2a_switch& sw, PairArgs& A) {
+  A.dmax2 = (float)sw.dmax_2;
+  double pr = sw.dmax + 1e-4;
+  A.prune2 = (float)(pr * pr);
+  A.f.dmax2 = (float)sw.dmax_2;
+  A.f.invr0_2 = (float)sw.invr0_2;
+  A.f.stretch = (float)sw.stretch;
+  A.f.shift = (float)sw.shift;
+  A.f.cder = (float)(-(double)sw.nn * sw.invr0_2 * sw.stretch);
+  A.sw = sw;
+}
+
+}  // namespace
+
+}  // namespace p2a

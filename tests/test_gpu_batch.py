@@ -1,0 +1,161 @@
+"""Batched COORDINATION (many systems per step) against the FP64 oracle.
+
+Each system is one single-group COORDINATION of its own triclinic cell,
+SWITCH={RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2}, list radius 1.2 nm (the
+reference's NLISTCELLS NL_CUTOFF=1.2 NL_STRIDE=1).  Per system: CV within
+1e-6 relative, derivatives within 1e-5 max|F|, virial within 1e-5 max|W|,
+pair count exact.  Graph replays must equal the eager step bit for bit; the
+pair-kernel decomposition (chunk, persistent grid) must not change results
+beyond the FP32 tolerance.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SWITCH = "RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2"
+CV_TOL, F_TOL, W_TOL = 1e-6, 1e-5, 1e-5
+
+
+def small_batch(seed=11, nsys=6):
+    from plumed2_amd.synthetic import batch_systems
+    return batch_systems(nsys=nsys, n_lo=2000, n_hi=6000, seed=seed)
+
+
+def run(P, torch, natoms, boxes, x, graph=False, tuning=None):
+    b = P.CoordinationBatch(natoms, SWITCH, 1.2)
+    b.set_boxes(boxes)
+    if tuning:
+        b.set_tuning(*tuning)
+    b.set_graph(graph)
+    xt = torch.tensor(x, dtype=torch.float64, device="cuda")
+    v, d, w = b.calculate_torch(xt)
+    torch.cuda.synchronize()
+    return b, xt, v.cpu().numpy(), d.cpu().numpy(), w.cpu().numpy()
+
+
+def check_system(k, natoms, boxes, x, v, d, w, counts):
+    o = np.concatenate([[0], np.cumsum(natoms)])
+    xs = x[o[k]:o[k + 1]]
+    ov, od, ow, npair = O.coordination(O.Switch(SWITCH), boxes[k], xs, int(natoms[k]), 0)
+    assert abs(v[k] - ov) <= CV_TOL * abs(ov), (k, v[k], ov)
+    fmax = np.max(np.abs(od))
+    assert np.max(np.abs(d[o[k]:o[k + 1]] - od)) <= F_TOL * fmax, k
+    wmax = np.max(np.abs(ow))
+    assert np.max(np.abs(w[k] - ow)) <= W_TOL * wmax, k
+    assert counts[k] == npair, (k, counts[k], npair)
+
+
+def test_small_triclinic_batch_matches_oracle():
+    import torch
+
+    import plumed2_amd as P
+    natoms, boxes, x = small_batch()
+    b, _, v, d, w = run(P, torch, natoms, boxes, x)
+    counts = b.pair_counts()
+    for k in range(len(natoms)):
+        check_system(k, natoms, boxes, x, v, d, w, counts)
+    st = b.stats()
+    assert st["contributing_pairs"] == counts.sum()
+    b.close()
+
+
+def test_graph_replay_equals_eager():
+    import torch
+
+    import plumed2_amd as P
+    natoms, boxes, x = small_batch(seed=12)
+    b0, _, v0, d0, w0 = run(P, torch, natoms, boxes, x)
+    b0.close()
+    b, xt, v, d, w = run(P, torch, natoms, boxes, x, graph=True)
+    assert b.stats()["graph"] == 1
+    np.testing.assert_array_equal(v, v0)
+    np.testing.assert_array_equal(d, d0)
+    np.testing.assert_array_equal(w, w0)
+    # replays on new coordinates written into the same buffer, with eager
+    # work and synchronisations between them
+    rng = np.random.default_rng(3)
+    for rep in range(3):
+        x2 = x + rng.normal(0.0, 0.01, x.shape)
+        xt.copy_(torch.tensor(x2, dtype=torch.float64, device="cuda"))
+        torch.cuda.synchronize()
+        out = (torch.empty(len(natoms), dtype=torch.float64, device="cuda"), torch.empty_like(xt),
+               torch.empty((len(natoms), 3, 3), dtype=torch.float64, device="cuda"))
+        vg, dg, wg = b.calculate_torch(xt, out)
+        _ = (xt * 2.0).sum().item()
+        be = P.CoordinationBatch(natoms, SWITCH, 1.2)
+        be.set_boxes(boxes)
+        ve, de, we = be.calculate_torch(xt)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(vg.cpu().numpy(), ve.cpu().numpy())
+        np.testing.assert_array_equal(dg.cpu().numpy(), de.cpu().numpy())
+        np.testing.assert_array_equal(wg.cpu().numpy(), we.cpu().numpy())
+        be.close()
+    b.close()
+
+
+@pytest.mark.parametrize("tuning", [(32, 0), (64, 4), (32, 2)])
+def test_decomposition_does_not_change_results(tuning):
+    import torch
+
+    import plumed2_amd as P
+    natoms, boxes, x = small_batch(seed=13, nsys=4)
+    b0, _, v0, d0, w0 = run(P, torch, natoms, boxes, x)
+    c0 = b0.pair_counts()
+    b0.close()
+    b, _, v, d, w = run(P, torch, natoms, boxes, x, tuning=tuning)
+    np.testing.assert_array_equal(b.pair_counts(), c0)
+    np.testing.assert_allclose(v, v0, rtol=1e-6)
+    assert np.max(np.abs(d - d0)) <= 1e-5 * np.max(np.abs(d0))
+    b.close()
+
+
+def test_autotune_picks_a_candidate_and_keeps_results():
+    import torch
+
+    import plumed2_amd as P
+    natoms, boxes, x = small_batch(seed=14, nsys=5)
+    b, xt, v0, _, _ = run(P, torch, natoms, boxes, x)
+    ch, bpc = b.autotune(xt.data_ptr(), reps=1)
+    assert ch in (32, 64) and bpc in (0, 2, 4, 8)
+    st = b.stats()
+    assert (st["chunk"], st["blocks_per_cu"]) == (ch, bpc)
+    v, _, _ = b.calculate_torch(xt)
+    np.testing.assert_allclose(v.cpu().numpy(), v0, rtol=1e-6)
+    b.close()
+
+
+def test_narrow_cell_is_refused():
+    import plumed2_amd as P
+    b = P.CoordinationBatch([3000, 3000], SWITCH, 1.2)
+    boxes = np.array([np.diag([5.0, 5.0, 5.0]), np.diag([2.3, 10.0, 10.0])])
+    with pytest.raises(P.Plumed2AmdError, match="system 1"):
+        b.set_boxes(boxes)
+    b.close()
+
+
+def test_full_batch_256_systems():
+    """256 systems, 2,000-64,000 atoms each (seed 4): per-system translation
+    invariance (sum of derivatives within 1e-5 max|F| sqrt(N)), symmetric
+    virials, and the smallest systems against the oracle."""
+    import torch
+
+    import plumed2_amd as P
+    from plumed2_amd.synthetic import batch_systems
+    natoms, boxes, x = batch_systems()
+    b, _, v, d, w = run(P, torch, natoms, boxes, x, graph=True)
+    counts = b.pair_counts()
+    o = np.concatenate([[0], np.cumsum(natoms)])
+    for k in range(len(natoms)):
+        dk = d[o[k]:o[k + 1]]
+        # each pair force is evaluated in FP32 in both orientations: the sum
+        # carries random rounding that grows like sqrt(N); a missing pair would
+        # leave an imbalance of the size of one pair force
+        assert np.max(np.abs(dk.sum(0))) <= 1e-5 * np.max(np.abs(dk)) * np.sqrt(natoms[k])
+        np.testing.assert_allclose(w[k], w[k].T, rtol=0, atol=1e-12 * np.max(np.abs(w[k])))
+        assert counts[k] > 0
+    for k in np.argsort(natoms)[:3]:
+        check_system(int(k), natoms, boxes, x, v, d, w, counts)
+    b.close()

@@ -286,3 +286,26 @@ def test_errors_are_raised():
     act = P.Coordination("c: COORDINATION GROUPA=1-10 R_0=1")
     with pytest.raises(ValueError):
         act.calculate(np.zeros((9, 3)), np.diag([3.0] * 3))
+
+
+@pytest.mark.parametrize("chunk,bpc", [(32, 0), (64, 2), (32, 8)])
+def test_pair_kernel_decomposition(chunk, bpc):
+    """Entries per wave and persistent grid change the work split, not the
+    pairs: exact pair count, CV and forces within the FP32 tolerance."""
+    P = _P()
+    x, L = uniform(20000, seed=21)
+    box = np.diag([L] * 3)
+    defn = "RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2"
+    line = f"c: COORDINATION GROUPA=1-20000 SWITCH={{{defn}}} NLISTCELLS NL_CUTOFF=1.5 NL_STRIDE=10"
+    a0 = P.Coordination(line)
+    v0, d0, w0 = a0.calculate(x, box, 0)
+    n0 = a0.stats()["contributing_pairs"]
+    a = P.Coordination(line)
+    a.set_tuning(chunk, bpc)
+    v, d, w = a.calculate(x, box, 0)
+    assert a.stats()["contributing_pairs"] == n0
+    ov, od, ow, npair = O.coordination(O.Switch(defn), box, x, 20000, 0)
+    assert npair == n0
+    check(v, d, w, ov, od, ow)
+    a.close()
+    a0.close()
