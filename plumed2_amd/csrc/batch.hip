@@ -54,7 +54,7 @@ struct p2a_batch {
   p2a::DBuf<double4> origin;
   p2a::DBuf<float4> bb0, bb, p;
   p2a::DBuf<float2> kr;
-  p2a::DBuf<int> ecount, estart, istart, totals;
+  p2a::DBuf<int> ecount, estart, istart, totals, redo;
   p2a::DBuf<p2a::Entry> ents;
   p2a::DBuf<int2> items;
   p2a::DBuf<double> fpart, partial, npairs, out_values, out_virials, out_deriv;
@@ -77,7 +77,8 @@ static int batch_reserve_lists(p2a_batch* b, int ent_cap, int items_cap) {
   int rc;
   if ((rc = b->ents.reserve(std::max(ent_cap, 1))) || (rc = b->items.reserve(std::max(items_cap, 1))) ||
       (rc = b->fpart.reserve((size_t)CL * 3 * std::max(items_cap, 1))) ||
-      (rc = b->partial.reserve(8 * (size_t)std::max(items_cap, 1))))
+      (rc = b->partial.reserve(8 * (size_t)std::max(items_cap, 1))) ||
+      (rc = b->redo.reserve(std::max(items_cap, 1))))
     return rc;
   b->ent_cap = ent_cap;
   b->items_cap = items_cap;
@@ -129,7 +130,9 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
                      b->items.p, chunk, b->items_cap);
   hipLaunchKernelGGL(k_pack, dim3(grid1(nslot, 256)), dim3(256), 0, s, (int)nslot, ncl_dev, b->slot_loc.p, b->info.p,
                      pos, 0, (const SysArg*)b->sys.p, (const int*)b->atom_sys.p, b->origin.p, b->p.p);
-  hipLaunchKernelGGL(k_bbox, dim3(grid1(clc, 256)), dim3(256), 0, s, clc, ncl_dev, b->p.p, b->bb.p);
+  // totals[4..5] serve as the pair kernels' redo counters
+  hipLaunchKernelGGL(k_bbox, dim3(grid1(std::max(clc, 2), 256)), dim3(256), 0, s, clc, ncl_dev, b->p.p, b->bb.p,
+                     b->totals.p + 4);
   int rc;
   if ((rc = launch_check("batch: list build"))) return rc;
   PairArgs A;
@@ -142,9 +145,11 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
   A.twogroup = 0;
   A.pbc = b->pbc.p;
   A.cl_sys = b->cl_sys.p;
+  A.redo = b->redo.p;
+  A.redo_count = b->totals.p + 4;
   int grid = grid1(std::max(b->items_cap, 1), PAIR_BLOCK / WAVE);
   if (b->tune.blocks_per_cu > 0) grid = std::min(grid, b->tune.blocks_per_cu * b->cu_count);
-  launch_pairs(A, grid, s, false, chunk, b->items.p, b->estart.p, b->ents.p, b->p.p, b->p.p, b->bb.p, b->bb.p,
+  launch_pairs(A, grid, std::min(grid, b->cu_count), s, false, chunk, b->items.p, b->estart.p, b->ents.p, b->p.p, b->p.p, b->bb.p, b->bb.p,
                (const int*)nullptr, (const int*)nullptr, pos, b->fpart.p, b->partial.p);
   if ((rc = launch_check("batch: pair kernel"))) return rc;
   hipLaunchKernelGGL(k_gather, dim3(grid1(nslot * 3, 256)), dim3(256), 0, s, clc, ncl_dev, b->p.p, b->istart.p,

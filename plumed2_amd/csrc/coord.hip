@@ -23,6 +23,7 @@ struct Dir {  // one cluster-pair list: i clusters of group I against group J
   DBuf<Entry> ents;
   DBuf<int2> items;
   DBuf<double> fpart;  // per item: derivative partials of the 8 i atoms
+  DBuf<int> redo;      // items redone with FP64 cut decisions
   int nitems = 0, item_offset = 0;
   int64_t nentries = 0;
 };
@@ -64,6 +65,7 @@ struct p2a_coord {
   p2a::PairTuning tune;
   int cu_count = 256;
   p2a::DBuf<double> partial, red, stats, pos_buf, deriv_buf, out_buf;
+  p2a::DBuf<int> redo_count;  // one counter per direction
   p2a::DBuf<long long> abs_dev;
   int64_t rebuilds = 0;
   bool profiling = false;
@@ -208,7 +210,7 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     c->nitems += D.nitems;
     tot_entries += D.nentries;
     if ((rc = D.ents.reserve(std::max<int64_t>(D.nentries, 1))) || (rc = D.items.reserve(std::max(D.nitems, 1))) ||
-        (rc = D.fpart.reserve((size_t)CL * 3 * std::max(D.nitems, 1))))
+        (rc = D.fpart.reserve((size_t)CL * 3 * std::max(D.nitems, 1))) || (rc = D.redo.reserve(std::max(D.nitems, 1))))
       return rc;
     Ls[d].ent_cap = (int)D.ents.n;
     if (GI.ncl > 0) {
@@ -221,7 +223,7 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     }
   }
   c->nentries = tot_entries;
-  if ((rc = c->partial.reserve(8 * (size_t)std::max(c->nitems, 1)))) return rc;
+  if ((rc = c->partial.reserve(8 * (size_t)std::max(c->nitems, 1))) || (rc = c->redo_count.reserve(2))) return rc;
   // absolute index per slot (two groups: pairs of one atom with itself are skipped)
   if (ngroups == 2) {
     std::vector<int> loc[2];
@@ -271,7 +273,8 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     hipLaunchKernelGGL(k_pack, dim3(grid1((size_t)G.ncl * CL, 256)), dim3(256), 0, s, G.ncl * CL,
                        (const int*)nullptr, G.slot_loc.p, G.info.p, pos_dev, G.off, (const SysArg*)G.sys.p,
                        (const int*)nullptr, G.origin.p, G.p.p);
-    hipLaunchKernelGGL(k_bbox, dim3(grid1(G.ncl, 256)), dim3(256), 0, s, G.ncl, (const int*)nullptr, G.p.p, G.bb.p);
+    hipLaunchKernelGGL(k_bbox, dim3(grid1(std::max(G.ncl, 2), 256)), dim3(256), 0, s, G.ncl, (const int*)nullptr,
+                       G.p.p, G.bb.p, c->redo_count.p);
   }
   PairArgs A;
   std::memset(&A, 0, sizeof(A));
@@ -308,7 +311,9 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     A.ent_cap = (int)D.ents.n;
     int grid = grid1(D.nitems, PAIR_BLOCK / WAVE);
     if (c->tune.blocks_per_cu > 0) grid = std::min(grid, c->tune.blocks_per_cu * c->cu_count);
-    launch_pairs(A, grid, s, value_only, c->tune.chunk, D.items.p, D.estart.p, D.ents.p, GI.p.p, GJ.p.p, GI.bb.p,
+    A.redo = D.redo.p;
+    A.redo_count = c->redo_count.p + d;
+    launch_pairs(A, grid, std::min(grid, c->cu_count), s, value_only, c->tune.chunk, D.items.p, D.estart.p, D.ents.p, GI.p.p, GJ.p.p, GI.bb.p,
                  GJ.bb.p, GI.abs_slot.p, GJ.abs_slot.p, pos_dev, D.fpart.p, c->partial.p + 8 * (size_t)D.item_offset);
   }
   if (c->profiling) {

@@ -531,10 +531,12 @@ __global__ void k_pack(int nslot, const int* __restrict__ ncl_dev, const int* __
   out[s] = make_float4((float)(w[0] - o.x), (float)(w[1] - o.y), (float)(w[2] - o.z), __int_as_float(off + a));
 }
 
-// current bounding box of each cluster in its local frame
+// current bounding box of each cluster in its local frame; also clears the
+// pair kernels' redo counters (zero2[0..1]) for the launches that follow
 __global__ void k_bbox(int ncl, const int* __restrict__ ncl_dev, const float4* __restrict__ p,
-                       float4* __restrict__ bb) {
+                       float4* __restrict__ bb, int* __restrict__ zero2) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (zero2 && c < 2) zero2[c] = 0;
   if (ncl_dev) ncl = *ncl_dev;
   if (c >= ncl) return;
   float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
@@ -622,6 +624,8 @@ struct PairArgs {
   p2a_switch sw;          // FP64 parameters (generic switch and the cut band)
   const PbcArg* pbc;      // device copies of the prepared cells (FP64 cut decisions)
   const int* cl_sys;      // system of each i cluster (null: one system)
+  int* redo;              // items that need the FP64 cut decision (capacity: items)
+  int* redo_count;        // their number; zeroed before the FP32 pass
 };
 
 // FP64 re-evaluation of the squared distance of one pair, used only when the
@@ -647,12 +651,16 @@ struct PairAcc {
 // lane (ii, jj) takes i = ii, j = staged atom g + jj.  KIND is uniform over
 // the group: 0 = the i cluster with itself (slot order), 1 = entries that
 // count the pair (value, virial, derivative), 2 = derivative only.
-template <int P, bool GENERIC, bool TWO, bool VO, int KIND>
+// A pair whose FP32 r^2 lies within `band` of D_MAX^2 needs the FP64 cut
+// decision: EXACT=false only raises `band_hit` (the item is then redone by
+// the EXACT=true kernel), EXACT=true decides it on the spot.
+template <int P, bool GENERIC, bool TWO, bool VO, int KIND, bool EXACT>
 __device__ __forceinline__ void eval_group(const PairArgs& A, const float4* __restrict__ sjw,
                                            const int* __restrict__ scw, const int* __restrict__ saw, int g,
                                            int ii, int jj, const float4& pi, int reqi, int ai,
                                            const float4* __restrict__ posJ, const PbcArg* __restrict__ pbc,
-                                           const double* __restrict__ pos64, float band, PairAcc& acc) {
+                                           const double* __restrict__ pos64, float band, bool& band_hit,
+                                           PairAcc& acc) {
   const float4 pj = sjw[g + jj];
   const float dx = pj.x - pi.x;
   const float dy = pj.y - pi.y;
@@ -660,11 +668,15 @@ __device__ __forceinline__ void eval_group(const PairArgs& A, const float4* __re
   const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
   bool cut = r2 <= A.dmax2;
   if (__builtin_expect(__ballot(fabsf(r2 - A.dmax2) < band) != 0ull, 0)) {
-    const int meta = __float_as_int(pj.w);  // (entry << 3 | slot), -1 for padding
-    if (fabsf(r2 - A.dmax2) < band && reqi >= 0 && meta >= 0) {
-      const int cj = scw[meta >> 3] & kCjMask;
-      const int reqj = __float_as_int(posJ[cj * CL + (meta & 7)].w);
-      if (reqj >= 0) cut = switch_inside(A.sw, exact_r2(pbc, pos64, reqi, reqj));
+    if constexpr (EXACT) {
+      const int meta = __float_as_int(pj.w);  // (entry << 3 | slot), -1 for padding
+      if (fabsf(r2 - A.dmax2) < band && reqi >= 0 && meta >= 0) {
+        const int cj = scw[meta >> 3] & kCjMask;
+        const int reqj = __float_as_int(posJ[cj * CL + (meta & 7)].w);
+        if (reqj >= 0) cut = switch_inside(A.sw, exact_r2(pbc, pos64, reqi, reqj));
+      }
+    } else {
+      band_hit = true;
     }
   }
   if (KIND == 0) cut = cut && (jj != ii);  // an atom is not paired with itself
@@ -736,7 +748,11 @@ __device__ __forceinline__ void eval_group(const PairArgs& A, const float4* __re
 //     (i = l&7 of cluster ci, j = staged atom 8g + (l>>3)).
 // VO (value only): no derivatives and no virial; derivative-only entries
 // are skipped, so the loop visits each unordered pair once.
-template <int P, bool GENERIC, bool TWO, bool VO, int CH>
+// EXACT=false is the FP32 pass over every item; an item with a pair in the
+// band around D_MAX^2 writes nothing and is appended to A.redo.  EXACT=true
+// redoes those items with the FP64 cut decision inline (it needs many more
+// registers, which is why it is a separate kernel).
+template <int P, bool GENERIC, bool TWO, bool VO, int CH, bool EXACT>
 __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __restrict__ items,
                                                        const int* __restrict__ estart,
                                                        const Entry* __restrict__ ents,
@@ -754,13 +770,14 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int ii = lane & 7, jj = lane >> 3;
-  const int nitems = A.nitems_dev ? min(*A.nitems_dev, A.items_cap) : A.nitems;
+  const int nitems = EXACT ? *A.redo_count : (A.nitems_dev ? min(*A.nitems_dev, A.items_cap) : A.nitems);
   const float band = 4.0e-6f * A.dmax2;
   float4* sjw = sj[w];
   int* scw = scode[w];
   int* saw = TWO ? sabs[w] : nullptr;
 
-  for (int item = blockIdx.x * NW + w; item < nitems; item += gridDim.x * NW) {
+  for (int k = blockIdx.x * NW + w; k < nitems; k += gridDim.x * NW) {
+    const int item = EXACT ? A.redo[k] : k;
     const int2 it = items[item];
     const int ci = it.x;
     const int e0 = estart[ci] + it.y * CH;
@@ -850,13 +867,24 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __
     __builtin_amdgcn_wave_barrier();
 
     PairAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0};
+    bool band_hit = false;  // uniform
     if (n0)
-      eval_group<P, GENERIC, TWO, VO, 0>(A, sjw, scw, saw, 0, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band, acc);
+      eval_group<P, GENERIC, TWO, VO, 0, EXACT>(A, sjw, scw, saw, 0, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band,
+                                                band_hit, acc);
     for (int g = b1; g < b1 + n1; g += CL)
-      eval_group<P, GENERIC, TWO, VO, 1>(A, sjw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band, acc);
+      eval_group<P, GENERIC, TWO, VO, 1, EXACT>(A, sjw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band,
+                                                band_hit, acc);
     if constexpr (!VO) {
       for (int g = b2; g < b2 + n2; g += CL)
-        eval_group<P, GENERIC, TWO, VO, 2>(A, sjw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band, acc);
+        eval_group<P, GENERIC, TWO, VO, 2, EXACT>(A, sjw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band,
+                                                  band_hit, acc);
+    }
+    if (!EXACT && band_hit) {
+      // the item is redone with FP64 cut decisions; its outputs come from there
+      if (lane == 0) A.redo[atomicAdd(A.redo_count, 1)] = item;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      continue;
     }
     // derivative of atom i: -sum over the 8 lanes sharing i (lanes ii + 8k);
     // chunks of one cluster are combined by k_gather in a fixed order
@@ -1168,15 +1196,21 @@ struct PairTuning {
   int blocks_per_cu = 0; // 0: one wave per item; > 0: persistent grid of that many blocks per CU
 };
 
-// launch of k_pairs for a switch, group mode and tuning
-inline void launch_pairs(const PairArgs& A, int grid, hipStream_t s, bool value_only, int chunk,
+// launches of k_pairs for a switch, group mode and tuning: the FP32 pass
+// over all items, then the FP64-decision pass over the items it set aside
+// (A.redo / A.redo_count must be set; exact_grid workgroups loop over them)
+inline void launch_pairs(const PairArgs& A, int grid, int exact_grid, hipStream_t s, bool value_only, int chunk,
                          const int2* items, const int* estart, const Entry* ents, const float4* posI,
                          const float4* posJ, const float4* bbI, const float4* bbJ, const int* absI, const int* absJ,
                          const double* pos64, double* fpart, double* partial) {
   const p2a_switch& sw = A.sw;
-#define P2A_K(P, GEN, TWO, VO, CH)                                                                              \
-  hipLaunchKernelGGL((k_pairs<P, GEN, TWO, VO, CH>), dim3(grid), dim3(PAIR_BLOCK), 0, s, A, items, estart, ents, \
-                     posI, posJ, bbI, bbJ, absI, absJ, pos64, fpart, partial)
+#define P2A_K(P, GEN, TWO, VO, CH)                                                                                \
+  do {                                                                                                            \
+    hipLaunchKernelGGL((k_pairs<P, GEN, TWO, VO, CH, false>), dim3(grid), dim3(PAIR_BLOCK), 0, s, A, items, estart, \
+                       ents, posI, posJ, bbI, bbJ, absI, absJ, pos64, fpart, partial);                            \
+    hipLaunchKernelGGL((k_pairs<P, GEN, TWO, VO, CH, true>), dim3(exact_grid), dim3(PAIR_BLOCK), 0, s, A, items,   \
+                       estart, ents, posI, posJ, bbI, bbJ, absI, absJ, pos64, fpart, partial);                    \
+  } while (0)
 #define P2A_KCH(P, GEN, TWO, VO)          \
   do {                                    \
     if (chunk == 32) P2A_K(P, GEN, TWO, VO, 32); \

@@ -159,3 +159,28 @@ def test_full_batch_256_systems():
     for k in np.argsort(natoms)[:3]:
         check_system(int(k), natoms, boxes, x, v, d, w, counts)
     b.close()
+
+
+def test_batch_lattice_pairs_exactly_at_dmax():
+    """Two cubic lattices with spacing D_MAX/2 in one batch: shells 1-3 lie
+    inside D_MAX and shell 4 (2a) exactly at D_MAX, so every work item goes
+    through the FP64 cut decision."""
+    import torch
+
+    import plumed2_amd as P
+    xs, boxes, nat = [], [], []
+    for m, shift in ((14, 0.011), (16, 0.027)):
+        g = np.arange(m) * 0.6
+        xl = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) + shift
+        # every other atom leaves its site (non-zero forces); pairs of the
+        # atoms left on sites stay exactly at D_MAX
+        xl[::2] += np.random.default_rng(m).normal(0.0, 0.01, xl[::2].shape)
+        xs.append(xl)
+        boxes.append(np.diag([m * 0.6] * 3))
+        nat.append(m ** 3)
+    natoms, boxes, x = np.array(nat), np.array(boxes), np.concatenate(xs)
+    b, _, v, d, w = run(P, torch, natoms, boxes, x)
+    counts = b.pair_counts()
+    for k in range(2):
+        check_system(k, natoms, boxes, x, v, d, w, counts)
+    b.close()

@@ -309,3 +309,28 @@ def test_pair_kernel_decomposition(chunk, bpc):
     check(v, d, w, ov, od, ow)
     a.close()
     a0.close()
+
+
+@pytest.mark.parametrize("chunk", [32, 64])
+def test_lattice_pairs_exactly_at_dmax(chunk):
+    """Simple cubic lattice with spacing D_MAX/2: shells 1-3 inside D_MAX,
+    shell 4 (six partners at 2a) exactly at D_MAX, so every work item needs
+    the FP64 cut decision (inclusive cut on the distance Pbc::distance
+    gives).  Exact pair count, CV, forces."""
+    P = _P()
+    a, m = 0.2, 20
+    g = np.arange(m) * a
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) + 0.013
+    # every other atom leaves its site (non-zero forces); pairs of the atoms
+    # left on sites stay exactly at D_MAX
+    x[::2] += np.random.default_rng(chunk).normal(0.0, 0.005, x[::2].shape)
+    n = len(x)
+    box = np.diag([m * a] * 3)
+    defn = f"RATIONAL R_0=0.15 NN=6 MM=12 D_MAX={2 * a}"
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{n} SWITCH={{{defn}}}")
+    act.set_tuning(chunk, 0)
+    v, d, w = act.calculate(x, box, 0)
+    ov, od, ow, npair = O.coordination(O.Switch(defn), box, x, n, 0)
+    assert act.stats()["contributing_pairs"] == npair
+    check(v, d, w, ov, od, ow)
+    act.close()
