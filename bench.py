@@ -39,6 +39,17 @@ PEAK_FP32_TFLOPS = 157.3
 METRIC = "COORDINATION pair-evals/s (value+forces+virial) @50k atoms; frac FP32 VALU peak"
 
 
+def measured_traffic():
+    """HBM bytes per launch of the pair kernel, from the committed counter runs
+    (FETCH_SIZE / WRITE_SIZE scaled by a plain copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "k_pairs_coordination_50k_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def box_length(n=N_ATOMS, rho=DENSITY):
     return (n / rho) ** (1.0 / 3.0)
 
@@ -262,6 +273,16 @@ def bench_batch(P, torch, steps=10):
             "clusters": st["clusters"], "list_entries": st["list_entries"], "items": st["items"]}
 
 
+def run_extras(P, torch, which: str):
+    runs = {"batch": ("batch_256_systems", bench_batch), "autograd": ("autograd_two_group_500k", bench_autograd),
+            "coord100k": ("coordination_100k", bench_coord_100k), "metad": ("metad_grid", bench_metad)}
+    out = {}
+    for key in [w.strip() for w in which.split(",") if w.strip()]:
+        name, fn = runs[key]
+        out[name] = fn(P, torch)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,6 +291,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--no-extras", action="store_true", help="skip the autograd / METAD / 100k measurements")
+    ap.add_argument("--extras", default="batch,autograd,coord100k,metad",
+                    help="comma list of extra measurements to run")
+    ap.add_argument("--only-extras", action="store_true", help="run the extras only (no 50k main line)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 256 MiB device copy (reference size for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
@@ -278,7 +304,21 @@ def main():
 
     import torch
 
+    if args.copy_calibration:
+        n = 256 << 20
+        src = torch.ones(n // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 5}))
+        return
+
     import plumed2_amd as P
+
+    if args.only_extras:
+        print(json.dumps(run_extras(P, torch, args.extras)))
+        return
 
     dev = torch.device("cuda:0")
     x0, L = initial_positions()
@@ -349,8 +389,8 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
-            "kernel": "k_pairs<3,false> (cluster-pair value/derivative/virial)",
+            "traffic": measured_traffic(),
+            "kernel": "k_pairs<3,false,false,false,64,*> (FP32 pass + FP64-decision pass, value/derivative/virial)",
             "kernel_ms": kern_ms,
             "flop_per_pair": FLOP_PER_PAIR,
         },
@@ -359,10 +399,7 @@ def main():
     del frames
     torch.cuda.empty_cache()
     if not args.no_extras:
-        result["extras"] = {"batch_256_systems": bench_batch(P, torch),
-                            "autograd_two_group_500k": bench_autograd(P, torch),
-                            "coordination_100k": bench_coord_100k(P, torch),
-                            "metad_grid": bench_metad(P, torch)}
+        result["extras"] = run_extras(P, torch, args.extras)
     if not args.no_cpu:
         nthreads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         result["cpu_baseline"] = cpu_baseline(x0, L, nthreads)

@@ -54,7 +54,7 @@ struct p2a_batch {
   p2a::DBuf<double4> origin;
   p2a::DBuf<float4> bb0, bb, p;
   p2a::DBuf<float2> kr;
-  p2a::DBuf<int> ecount, estart, istart, totals, redo;
+  p2a::DBuf<int> ecount, estart, istart, totals, redo, scan_bsum;
   p2a::DBuf<p2a::Entry> ents;
   p2a::DBuf<int2> items;
   p2a::DBuf<double> fpart, partial, npairs, out_values, out_virials, out_deriv;
@@ -120,19 +120,16 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
   hipLaunchKernelGGL(k_list<0>, dim3(grid1(clc, wpb)), dim3(256), 0, s, L, b->cl_col.p, (const int*)b->cl_sys.p,
                      (const SysArg*)b->sys.p, (const SysArg*)b->sys.p, b->bb0.p, b->bb0.p, b->kr.p, b->kr.p,
                      b->col_cl.p, b->origin.p, b->origin.p, b->ecount.p, (const int*)nullptr, (Entry*)nullptr);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, clc, ncl_dev, b->ecount.p, b->estart.p, b->totals.p, 2, 0);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, clc, ncl_dev, b->ecount.p, b->istart.p, b->totals.p, 3,
-                     chunk);
+  launch_list_scans(s, clc, ncl_dev, b->ecount.p, chunk, b->scan_bsum.p, b->estart.p, b->istart.p, b->totals.p, 2, 3);
   hipLaunchKernelGGL(k_list<1>, dim3(grid1(clc, wpb)), dim3(256), 0, s, L, b->cl_col.p, (const int*)b->cl_sys.p,
                      (const SysArg*)b->sys.p, (const SysArg*)b->sys.p, b->bb0.p, b->bb0.p, b->kr.p, b->kr.p,
                      b->col_cl.p, b->origin.p, b->origin.p, b->ecount.p, (const int*)b->estart.p, b->ents.p);
   hipLaunchKernelGGL(k_items, dim3(grid1(clc, 256)), dim3(256), 0, s, clc, ncl_dev, b->ecount.p, b->istart.p,
                      b->items.p, chunk, b->items_cap);
-  hipLaunchKernelGGL(k_pack, dim3(grid1(nslot, 256)), dim3(256), 0, s, (int)nslot, ncl_dev, b->slot_loc.p, b->info.p,
-                     pos, 0, (const SysArg*)b->sys.p, (const int*)b->atom_sys.p, b->origin.p, b->p.p);
   // totals[4..5] serve as the pair kernels' redo counters
-  hipLaunchKernelGGL(k_bbox, dim3(grid1(std::max(clc, 2), 256)), dim3(256), 0, s, clc, ncl_dev, b->p.p, b->bb.p,
-                     b->totals.p + 4);
+  hipLaunchKernelGGL(k_pack_bbox, dim3(grid1(nslot, 256)), dim3(256), 0, s, (int)nslot, ncl_dev, b->slot_loc.p,
+                     b->info.p, pos, 0, (const SysArg*)b->sys.p, (const int*)b->atom_sys.p, b->origin.p, b->p.p,
+                     b->bb.p, b->totals.p + 4);
   int rc;
   if ((rc = launch_check("batch: list build"))) return rc;
   PairArgs A;
@@ -346,7 +343,8 @@ int p2a_batch_set_boxes(p2a_batch* b, const double* boxes) {
       (rc = b->slot_loc.reserve((size_t)clc * CL)) || (rc = b->cl_col.reserve(clc)) || (rc = b->cl_sys.reserve(clc)) ||
       (rc = b->origin.reserve(clc)) || (rc = b->bb0.reserve(2 * (size_t)clc)) || (rc = b->bb.reserve(2 * (size_t)clc)) ||
       (rc = b->p.reserve((size_t)clc * CL)) || (rc = b->kr.reserve(clc)) || (rc = b->ecount.reserve(clc + 1)) ||
-      (rc = b->estart.reserve(clc + 1)) || (rc = b->istart.reserve(clc + 1)))
+      (rc = b->estart.reserve(clc + 1)) || (rc = b->istart.reserve(clc + 1)) ||
+      (rc = b->scan_bsum.reserve(2 * (size_t)grid1(clc, kScanBlock) + 2)))
     return rc;
   P2A_HIP(hipMemcpy(b->sys.p, S.data(), S.size() * sizeof(SysArg), hipMemcpyHostToDevice));
   P2A_HIP(hipMemcpy(b->pbc.p, Pb.data(), Pb.size() * sizeof(PbcArg), hipMemcpyHostToDevice));

@@ -66,6 +66,8 @@ struct p2a_coord {
   int cu_count = 256;
   p2a::DBuf<double> partial, red, stats, pos_buf, deriv_buf, out_buf;
   p2a::DBuf<int> redo_count;  // one counter per direction
+  p2a::DBuf<int> scan_bsum;   // block sums of the list scans
+  p2a::DBuf<unsigned> ticket; // last-workgroup ticket of the reduction
   p2a::DBuf<long long> abs_dev;
   int64_t rebuilds = 0;
   bool profiling = false;
@@ -187,10 +189,9 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
       hipLaunchKernelGGL(k_list<0>, dim3(grid1(GI.ncl, wpb)), dim3(256), 0, s, L, GI.cl_col.p, (const int*)nullptr,
                          (const SysArg*)GI.sys.p, (const SysArg*)GJ.sys.p, GI.bb0.p, GJ.bb0.p, GI.kr.p, GJ.kr.p,
                          GJ.col_cl.p, GI.origin.p, GJ.origin.p, D.ecount.p, (const int*)nullptr, (Entry*)nullptr);
-      hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, GI.ncl, (const int*)nullptr, D.ecount.p, D.estart.p,
-                         c->totals.p, 8 + 2 * d, 0);
-      hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, GI.ncl, (const int*)nullptr, D.ecount.p, D.istart.p,
-                         c->totals.p, 9 + 2 * d, chunk);
+      if ((rc = c->scan_bsum.reserve(2 * (size_t)grid1(GI.ncl, kScanBlock) + 2))) return rc;
+      launch_list_scans(s, GI.ncl, nullptr, D.ecount.p, chunk, c->scan_bsum.p, D.estart.p, D.istart.p, c->totals.p,
+                        8 + 2 * d, 9 + 2 * d);
     } else {
       hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, c->totals.p + 8 + 2 * d, 2, 0);
     }
@@ -262,6 +263,24 @@ static int upload_pbc(p2a_coord* c, const double* box9) {
   return P2A_OK;
 }
 
+// deterministic sum of nred 8-double partials into value / virial / stats
+static int reduce_partials(p2a_coord* c, int nred, double* value_dev, double* virial_dev) {
+  int rc;
+  hipStream_t s = c->stream;
+  if (nred > 4 * kRedBlocks) {
+    if (!c->ticket.p) {
+      if ((rc = c->ticket.reserve(1))) return rc;
+      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int*)c->ticket.p, 1, 0);
+    }
+    if ((rc = c->red.reserve(8 * kRedBlocks))) return rc;
+    hipLaunchKernelGGL(k_reduce_all, dim3(kRedBlocks), dim3(256), 0, s, nred, c->partial.p, c->red.p, c->ticket.p,
+                       value_dev, virial_dev, c->stats.p);
+  } else {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(128), 0, s, nred, c->partial.p, value_dev, virial_dev, c->stats.p);
+  }
+  return P2A_OK;
+}
+
 static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, double* value_dev, double* deriv_dev,
                      double* virial_dev, bool value_only) {
   int rc;
@@ -270,11 +289,9 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
   for (int gi = 0; gi < ngroups; gi++) {
     Group& G = c->g[gi];
     if (G.ncl == 0) continue;
-    hipLaunchKernelGGL(k_pack, dim3(grid1((size_t)G.ncl * CL, 256)), dim3(256), 0, s, G.ncl * CL,
+    hipLaunchKernelGGL(k_pack_bbox, dim3(grid1((size_t)G.ncl * CL, 256)), dim3(256), 0, s, G.ncl * CL,
                        (const int*)nullptr, G.slot_loc.p, G.info.p, pos_dev, G.off, (const SysArg*)G.sys.p,
-                       (const int*)nullptr, G.origin.p, G.p.p);
-    hipLaunchKernelGGL(k_bbox, dim3(grid1(std::max(G.ncl, 2), 256)), dim3(256), 0, s, G.ncl, (const int*)nullptr,
-                       G.p.p, G.bb.p, c->redo_count.p);
+                       (const int*)nullptr, G.origin.p, G.p.p, G.bb.p, c->redo_count.p);
   }
   PairArgs A;
   std::memset(&A, 0, sizeof(A));
@@ -328,14 +345,7 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     hipLaunchKernelGGL(k_gather, dim3(grid1((size_t)GI.ncl * CL * 3, 256)), dim3(256), 0, s, GI.ncl,
                        (const int*)nullptr, GI.p.p, D.istart.p, (int)D.items.n, D.fpart.p, deriv_dev);
   }
-  if ((rc = c->red.reserve(8 * kRedBlocks))) return rc;
-  const int nred = value_only ? c->dir[0].nitems : c->nitems;
-  if (nred > 4 * kRedBlocks) {
-    hipLaunchKernelGGL(k_reduce1, dim3(kRedBlocks), dim3(256), 0, s, nred, c->partial.p, c->red.p);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(128), 0, s, kRedBlocks, c->red.p, value_dev, virial_dev, c->stats.p);
-  } else {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(128), 0, s, nred, c->partial.p, value_dev, virial_dev, c->stats.p);
-  }
+  if ((rc = reduce_partials(c, value_only ? c->dir[0].nitems : c->nitems, value_dev, virial_dev))) return rc;
   return launch_check("finalize");
 }
 
@@ -360,14 +370,7 @@ static int run_fallback(p2a_coord* c, const double* box9, const double* pos_dev,
                        pos_dev, c->abs_dev.p, deriv_dev, c->partial.p);
   }
   if ((rc = launch_check("fp64 pair kernel"))) return rc;
-  if ((rc = c->red.reserve(8 * kRedBlocks))) return rc;
-  const int nred = std::max(nblocks, 1);
-  if (nred > 4 * kRedBlocks) {
-    hipLaunchKernelGGL(k_reduce1, dim3(kRedBlocks), dim3(256), 0, s, nred, c->partial.p, c->red.p);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(128), 0, s, kRedBlocks, c->red.p, value_dev, virial_dev, c->stats.p);
-  } else {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(128), 0, s, nred, c->partial.p, value_dev, virial_dev, c->stats.p);
-  }
+  if ((rc = reduce_partials(c, std::max(nblocks, 1), value_dev, virial_dev))) return rc;
   return launch_check("finalize");
 }
 

@@ -9,13 +9,13 @@
 //                     (NeighborList.cpp:168-236, LinkCells.cpp:99-239); the
 //                     O(N^2) pair array is never materialised, the list is one
 //                     of 8x8-atom cluster pairs
-//   k_pack            ActionAtomistic::retrieveAtoms gather + the image
+//   k_pack_bbox       ActionAtomistic::retrieveAtoms gather + the image
 //                     bookkeeping of Pbc::distance
 //   k_pairs           CoordinationBase::calculate pair loop
 //                     (CoordinationBase.cpp:178-207) with
 //                     fixedRational<N>::calculateSqr (SwitchingFunction.cpp:
 //                     203-215) inlined
-//   k_finalize,       the OpenMP critical merge + comm.Sum
+//   k_reduce_all,     the OpenMP critical merge + comm.Sum
 //   k_sys_reduce      (CoordinationBase.cpp:209-224)
 //   k_allpairs,       the same loop in FP64 for NOPBC, PAIR, unbounded D_MAX,
 //   k_pairmode        cells narrower than 2*cutoff
@@ -212,33 +212,117 @@ __global__ void k_scan_cols(int ncol, const int* __restrict__ cnt, int* __restri
   }
 }
 
-// exclusive scan of n ints (one workgroup; n read from n_dev when given);
-// chunk > 0 scans ceil(in/chunk) instead of in; totals[slot] = sum
-__global__ void k_scan(int n, const int* __restrict__ n_dev, const int* __restrict__ in, int* __restrict__ out,
-                       int* __restrict__ totals, int slot, int chunk) {
-  __shared__ int s[1024];
+// Two exclusive scans of one count array in one pass over it (out0 of in[i],
+// out1 of ceil(in[i]/chunk)), across many workgroups: per-block sums, a
+// scan of the block sums, then per-block local scans with the block offsets.
+// n is read from n_dev when given; out0[n], out1[n] and totals[slot0/1]
+// receive the two totals.
+constexpr int kScanBlock = 1024;  // elements per workgroup (256 threads x 4)
+
+__device__ __forceinline__ int block_excl_scan256(int v, int* sh, int& total) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const int u = (t >= o) ? sh[t - o] : 0;
+    __syncthreads();
+    sh[t] += u;
+    __syncthreads();
+  }
+  total = sh[255];
+  const int r = sh[t] - v;
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_scan2_sums(int n, const int* __restrict__ n_dev, const int* __restrict__ in,
+                                                    int chunk, int* __restrict__ bsum) {
+  __shared__ int sh[256];
+  if (n_dev) n = *n_dev;
+  const int base = blockIdx.x * kScanBlock + threadIdx.x * 4;
+  int a = 0, c = 0;
+  for (int k = 0; k < 4; k++) {
+    const int i = base + k;
+    if (i < n) {
+      a += in[i];
+      c += (in[i] + chunk - 1) / chunk;
+    }
+  }
+  int ta, tc;
+  block_excl_scan256(a, sh, ta);
+  block_excl_scan256(c, sh, tc);
+  if (threadIdx.x == 0) {
+    bsum[2 * blockIdx.x] = ta;
+    bsum[2 * blockIdx.x + 1] = tc;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_scan2_blocks(int nb, int n, const int* __restrict__ n_dev,
+                                                       int* __restrict__ bsum, int* __restrict__ out0,
+                                                       int* __restrict__ out1, int* __restrict__ totals, int slot0,
+                                                       int slot1) {
+  __shared__ int sa[1024], sc[1024];
   if (n_dev) n = *n_dev;
   const int t = threadIdx.x, T = blockDim.x;
-  int per = (n + T - 1) / T;
-  int b = t * per, e = min(n, b + per);
-  int l = 0;
-  for (int i = b; i < e; i++) l += chunk ? (in[i] + chunk - 1) / chunk : in[i];
-  s[t] = l;
+  const int per = (nb + T - 1) / T;
+  const int b = t * per, e = min(nb, b + per);
+  int la = 0, lc = 0;
+  for (int i = b; i < e; i++) {
+    la += bsum[2 * i];
+    lc += bsum[2 * i + 1];
+  }
+  sa[t] = la;
+  sc[t] = lc;
   __syncthreads();
   for (int o = 1; o < T; o <<= 1) {
-    int v = (t >= o) ? s[t - o] : 0;
+    const int va = (t >= o) ? sa[t - o] : 0, vc = (t >= o) ? sc[t - o] : 0;
     __syncthreads();
-    s[t] += v;
+    sa[t] += va;
+    sc[t] += vc;
     __syncthreads();
   }
-  int r = s[t] - l;
+  int ra = sa[t] - la, rc = sc[t] - lc;
   for (int i = b; i < e; i++) {
-    out[i] = r;
-    r += chunk ? (in[i] + chunk - 1) / chunk : in[i];
+    const int xa = bsum[2 * i], xc = bsum[2 * i + 1];
+    bsum[2 * i] = ra;
+    bsum[2 * i + 1] = rc;
+    ra += xa;
+    rc += xc;
   }
   if (t == T - 1) {
-    out[n] = s[t];
-    totals[slot] = s[t];
+    out0[n] = sa[t];
+    out1[n] = sc[t];
+    totals[slot0] = sa[t];
+    totals[slot1] = sc[t];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan2_apply(int n, const int* __restrict__ n_dev, const int* __restrict__ in,
+                                                     int chunk, const int* __restrict__ bsum, int* __restrict__ out0,
+                                                     int* __restrict__ out1) {
+  __shared__ int sh[256];
+  if (n_dev) n = *n_dev;
+  if (blockIdx.x * kScanBlock >= n) return;  // uniform per workgroup
+  const int base = blockIdx.x * kScanBlock + threadIdx.x * 4;
+  int va[4], vc[4], a = 0, c = 0;
+  for (int k = 0; k < 4; k++) {
+    const int i = base + k;
+    va[k] = (i < n) ? in[i] : 0;
+    vc[k] = (i < n) ? (va[k] + chunk - 1) / chunk : 0;
+    a += va[k];
+    c += vc[k];
+  }
+  int ta, tc;
+  int ra = block_excl_scan256(a, sh, ta) + bsum[2 * blockIdx.x];
+  int rc = block_excl_scan256(c, sh, tc) + bsum[2 * blockIdx.x + 1];
+  for (int k = 0; k < 4; k++) {
+    const int i = base + k;
+    if (i < n) {
+      out0[i] = ra;
+      out1[i] = rc;
+    }
+    ra += va[k];
+    rc += vc[k];
   }
 }
 
@@ -508,50 +592,52 @@ __global__ void k_items(int ncl, const int* __restrict__ ncl_dev, const int* __r
 // per-step kernels
 // ------------------------------------------------------------------------
 
-// local FP32 coordinates of every slot; padding slots far away and distinct
-__global__ void k_pack(int nslot, const int* __restrict__ ncl_dev, const int* __restrict__ slot_loc,
-                       const int4* __restrict__ info, const double* __restrict__ pos, int off,
-                       const SysArg* __restrict__ sys, const int* __restrict__ atom_sys,
-                       const double4* __restrict__ origin, float4* __restrict__ out) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+// k_pack and k_bbox in one pass: one thread per slot, the 8 slots of a
+// cluster are 8 adjacent lanes, which reduce the cluster's box by shuffles.
+// Also clears the pair kernels' redo counters (zero2[0..1]).
+__global__ __launch_bounds__(256) void k_pack_bbox(int nslot, const int* __restrict__ ncl_dev,
+                                                   const int* __restrict__ slot_loc, const int4* __restrict__ info,
+                                                   const double* __restrict__ pos, int off,
+                                                   const SysArg* __restrict__ sys, const int* __restrict__ atom_sys,
+                                                   const double4* __restrict__ origin, float4* __restrict__ out,
+                                                   float4* __restrict__ bb, int* __restrict__ zero2) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (zero2 && s < 2) zero2[s] = 0;
   if (ncl_dev) nslot = *ncl_dev * CL;
+  // whole groups of 8 lanes stay together (nslot is a multiple of 8)
   if (s >= nslot) return;
-  int a = slot_loc[s];
+  const int a = slot_loc[s];
+  float4 q;
+  float lo[3], hi[3];
   if (a < 0) {
-    out[s] = make_float4(1.0e5f + 1.0e3f * (float)(s & 1023), 1.0e5f + 1.0e3f * (float)((s >> 10) & 1023),
-                         1.0e5f + 1.0e3f * (float)(s >> 20), __int_as_float(-1));
-    return;
+    q = make_float4(1.0e5f + 1.0e3f * (float)(s & 1023), 1.0e5f + 1.0e3f * (float)((s >> 10) & 1023),
+                    1.0e5f + 1.0e3f * (float)(s >> 20), __int_as_float(-1));
+    lo[0] = lo[1] = lo[2] = FLT_MAX;
+    hi[0] = hi[1] = hi[2] = -FLT_MAX;
+  } else {
+    const SysArg& S = sys[nsys_of(atom_sys, a)];
+    const int4 m = info[a];
+    const double4 o = origin[s / CL];
+    const double* x = pos + 3 * (size_t)(off + a);
+    double w[3];
+    for (int d = 0; d < 3; d++) w[d] = x[d] - (m.x * S.a[0 * 3 + d] + m.y * S.a[1 * 3 + d] + m.z * S.a[2 * 3 + d]);
+    q = make_float4((float)(w[0] - o.x), (float)(w[1] - o.y), (float)(w[2] - o.z), __int_as_float(off + a));
+    lo[0] = hi[0] = q.x;
+    lo[1] = hi[1] = q.y;
+    lo[2] = hi[2] = q.z;
   }
-  const SysArg& S = sys[nsys_of(atom_sys, a)];
-  int4 m = info[a];
-  double4 o = origin[s / CL];
-  const double* x = pos + 3 * (size_t)(off + a);
-  double w[3];
-  for (int d = 0; d < 3; d++) w[d] = x[d] - (m.x * S.a[0 * 3 + d] + m.y * S.a[1 * 3 + d] + m.z * S.a[2 * 3 + d]);
-  out[s] = make_float4((float)(w[0] - o.x), (float)(w[1] - o.y), (float)(w[2] - o.z), __int_as_float(off + a));
-}
-
-// current bounding box of each cluster in its local frame; also clears the
-// pair kernels' redo counters (zero2[0..1]) for the launches that follow
-__global__ void k_bbox(int ncl, const int* __restrict__ ncl_dev, const float4* __restrict__ p,
-                       float4* __restrict__ bb, int* __restrict__ zero2) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (zero2 && c < 2) zero2[c] = 0;
-  if (ncl_dev) ncl = *ncl_dev;
-  if (c >= ncl) return;
-  float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-  for (int k = 0; k < CL; k++) {
-    float4 q = p[c * CL + k];
-    if (__float_as_int(q.w) < 0) continue;
-    lo[0] = fminf(lo[0], q.x);
-    hi[0] = fmaxf(hi[0], q.x);
-    lo[1] = fminf(lo[1], q.y);
-    hi[1] = fmaxf(hi[1], q.y);
-    lo[2] = fminf(lo[2], q.z);
-    hi[2] = fmaxf(hi[2], q.z);
+  out[s] = q;
+  for (int o = 1; o < CL; o <<= 1) {
+    for (int d = 0; d < 3; d++) {
+      lo[d] = fminf(lo[d], __shfl_xor(lo[d], o, 64));
+      hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], o, 64));
+    }
   }
-  bb[2 * c] = make_float4(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]), 0.f);
-  bb[2 * c + 1] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]), 0.5f * (hi[2] - lo[2]), 0.f);
+  if ((s & (CL - 1)) == 0) {
+    const int c = s / CL;
+    bb[2 * c] = make_float4(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]), 0.f);
+    bb[2 * c + 1] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]), 0.5f * (hi[2] - lo[2]), 0.f);
+  }
 }
 
 // Tools::pbc (Tools.h:547-570)
@@ -931,12 +1017,26 @@ __global__ void k_gather(int ncl, const int* __restrict__ ncl_dev, const float4*
   deriv[3 * (size_t)req + r % 3] = sum;
 }
 
-// deterministic sum of the per-item partials -> value, virial, pair count.
-// Stage 1: 128 blocks reduce contiguous ranges; stage 2: one block.
+// virial = -sum dd (x) d (CoordinationBase.cpp:200-206), symmetric
+__device__ __forceinline__ void write_virial(double* virial, const double* s6) {
+  double xx = -s6[0], xy = -s6[1], xz = -s6[2], yy = -s6[3], yz = -s6[4], zz = -s6[5];
+  virial[0] = xx; virial[1] = xy; virial[2] = xz;
+  virial[3] = xy; virial[4] = yy; virial[5] = yz;
+  virial[6] = xz; virial[7] = yz; virial[8] = zz;
+}
+
 constexpr int kRedBlocks = 128;
-__global__ __launch_bounds__(256) void k_reduce1(int nitems, const double* __restrict__ partial,
-                                                 double* __restrict__ out) {
+
+// Deterministic sum of the per-item partials -> value, virial, pair count:
+// kRedBlocks workgroups reduce contiguous ranges, and the last workgroup to
+// finish sums the block partials in block order.  *ticket must be 0
+// before the launch; the last workgroup leaves it at 0.
+__global__ __launch_bounds__(256) void k_reduce_all(int nitems, const double* __restrict__ partial,
+                                                    double* __restrict__ blocks, unsigned* __restrict__ ticket,
+                                                    double* __restrict__ value, double* __restrict__ virial,
+                                                    double* __restrict__ stats_out) {
   __shared__ double s[8][256];
+  __shared__ bool last;
   const int t = threadIdx.x;
   const int per = (nitems + gridDim.x - 1) / gridDim.x;
   const int b = blockIdx.x * per, e = min(nitems, b + per);
@@ -954,15 +1054,40 @@ __global__ __launch_bounds__(256) void k_reduce1(int nitems, const double* __res
       for (int q = 0; q < 8; q++) s[q][t] += s[q][t + o];
     __syncthreads();
   }
-  if (t < 8) out[(size_t)blockIdx.x * 8 + t] = s[t][0];
-}
-
-// virial = -sum dd (x) d (CoordinationBase.cpp:200-206), symmetric
-__device__ __forceinline__ void write_virial(double* virial, const double* s6) {
-  double xx = -s6[0], xy = -s6[1], xz = -s6[2], yy = -s6[3], yz = -s6[4], zz = -s6[5];
-  virial[0] = xx; virial[1] = xy; virial[2] = xz;
-  virial[3] = xy; virial[4] = yy; virial[5] = yz;
-  virial[6] = xz; virial[7] = yz; virial[8] = zz;
+  if (t < 8) blocks[(size_t)blockIdx.x * 8 + t] = s[t][0];
+  __threadfence();
+  __syncthreads();
+  if (t == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // thread t sums component t&7 of blocks t>>3, t>>3 + 32, ...: independent
+  // loads in flight together; then 32 fixed-order terms per component
+  __shared__ double s2[256];
+  {
+    const int q = t & 7;
+    double a = 0.0;
+    for (int i = t >> 3; i < (int)gridDim.x; i += 32)
+      a += __hip_atomic_load(&blocks[(size_t)i * 8 + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s2[t] = a;
+  }
+  __syncthreads();
+  if (t < 8) {
+    double a = 0.0;
+    for (int r = 0; r < 32; r++) a += s2[t + 8 * r];
+    s[t][0] = a;
+  }
+  __syncthreads();
+  if (t == 0) {
+    if (value) *value = s[0][0];
+    if (virial) {
+      double v6[6] = {s[1][0], s[2][0], s[3][0], s[4][0], s[5][0], s[6][0]};
+      write_virial(virial, v6);
+    }
+    stats_out[0] = s[7][0];
+    stats_out[1] += s[7][0];
+    *ticket = 0u;
+  }
 }
 
 __global__ void k_finalize(int nitems, const double* __restrict__ partial, double* __restrict__ value,
@@ -1242,6 +1367,18 @@ inline void launch_pairs(const PairArgs& A, int grid, int exact_grid, hipStream_
 #undef P2A_KSW
 #undef P2A_KCH
 #undef P2A_K
+}
+
+// entry starts (out0) and work-item starts (out1, chunks of `chunk`) of the
+// n = *n_dev (or n) i clusters; bsum holds 2 * ceil(ncap / kScanBlock) ints
+inline void launch_list_scans(hipStream_t s, int ncap, const int* n_dev, const int* count, int chunk, int* bsum,
+                              int* out0, int* out1, int* totals, int slot0, int slot1) {
+  const int nb = std::max(1, grid1(ncap, kScanBlock));
+  hipLaunchKernelGGL(k_scan2_sums, dim3(nb), dim3(256), 0, s, ncap, n_dev, count, chunk, bsum);
+  hipLaunchKernelGGL(k_scan2_blocks, dim3(1), dim3(1024), 0, s, nb, ncap, n_dev, bsum, out0, out1, totals, slot0,
+                     slot1);
+  hipLaunchKernelGGL(k_scan2_apply, dim3(nb), dim3(256), 0, s, ncap, n_dev, count, chunk, (const int*)bsum, out0,
+                     out1);
 }
 
 // FP32 switch constants and cut parameters of the pair kernel

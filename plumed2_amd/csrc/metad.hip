@@ -40,6 +40,7 @@ namespace p2a {
 
 constexpr int kMaxCv = 4;
 constexpr int kDepositThreads = 256;
+constexpr int WAVE_M = 64;
 
 struct GridArg {
   int ncv;
@@ -196,36 +197,75 @@ __device__ __forceinline__ double wt_height(const GridArg& G, double vbias) {
   return G.height0 * exp(-vbias / G.kbt);
 }
 
-// sequential well-tempered deposition of many hills in one workgroup
-__global__ __launch_bounds__(kDepositThreads) void k_deposit_seq(GridArg G, double* __restrict__ val,
-                                                                  double* __restrict__ der, long long nhills,
-                                                                  const double* __restrict__ centers,
-                                                                  double* __restrict__ heights,
-                                                                  int* __restrict__ err) {
+// spline value (no gradient) at x from the corner ip of its cell: the term
+// `value += grid * ff` of GridBase::getValueAndDerivatives for one ip
+__device__ double spline_corner(const GridArg& G, const double* __restrict__ val, const double* __restrict__ der,
+                                const double* x, const unsigned* ind, unsigned ip) {
+  unsigned ni[kMaxCv];
+  unsigned t = ip;
+  int cnt = 0;
+  for (int j = 0; j < G.ncv; j++) {
+    unsigned i0 = t % 2 + ind[j];
+    t /= 2;
+    if (!G.pbc[j] && i0 == G.nbin[j]) continue;
+    if (G.pbc[j] && i0 == G.nbin[j]) i0 = 0;
+    ni[cnt++] = i0;
+  }
+  if (cnt != G.ncv) return 0.0;
+  const long long node = flat(G, ni);
+  const double grid = val[node];
+  double ff = 1.0;
+  for (int j = 0; j < G.ncv; j++) {
+    const int x0 = (ni[j] == ind[j]) ? 0 : 1;
+    const double dx = G.dx[j];
+    const double xfloor = G.min[j] + (double)ind[j] * dx;
+    const double X = fabs((x[j] - xfloor) / dx - (double)x0);
+    const double X2 = X * X, X3 = X2 * X;
+    const double yy = fabs(grid) < 0.0000001 ? 0.0 : -der[node * G.ncv + j] / grid;
+    const double sg = x0 ? -1.0 : 1.0;
+    ff *= (1.0 - 3.0 * X2 + 2.0 * X3) - sg * yy * (X - 2.0 * X2 + X3) * dx;
+  }
+  return grid * ff;
+}
+
+// Sequential well-tempered deposition of many hills in one workgroup: each
+// hill's height needs the bias of the grid left by the previous hill
+// (MetaD::update -> getHeight).  Per hill, the first wave evaluates the
+// 2^ncv spline corners in parallel and sums them in corner order; then the
+// whole workgroup adds the hill, one support node per thread.
+constexpr int kSeqThreads = 1024;
+__global__ __launch_bounds__(kSeqThreads) void k_deposit_seq(GridArg G, double* __restrict__ val,
+                                                             double* __restrict__ der, long long nhills,
+                                                             const double* __restrict__ centers,
+                                                             double* __restrict__ heights, int* __restrict__ err) {
   __shared__ double sh_h;
   __shared__ int sh_ok;
   __shared__ unsigned sh_ind[kMaxCv];
   __shared__ double sh_c[kMaxCv];
+  const int t = threadIdx.x;
+  const unsigned nn = 1u << G.ncv;
   for (long long k = 0; k < nhills; k++) {
-    if (threadIdx.x == 0) {
+    if (t < WAVE_M) {
       double c[kMaxCv];
       for (int i = 0; i < G.ncv; i++) c[i] = centers[k * G.ncv + i];
       unsigned ind[kMaxCv];
-      int ok = grid_index_of(G, c, ind) >= 0;
-      double h = G.height0;
-      if (ok && G.welltemp) {
-        double v, dv[kMaxCv];
-        ok = spline(G, val, der, c, v, dv);
-        h = wt_height(G, v);
+      const int ok = grid_index_of(G, c, ind) >= 0;
+      double v = 0.0;
+      if (ok && G.welltemp && (unsigned)t < nn) v = spline_corner(G, val, der, c, ind, (unsigned)t);
+      // corner order 0, 1, 2, ... as the sequential loop adds them
+      double tot = 0.0;
+      for (unsigned q = 0; q < nn; q++) tot += __shfl(v, (int)q, WAVE_M);
+      if (t == 0) {
+        const double h = (ok && G.welltemp) ? wt_height(G, tot) : G.height0;
+        if (!ok) atomicCAS(err, 0, (int)(1 + (k < 0x3fffffff ? k : 0x3fffffff)));
+        for (int i = 0; i < G.ncv; i++) {
+          sh_ind[i] = ind[i];
+          sh_c[i] = c[i];
+        }
+        sh_h = h;
+        sh_ok = ok;
+        if (heights) heights[k] = ok ? h : 0.0;
       }
-      if (!ok) atomicCAS(err, 0, (int)(1 + (k < 0x3fffffff ? k : 0x3fffffff)));
-      for (int i = 0; i < G.ncv; i++) {
-        sh_ind[i] = ind[i];
-        sh_c[i] = c[i];
-      }
-      sh_h = h;
-      sh_ok = ok;
-      if (heights) heights[k] = ok ? h : 0.0;
     }
     __syncthreads();
     if (!sh_ok) return;  // uniform: every thread reads the same flag
@@ -235,8 +275,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit_seq(GridArg G, doub
       ind[i] = sh_ind[i];
       c[i] = sh_c[i];
     }
-    const double h = sh_h;
-    add_hill_block(G, val, der, ind, c, h);
+    add_hill_block(G, val, der, ind, c, sh_h);
     __syncthreads();
   }
 }
@@ -482,7 +521,7 @@ int p2a_metad_deposit_device(p2a_metad* m, int64_t nhills, const double* centers
   if (!m || (nhills > 0 && !centers_dev)) return fail(P2A_ERR_ARG, "null argument");
   if (nhills <= 0) return P2A_OK;
   P2A_HIPM(hipSetDevice(m->device));
-  hipLaunchKernelGGL(k_deposit_seq, dim3(1), dim3(kDepositThreads), 0, m->stream, m->G, m->val, m->der,
+  hipLaunchKernelGGL(k_deposit_seq, dim3(1), dim3(kSeqThreads), 0, m->stream, m->G, m->val, m->der,
                      (long long)nhills, centers_dev, heights_dev, m->err);
   P2A_HIPM(hipGetLastError());
   return P2A_OK;
