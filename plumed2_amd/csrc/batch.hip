@@ -149,8 +149,9 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
   launch_pairs(A, grid, std::min(grid, b->cu_count), s, false, chunk, b->items.p, b->estart.p, b->ents.p, b->p.p, b->p.p, b->bb.p, b->bb.p,
                (const int*)nullptr, (const int*)nullptr, pos, b->fpart.p, b->partial.p);
   if ((rc = launch_check("batch: pair kernel"))) return rc;
-  hipLaunchKernelGGL(k_gather, dim3(grid1(nslot * 3, 256)), dim3(256), 0, s, clc, ncl_dev, b->p.p, b->istart.p,
-                     b->items_cap, b->fpart.p, deriv);
+  hipLaunchKernelGGL(k_gather_reduce, dim3(grid1(nslot * 3, 256)), dim3(256), 0, s, grid1(nslot * 3, 256), clc,
+                     ncl_dev, b->p.p, b->istart.p, b->items_cap, b->fpart.p, deriv, 0, (const double*)nullptr,
+                     (double*)nullptr);
   hipLaunchKernelGGL(k_sys_reduce, dim3(b->nsys), dim3(256), 0, s, b->nsys, b->sys_colbase.p, b->col_cl.p,
                      b->istart.p, b->items_cap, b->partial.p, values, virials, b->npairs.p);
   return launch_check("batch: reductions");

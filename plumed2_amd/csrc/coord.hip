@@ -31,6 +31,9 @@ struct Dir {  // one cluster-pair list: i clusters of group I against group J
 struct Group {
   int off = 0, n = 0;
   SysArg S{};   // cell and column grid of this group
+  SysArg S_dev{};  // the copy the device holds
+  bool S_valid = false;
+  int maxcl = 0;   // cluster capacity: n/8 + one partial cluster per column
   int ncol = 1;
   int ncl = 0;  // clusters (device total copied at rebuild)
   DBuf<SysArg> sys;
@@ -67,7 +70,6 @@ struct p2a_coord {
   p2a::DBuf<double> partial, red, stats, pos_buf, deriv_buf, out_buf;
   p2a::DBuf<int> redo_count;  // one counter per direction
   p2a::DBuf<int> scan_bsum;   // block sums of the list scans
-  p2a::DBuf<unsigned> ticket; // last-workgroup ticket of the reduction
   p2a::DBuf<long long> abs_dev;
   int64_t rebuilds = 0;
   bool profiling = false;
@@ -81,12 +83,23 @@ struct p2a_coord {
   p2a::DBuf<p2a::PbcArg> pbc_dev;
   double pbc_box[9] = {0};
   bool pbc_valid = false;
+  double path_box[9] = {0};  // box of the last path decision
+  int path_cached = -1;
 };
 
 namespace p2a {
 
+static int choose_path_uncached(p2a_coord* c, const double* box9);
+
 // decides the evaluation path for this box; 0 = cluster kernel
 static int choose_path(p2a_coord* c, const double* box9) {
+  if (c->path_cached >= 0 && std::memcmp(c->path_box, box9, sizeof(c->path_box)) == 0) return c->path_cached;
+  c->path_cached = choose_path_uncached(c, box9);
+  std::memcpy(c->path_box, box9, sizeof(c->path_box));
+  return c->path_cached;
+}
+
+static int choose_path_uncached(p2a_coord* c, const double* box9) {
   const p2a_switch& sw = c->spec.sw;
   if (c->spec.pair && c->spec.nb > 0) return 2;
   if (!c->spec.pbc || c->has_duplicates) return 1;
@@ -120,6 +133,7 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     G.S.colbase = 0;
     G.ncol = G.S.ncx * G.S.ncy;
     int maxcl = G.n / CL + G.ncol + 1;
+    G.maxcl = maxcl;
     if ((rc = G.sys.reserve(1)) || (rc = G.info.reserve(G.n)) || (rc = G.key.reserve(G.n)) ||
         (rc = G.col_count.reserve(G.ncol + 1)) || (rc = G.col_start.reserve(G.ncol + 1)) ||
         (rc = G.col_cl.reserve(G.ncol + 1)) || (rc = G.fill.reserve(G.ncol + 1)) || (rc = G.tmp.reserve(G.n)) ||
@@ -128,9 +142,13 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
         (rc = G.bb.reserve(2 * (size_t)maxcl)) || (rc = G.p.reserve((size_t)maxcl * CL)) ||
         (rc = G.kr.reserve(maxcl)) || (rc = G.abs_slot.reserve((size_t)maxcl * CL)))
       return rc;
-    // the previous step may still read the old descriptor
-    P2A_HIP(hipStreamSynchronize(s));
-    P2A_HIP(hipMemcpy(G.sys.p, &G.S, sizeof(SysArg), hipMemcpyHostToDevice));
+    // upload the descriptor when it changed (the previous step may still read the old one)
+    if (!G.S_valid || std::memcmp(&G.S, &G.S_dev, sizeof(SysArg)) != 0) {
+      P2A_HIP(hipStreamSynchronize(s));
+      P2A_HIP(hipMemcpy(G.sys.p, &G.S, sizeof(SysArg), hipMemcpyHostToDevice));
+      G.S_dev = G.S;
+      G.S_valid = true;
+    }
   }
   if ((rc = c->totals.reserve(16))) return rc;
   for (int gi = 0; gi < ngroups; gi++) {
@@ -152,16 +170,13 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     hipLaunchKernelGGL(k_cl_col, dim3(grid1(G.ncol, 256)), dim3(256), 0, s, G.ncol, G.col_cl.p, G.cl_col.p);
     if ((rc = launch_check("rebuild: binning"))) return rc;
   }
-  int hostTot[8];
-  P2A_HIP(hipMemcpyAsync(hostTot, c->totals.p, sizeof(hostTot), hipMemcpyDeviceToHost, s));
-  P2A_HIP(hipStreamSynchronize(s));
+  // cluster counts stay on the device (totals[5 + 2 g]) until the one host
+  // read below; the kernels in between are sized by the cluster capacity
   for (int gi = 0; gi < ngroups; gi++) {
     Group& G = c->g[gi];
-    G.ncl = hostTot[4 + 2 * gi + 1];
-    if (G.ncl > 0)
-      hipLaunchKernelGGL(k_origin, dim3(grid1(G.ncl, 128)), dim3(128), 0, s, G.ncl, (const int*)nullptr,
-                         G.slot_loc.p, G.info.p, G.key.p, pos_dev, G.off, (const SysArg*)G.sys.p,
-                         (const int*)nullptr, G.origin.p, G.bb0.p, G.kr.p, (int*)nullptr);
+    hipLaunchKernelGGL(k_origin, dim3(grid1(G.maxcl, 128)), dim3(128), 0, s, G.maxcl, (const int*)c->totals.p + 5 + 2 * gi,
+                       G.slot_loc.p, G.info.p, G.key.p, pos_dev, G.off, (const SysArg*)G.sys.p, (const int*)nullptr,
+                       G.origin.p, G.bb0.p, G.kr.p, (int*)nullptr);
   }
   // full cluster-pair lists: one direction for a single group, A->B (the
   // counting side) and B->A for two groups
@@ -177,29 +192,26 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     Group& GI = c->g[D.I];
     Group& GJ = c->g[D.J];
     ListArgs& L = Ls[d];
-    L.nclI = GI.ncl;
-    L.nclI_dev = nullptr;
+    const int* ncl_dev = c->totals.p + 5 + 2 * D.I;
+    L.nclI = GI.maxcl;
+    L.nclI_dev = ncl_dev;
     L.mode = (ngroups == 1) ? 0 : (d == 0 ? 1 : 2);
     L.ent_cap = 0;
     L.rlist2 = (float)(c->rlist * c->rlist);
-    if ((rc = D.ecount.reserve(GI.ncl + 1)) || (rc = D.estart.reserve(GI.ncl + 1)) ||
-        (rc = D.istart.reserve(GI.ncl + 1)))
+    if ((rc = D.ecount.reserve(GI.maxcl + 1)) || (rc = D.estart.reserve(GI.maxcl + 1)) ||
+        (rc = D.istart.reserve(GI.maxcl + 1)) || (rc = c->scan_bsum.reserve(2 * (size_t)grid1(GI.maxcl, kScanBlock) + 2)))
       return rc;
-    if (GI.ncl > 0) {
-      hipLaunchKernelGGL(k_list<0>, dim3(grid1(GI.ncl, wpb)), dim3(256), 0, s, L, GI.cl_col.p, (const int*)nullptr,
-                         (const SysArg*)GI.sys.p, (const SysArg*)GJ.sys.p, GI.bb0.p, GJ.bb0.p, GI.kr.p, GJ.kr.p,
-                         GJ.col_cl.p, GI.origin.p, GJ.origin.p, D.ecount.p, (const int*)nullptr, (Entry*)nullptr);
-      if ((rc = c->scan_bsum.reserve(2 * (size_t)grid1(GI.ncl, kScanBlock) + 2))) return rc;
-      launch_list_scans(s, GI.ncl, nullptr, D.ecount.p, chunk, c->scan_bsum.p, D.estart.p, D.istart.p, c->totals.p,
-                        8 + 2 * d, 9 + 2 * d);
-    } else {
-      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, c->totals.p + 8 + 2 * d, 2, 0);
-    }
+    hipLaunchKernelGGL(k_list<0>, dim3(grid1(GI.maxcl, wpb)), dim3(256), 0, s, L, GI.cl_col.p, (const int*)nullptr,
+                       (const SysArg*)GI.sys.p, (const SysArg*)GJ.sys.p, GI.bb0.p, GJ.bb0.p, GI.kr.p, GJ.kr.p,
+                       GJ.col_cl.p, GI.origin.p, GJ.origin.p, D.ecount.p, (const int*)nullptr, (Entry*)nullptr);
+    launch_list_scans(s, GI.maxcl, ncl_dev, D.ecount.p, chunk, c->scan_bsum.p, D.estart.p, D.istart.p, c->totals.p,
+                      8 + 2 * d, 9 + 2 * d);
   }
   if ((rc = launch_check("rebuild: list count"))) return rc;
   int hostT[16];
   P2A_HIP(hipMemcpyAsync(hostT, c->totals.p, sizeof(hostT), hipMemcpyDeviceToHost, s));
   P2A_HIP(hipStreamSynchronize(s));
+  for (int gi = 0; gi < ngroups; gi++) c->g[gi].ncl = hostT[5 + 2 * gi];
   c->nitems = 0;
   for (int d = 0; d < c->ndir; d++) {
     Dir& D = c->dir[d];
@@ -263,18 +275,32 @@ static int upload_pbc(p2a_coord* c, const double* box9) {
   return P2A_OK;
 }
 
-// deterministic sum of nred 8-double partials into value / virial / stats
-static int reduce_partials(p2a_coord* c, int nred, double* value_dev, double* virial_dev) {
+// deterministic sum of nred 8-double partials into value / virial / stats;
+// with gather_dir >= 0 the first stage also forms that direction's derivatives
+static int reduce_partials(p2a_coord* c, int nred, double* value_dev, double* virial_dev, int gather_dir = -1,
+                           double* deriv_dev = nullptr) {
   int rc;
   hipStream_t s = c->stream;
-  if (nred > 4 * kRedBlocks) {
-    if (!c->ticket.p) {
-      if ((rc = c->ticket.reserve(1))) return rc;
-      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, (int*)c->ticket.p, 1, 0);
-    }
+  int ngather = 0, ncl = 0;
+  const float4* gp = nullptr;
+  const int* istart = nullptr;
+  int items_cap = 0;
+  const double* fpart = nullptr;
+  if (gather_dir >= 0) {
+    Dir& D = c->dir[gather_dir];
+    Group& GI = c->g[D.I];
+    ncl = GI.ncl;
+    ngather = grid1((size_t)ncl * CL * 3, 256);
+    gp = GI.p.p;
+    istart = D.istart.p;
+    items_cap = (int)D.items.n;
+    fpart = D.fpart.p;
+  }
+  if (nred > 4 * kRedBlocks || ngather > 0) {
     if ((rc = c->red.reserve(8 * kRedBlocks))) return rc;
-    hipLaunchKernelGGL(k_reduce_all, dim3(kRedBlocks), dim3(256), 0, s, nred, c->partial.p, c->red.p, c->ticket.p,
-                       value_dev, virial_dev, c->stats.p);
+    hipLaunchKernelGGL(k_gather_reduce, dim3(ngather + kRedBlocks), dim3(256), 0, s, ngather, ncl,
+                       (const int*)nullptr, gp, istart, items_cap, fpart, deriv_dev, nred, c->partial.p, c->red.p);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(128), 0, s, kRedBlocks, c->red.p, value_dev, virial_dev, c->stats.p);
   } else {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(128), 0, s, nred, c->partial.p, value_dev, virial_dev, c->stats.p);
   }
@@ -338,14 +364,18 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     c->pending.emplace_back(e0, e1);
   }
   if ((rc = launch_check("pair kernel"))) return rc;
-  for (int d = 0; d < c->ndir && !value_only; d++) {
+  // derivatives of the last direction go with the first reduction stage
+  const int last = value_only ? -1 : c->ndir - 1;
+  for (int d = 0; d < last; d++) {
     Dir& D = c->dir[d];
     Group& GI = c->g[D.I];
     if (GI.ncl == 0) continue;
-    hipLaunchKernelGGL(k_gather, dim3(grid1((size_t)GI.ncl * CL * 3, 256)), dim3(256), 0, s, GI.ncl,
-                       (const int*)nullptr, GI.p.p, D.istart.p, (int)D.items.n, D.fpart.p, deriv_dev);
+    hipLaunchKernelGGL(k_gather_reduce, dim3(grid1((size_t)GI.ncl * CL * 3, 256)), dim3(256), 0, s,
+                       grid1((size_t)GI.ncl * CL * 3, 256), GI.ncl, (const int*)nullptr, GI.p.p, D.istart.p,
+                       (int)D.items.n, D.fpart.p, deriv_dev, 0, (const double*)nullptr, (double*)nullptr);
   }
-  if ((rc = reduce_partials(c, value_only ? c->dir[0].nitems : c->nitems, value_dev, virial_dev))) return rc;
+  if ((rc = reduce_partials(c, value_only ? c->dir[0].nitems : c->nitems, value_dev, virial_dev, last, deriv_dev)))
+    return rc;
   return launch_check("finalize");
 }
 

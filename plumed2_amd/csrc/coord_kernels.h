@@ -15,8 +15,9 @@
 //                     (CoordinationBase.cpp:178-207) with
 //                     fixedRational<N>::calculateSqr (SwitchingFunction.cpp:
 //                     203-215) inlined
-//   k_reduce_all,     the OpenMP critical merge + comm.Sum
-//   k_sys_reduce      (CoordinationBase.cpp:209-224)
+//   k_gather_reduce,  the OpenMP critical merge + comm.Sum
+//   k_finalize,       (CoordinationBase.cpp:209-224)
+//   k_sys_reduce
 //   k_allpairs,       the same loop in FP64 for NOPBC, PAIR, unbounded D_MAX,
 //   k_pairmode        cells narrower than 2*cutoff
 //
@@ -973,7 +974,7 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __
       continue;
     }
     // derivative of atom i: -sum over the 8 lanes sharing i (lanes ii + 8k);
-    // chunks of one cluster are combined by k_gather in a fixed order
+    // chunks of one cluster are combined by k_gather_reduce in a fixed order
     if constexpr (!VO) {
       double dfx = acc.fx, dfy = acc.fy, dfz = acc.fz;
       for (int o = 8; o < 64; o <<= 1) {
@@ -1001,22 +1002,6 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __
   }
 }
 
-// derivatives of the atoms of every i cluster: sum of its chunks' partials
-__global__ void k_gather(int ncl, const int* __restrict__ ncl_dev, const float4* __restrict__ p,
-                         const int* __restrict__ item_start, int items_cap, const double* __restrict__ fpart,
-                         double* __restrict__ deriv) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ncl_dev) ncl = *ncl_dev;
-  if (t >= ncl * CL * 3) return;
-  const int c = t / (CL * 3), r = t % (CL * 3);
-  const int req = __float_as_int(p[c * CL + r / 3].w);
-  if (req < 0) return;
-  double sum = 0.0;
-  const int i1 = min(item_start[c + 1], items_cap);
-  for (int it = item_start[c]; it < i1; it++) sum += fpart[(size_t)it * (CL * 3) + r];
-  deriv[3 * (size_t)req + r % 3] = sum;
-}
-
 // virial = -sum dd (x) d (CoordinationBase.cpp:200-206), symmetric
 __device__ __forceinline__ void write_virial(double* virial, const double* s6) {
   double xx = -s6[0], xy = -s6[1], xz = -s6[2], yy = -s6[3], yz = -s6[4], zz = -s6[5];
@@ -1027,19 +1012,36 @@ __device__ __forceinline__ void write_virial(double* virial, const double* s6) {
 
 constexpr int kRedBlocks = 128;
 
-// Deterministic sum of the per-item partials -> value, virial, pair count:
-// kRedBlocks workgroups reduce contiguous ranges, and the last workgroup to
-// finish sums the block partials in block order.  *ticket must be 0
-// before the launch; the last workgroup leaves it at 0.
-__global__ __launch_bounds__(256) void k_reduce_all(int nitems, const double* __restrict__ partial,
-                                                    double* __restrict__ blocks, unsigned* __restrict__ ticket,
-                                                    double* __restrict__ value, double* __restrict__ virial,
-                                                    double* __restrict__ stats_out) {
-  __shared__ double s[8][256];
-  __shared__ bool last;
+// After the pair passes, in one launch: workgroups [0, ngather) form the
+// derivatives of the atoms of every i cluster (fixed-order sums of the
+// cluster's item partials); the kRedBlocks workgroups after them sum
+// contiguous ranges of the item partials (value, virial, pair count) for
+// k_finalize.  Two roles in one grid save a kernel boundary.
+__global__ __launch_bounds__(256) void k_gather_reduce(int ngather, int ncl, const int* __restrict__ ncl_dev,
+                                                       const float4* __restrict__ p,
+                                                       const int* __restrict__ item_start, int items_cap,
+                                                       const double* __restrict__ fpart, double* __restrict__ deriv,
+                                                       int nitems, const double* __restrict__ partial,
+                                                       double* __restrict__ out) {
   const int t = threadIdx.x;
-  const int per = (nitems + gridDim.x - 1) / gridDim.x;
-  const int b = blockIdx.x * per, e = min(nitems, b + per);
+  if ((int)blockIdx.x < ngather) {
+    if (!deriv) return;
+    const int g = blockIdx.x * blockDim.x + t;
+    if (ncl_dev) ncl = *ncl_dev;
+    if (g >= ncl * CL * 3) return;
+    const int c = g / (CL * 3), r = g % (CL * 3);
+    const int req = __float_as_int(p[c * CL + r / 3].w);
+    if (req < 0) return;
+    double sum = 0.0;
+    const int i1 = min(item_start[c + 1], items_cap);
+    for (int it = item_start[c]; it < i1; it++) sum += fpart[(size_t)it * (CL * 3) + r];
+    deriv[3 * (size_t)req + r % 3] = sum;
+    return;
+  }
+  __shared__ double s[8][256];
+  const int blk = blockIdx.x - ngather, nblk = gridDim.x - ngather;
+  const int per = (nitems + nblk - 1) / nblk;
+  const int b = blk * per, e = min(nitems, b + per);
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const double4* p4 = reinterpret_cast<const double4*>(partial);
   for (int i = b + t; i < e; i += 256) {
@@ -1054,40 +1056,7 @@ __global__ __launch_bounds__(256) void k_reduce_all(int nitems, const double* __
       for (int q = 0; q < 8; q++) s[q][t] += s[q][t + o];
     __syncthreads();
   }
-  if (t < 8) blocks[(size_t)blockIdx.x * 8 + t] = s[t][0];
-  __threadfence();
-  __syncthreads();
-  if (t == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // thread t sums component t&7 of blocks t>>3, t>>3 + 32, ...: independent
-  // loads in flight together; then 32 fixed-order terms per component
-  __shared__ double s2[256];
-  {
-    const int q = t & 7;
-    double a = 0.0;
-    for (int i = t >> 3; i < (int)gridDim.x; i += 32)
-      a += __hip_atomic_load(&blocks[(size_t)i * 8 + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s2[t] = a;
-  }
-  __syncthreads();
-  if (t < 8) {
-    double a = 0.0;
-    for (int r = 0; r < 32; r++) a += s2[t + 8 * r];
-    s[t][0] = a;
-  }
-  __syncthreads();
-  if (t == 0) {
-    if (value) *value = s[0][0];
-    if (virial) {
-      double v6[6] = {s[1][0], s[2][0], s[3][0], s[4][0], s[5][0], s[6][0]};
-      write_virial(virial, v6);
-    }
-    stats_out[0] = s[7][0];
-    stats_out[1] += s[7][0];
-    *ticket = 0u;
-  }
+  if (t < 8) out[(size_t)blk * 8 + t] = s[t][0];
 }
 
 __global__ void k_finalize(int nitems, const double* __restrict__ partial, double* __restrict__ value,
