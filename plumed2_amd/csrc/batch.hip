@@ -56,7 +56,7 @@ struct p2a_batch {
   p2a::DBuf<float2> kr;
   p2a::DBuf<int> ecount, estart, istart, totals, redo, scan_bsum;
   p2a::DBuf<p2a::Entry> ents;
-  p2a::DBuf<int2> items;
+  p2a::DBuf<int4> items;
   p2a::DBuf<double> fpart, partial, npairs, out_values, out_virials, out_deriv;
   int* totals_host = nullptr;  // pinned
   // graph
@@ -124,8 +124,8 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
   hipLaunchKernelGGL(k_list<1>, dim3(grid1(clc, wpb)), dim3(256), 0, s, L, b->cl_col.p, (const int*)b->cl_sys.p,
                      (const SysArg*)b->sys.p, (const SysArg*)b->sys.p, b->bb0.p, b->bb0.p, b->kr.p, b->kr.p,
                      b->col_cl.p, b->origin.p, b->origin.p, b->ecount.p, (const int*)b->estart.p, b->ents.p);
-  hipLaunchKernelGGL(k_items, dim3(grid1(clc, 256)), dim3(256), 0, s, clc, ncl_dev, b->ecount.p, b->istart.p,
-                     b->items.p, chunk, b->items_cap);
+  hipLaunchKernelGGL(k_items, dim3(grid1(clc, 256)), dim3(256), 0, s, clc, ncl_dev, b->ecount.p, b->estart.p,
+                     b->istart.p, b->items.p, chunk, b->items_cap);
   // totals[4..5] serve as the pair kernels' redo counters
   hipLaunchKernelGGL(k_pack_bbox, dim3(grid1(nslot, 256)), dim3(256), 0, s, (int)nslot, ncl_dev, b->slot_loc.p,
                      b->info.p, pos, 0, (const SysArg*)b->sys.p, (const int*)b->atom_sys.p, b->origin.p, b->p.p,
@@ -146,7 +146,7 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
   A.redo_count = b->totals.p + 4;
   int grid = grid1(std::max(b->items_cap, 1), PAIR_BLOCK / WAVE);
   if (b->tune.blocks_per_cu > 0) grid = std::min(grid, b->tune.blocks_per_cu * b->cu_count);
-  launch_pairs(A, grid, std::min(grid, b->cu_count), s, false, chunk, b->items.p, b->estart.p, b->ents.p, b->p.p, b->p.p, b->bb.p, b->bb.p,
+  launch_pairs(A, grid, std::min(grid, b->cu_count), s, false, chunk, b->items.p, b->ents.p, b->p.p, b->p.p, b->bb.p, b->bb.p,
                (const int*)nullptr, (const int*)nullptr, pos, b->fpart.p, b->partial.p);
   if ((rc = launch_check("batch: pair kernel"))) return rc;
   hipLaunchKernelGGL(k_gather_reduce, dim3(grid1(nslot * 3, 256)), dim3(256), 0, s, grid1(nslot * 3, 256), clc,

@@ -21,7 +21,7 @@ struct Dir {  // one cluster-pair list: i clusters of group I against group J
   int I = 0, J = 0;
   DBuf<int> ecount, estart, istart;
   DBuf<Entry> ents;
-  DBuf<int2> items;
+  DBuf<int4> items;
   DBuf<double> fpart;  // per item: derivative partials of the 8 i atoms
   DBuf<int> redo;      // items redone with FP64 cut decisions
   int nitems = 0, item_offset = 0;
@@ -232,7 +232,7 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
                          GI.kr.p, GJ.kr.p, GJ.col_cl.p, GI.origin.p, GJ.origin.p, D.ecount.p,
                          (const int*)D.estart.p, D.ents.p);
       hipLaunchKernelGGL(k_items, dim3(grid1(GI.ncl, 256)), dim3(256), 0, s, GI.ncl, (const int*)nullptr,
-                         D.ecount.p, D.istart.p, D.items.p, chunk, (int)D.items.n);
+                         D.ecount.p, D.estart.p, D.istart.p, D.items.p, chunk, (int)D.items.n);
     }
   }
   c->nentries = tot_entries;
@@ -356,7 +356,7 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     if (c->tune.blocks_per_cu > 0) grid = std::min(grid, c->tune.blocks_per_cu * c->cu_count);
     A.redo = D.redo.p;
     A.redo_count = c->redo_count.p + d;
-    launch_pairs(A, grid, std::min(grid, c->cu_count), s, value_only, c->tune.chunk, D.items.p, D.estart.p, D.ents.p, GI.p.p, GJ.p.p, GI.bb.p,
+    launch_pairs(A, grid, std::min(grid, c->cu_count), s, value_only, c->tune.chunk, D.items.p, D.ents.p, GI.p.p, GJ.p.p, GI.bb.p,
                  GJ.bb.p, GI.abs_slot.p, GJ.abs_slot.p, pos_dev, D.fpart.p, c->partial.p + 8 * (size_t)D.item_offset);
   }
   if (c->profiling) {

@@ -576,16 +576,19 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
   if (!FILL && lane == 0) count[ci] = n;
 }
 
-// work items (i cluster, chunk of `chunk` entries); writes past items_cap are dropped
+// work items {i cluster, first entry, end entry} of `chunk` entries each;
+// writes past items_cap are dropped
 __global__ void k_items(int ncl, const int* __restrict__ ncl_dev, const int* __restrict__ count,
-                        const int* __restrict__ item_start, int2* __restrict__ items, int chunk, int items_cap) {
+                        const int* __restrict__ estart, const int* __restrict__ item_start, int4* __restrict__ items,
+                        int chunk, int items_cap) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (ncl_dev) ncl = *ncl_dev;
   if (c >= ncl) return;
-  int nch = (count[c] + chunk - 1) / chunk;
+  const int n = count[c], e = estart[c];
+  int nch = (n + chunk - 1) / chunk;
   for (int k = 0; k < nch; k++) {
     const int it = item_start[c] + k;
-    if (it < items_cap) items[it] = make_int2(c, k);
+    if (it < items_cap) items[it] = make_int4(c, e + k * chunk, e + min(n, (k + 1) * chunk), 0);
   }
 }
 
@@ -840,8 +843,7 @@ __device__ __forceinline__ void eval_group(const PairArgs& A, const float4* __re
 // redoes those items with the FP64 cut decision inline (it needs many more
 // registers, which is why it is a separate kernel).
 template <int P, bool GENERIC, bool TWO, bool VO, int CH, bool EXACT>
-__global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __restrict__ items,
-                                                       const int* __restrict__ estart,
+__global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int4* __restrict__ items,
                                                        const Entry* __restrict__ ents,
                                                        const float4* __restrict__ posI,
                                                        const float4* __restrict__ posJ,
@@ -863,12 +865,30 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __
   int* scw = scode[w];
   int* saw = TWO ? sabs[w] : nullptr;
 
-  for (int k = blockIdx.x * NW + w; k < nitems; k += gridDim.x * NW) {
-    const int item = EXACT ? A.redo[k] : k;
-    const int2 it = items[item];
+  // the next item's descriptor and entries are loaded while the current item
+  // computes (a wave handles several items when the grid is persistent)
+  const int kstride = gridDim.x * NW;
+  int k = blockIdx.x * NW + w;
+  int item_n = 0;
+  int4 it_n = make_int4(0, 0, 0, 0);
+  Entry my_n = {0, 0.f, 0.f, 0.f};
+  if (k < nitems) {
+    item_n = EXACT ? A.redo[k] : k;
+    it_n = items[item_n];
+    const int e = it_n.y + lane;
+    if (lane < CH && e < min(it_n.z, A.ent_cap)) my_n = ents[e];
+  }
+  for (; k < nitems; k += kstride) {
+    const int item = item_n;
+    const int4 it = it_n;
+    const Entry my_e = my_n;
+    const int kn = k + kstride;
+    if (kn < nitems) {
+      item_n = EXACT ? A.redo[kn] : kn;
+      it_n = items[item_n];
+    }
     const int ci = it.x;
-    const int e0 = estart[ci] + it.y * CH;
-    const int e1 = min(min(estart[ci + 1], e0 + CH), A.ent_cap);
+    const int e1 = min(it.z, A.ent_cap);
     const float4 pi = posI[ci * CL + ii];
     const int reqi = __float_as_int(pi.w);
     const int ai = TWO ? absI[ci * CL + ii] : 0;
@@ -879,9 +899,9 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __
     Entry my = {0, 0.f, 0.f, 0.f};
     bool pass = false;
     {
-      const int e = e0 + lane;
+      const int e = it.y + lane;
       if (lane < CH && e < e1) {
-        my = ents[e];
+        my = my_e;
         if (!VO || (my.code & kCountBit) || my.code < 0) {
           const int cj = my.code & kCjMask;
           const float4 jc = bbJ[2 * cj], jh = bbJ[2 * cj + 1];
@@ -953,6 +973,10 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int2* __
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
 
+    if (kn < nitems) {
+      const int e = it_n.y + lane;
+      my_n = (lane < CH && e < min(it_n.z, A.ent_cap)) ? ents[e] : Entry{0, 0.f, 0.f, 0.f};
+    }
     PairAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0};
     bool band_hit = false;  // uniform
     if (n0)
@@ -1294,16 +1318,16 @@ struct PairTuning {
 // over all items, then the FP64-decision pass over the items it set aside
 // (A.redo / A.redo_count must be set; exact_grid workgroups loop over them)
 inline void launch_pairs(const PairArgs& A, int grid, int exact_grid, hipStream_t s, bool value_only, int chunk,
-                         const int2* items, const int* estart, const Entry* ents, const float4* posI,
+                         const int4* items, const Entry* ents, const float4* posI,
                          const float4* posJ, const float4* bbI, const float4* bbJ, const int* absI, const int* absJ,
                          const double* pos64, double* fpart, double* partial) {
   const p2a_switch& sw = A.sw;
 #define P2A_K(P, GEN, TWO, VO, CH)                                                                                \
   do {                                                                                                            \
-    hipLaunchKernelGGL((k_pairs<P, GEN, TWO, VO, CH, false>), dim3(grid), dim3(PAIR_BLOCK), 0, s, A, items, estart, \
-                       ents, posI, posJ, bbI, bbJ, absI, absJ, pos64, fpart, partial);                            \
+    hipLaunchKernelGGL((k_pairs<P, GEN, TWO, VO, CH, false>), dim3(grid), dim3(PAIR_BLOCK), 0, s, A, items, ents,  \
+                       posI, posJ, bbI, bbJ, absI, absJ, pos64, fpart, partial);                                  \
     hipLaunchKernelGGL((k_pairs<P, GEN, TWO, VO, CH, true>), dim3(exact_grid), dim3(PAIR_BLOCK), 0, s, A, items,   \
-                       estart, ents, posI, posJ, bbI, bbJ, absI, absJ, pos64, fpart, partial);                    \
+                       ents, posI, posJ, bbI, bbJ, absI, absJ, pos64, fpart, partial);                            \
   } while (0)
 #define P2A_KCH(P, GEN, TWO, VO)          \
   do {                                    \
