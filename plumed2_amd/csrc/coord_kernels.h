@@ -458,7 +458,7 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
                                               const double4* __restrict__ orgI, const double4* __restrict__ orgJ,
                                               int* __restrict__ count, const int* __restrict__ start,
                                               Entry* __restrict__ out) {
-  __shared__ int s_first[4][WAVE], s_off[4][WAVE], s_img[4][WAVE];
+  __shared__ int s_first[4][3][WAVE], s_n[4][2][WAVE], s_off[4][WAVE], s_img[4][WAVE];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x / WAVE;
   const int ci = blockIdx.x * (blockDim.x / WAVE) + wv;
@@ -480,43 +480,49 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
   const int xa = max(x0, -SJ.ncx), xb = min(x1, 2 * SJ.ncx - 1);
   const int ya = max(y0, -SJ.ncy), yb = min(y1, 2 * SJ.ncy - 1);
   const int nx = xb - xa + 1, ny = yb - ya + 1;
-  const int ntup = nx * ny * 3;
+  const int ncolw = nx * ny;
   const float zlo = ki.x - SJ.rb[2] - 1.0e-6f, zhi = ki.y + SJ.rb[2] + 1.0e-6f;
   const double4 oi = orgI[ci];
   int n = 0;
   const int base = FILL ? start[ci] : 0;
-  for (int t0 = 0; t0 < ntup; t0 += WAVE) {
+  // one lane per window column; it covers the images -1, 0, +1 along c, of
+  // which only those whose shifted window meets [0, 1) are searched
+  for (int t0 = 0; t0 < ncolw; t0 += WAVE) {
     const int t = t0 + lane;
-    int first = 0, cnt = 0, img = 0;
-    if (t < ntup) {
-      const int mzi = t % 3, r = t / 3;
-      const int cxu = xa + r % nx, cyu = ya + r / nx;
+    int f[3] = {0, 0, 0}, cnt[3] = {0, 0, 0}, img = 0;
+    if (t < ncolw) {
+      const int cxu = xa + t % nx, cyu = ya + t / nx;
       const int mx = floordiv(cxu, SJ.ncx), cxw = cxu - mx * SJ.ncx;
       const int my = floordiv(cyu, SJ.ncy), cyw = cyu - my * SJ.ncy;
       const int col = SJ.colbase + cxw + SJ.ncx * cyw;
       const int c0 = col_clJ[col], c1 = col_clJ[col + 1];
-      const float lo = zlo - (float)(mzi - 1), hi = zhi - (float)(mzi - 1);
-      if (hi >= 0.f && lo < 1.f && c1 > c0) {
-        int a = c0, b = c1;
-        while (a < b) {  // first cluster whose upper key reaches lo
-          const int m = (a + b) >> 1;
-          if (krJ[m].y < lo) a = m + 1;
-          else b = m;
+      img = (mx + 1) | ((my + 1) << 2);
+#pragma unroll
+      for (int m = 0; m < 3; m++) {
+        const float lo = zlo - (float)(m - 1), hi = zhi - (float)(m - 1);
+        if (hi >= 0.f && lo < 1.f && c1 > c0) {
+          int a = c0, b = c1;
+          while (a < b) {  // first cluster whose upper key reaches lo
+            const int q = (a + b) >> 1;
+            if (krJ[q].y < lo) a = q + 1;
+            else b = q;
+          }
+          f[m] = a;
+          b = c1;
+          while (a < b) {  // first cluster whose lower key exceeds hi
+            const int q = (a + b) >> 1;
+            if (krJ[q].x <= hi) a = q + 1;
+            else b = q;
+          }
+          cnt[m] = a - f[m];
         }
-        first = a;
-        b = c1;
-        while (a < b) {  // first cluster whose lower key exceeds hi
-          const int m = (a + b) >> 1;
-          if (krJ[m].x <= hi) a = m + 1;
-          else b = m;
-        }
-        cnt = a - first;
       }
-      img = (mx + 1) | ((my + 1) << 2) | (mzi << 4);
     }
     int total;
-    const int off = wave_excl_scan(cnt, lane, total);
-    s_first[wv][lane] = first;
+    const int off = wave_excl_scan(cnt[0] + cnt[1] + cnt[2], lane, total);
+    for (int m = 0; m < 3; m++) s_first[wv][m][lane] = f[m];
+    s_n[wv][0][lane] = cnt[0];
+    s_n[wv][1][lane] = cnt[1];
     s_off[wv][lane] = off;
     s_img[wv][lane] = img;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -526,17 +532,28 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
       bool pass = false;
       int cj = 0, mx = 0, my = 0, mz = 0;
       if (u < total) {
-        int a = 0, b = WAVE - 1;  // last tuple whose offset is <= u
+        int a = 0, b = WAVE - 1;  // last column lane whose offset is <= u
         while (a < b) {
-          const int m = (a + b + 1) >> 1;
-          if (s_off[wv][m] <= u) a = m;
-          else b = m - 1;
+          const int q = (a + b + 1) >> 1;
+          if (s_off[wv][q] <= u) a = q;
+          else b = q - 1;
         }
-        cj = s_first[wv][a] + (u - s_off[wv][a]);
+        int r = u - s_off[wv][a];
+        const int n0 = s_n[wv][0][a], n1 = s_n[wv][1][a];
+        int m = 0;
+        if (r >= n0) {
+          r -= n0;
+          m = 1;
+          if (r >= n1) {
+            r -= n1;
+            m = 2;
+          }
+        }
+        cj = s_first[wv][m][a] + r;
         const int im = s_img[wv][a];
         mx = (im & 3) - 1;
         my = ((im >> 2) & 3) - 1;
-        mz = ((im >> 4) & 3) - 1;
+        mz = m - 1;
         const float Tx = mx * SJ.af[0] + my * SJ.af[3] + mz * SJ.af[6];
         const float Ty = mx * SJ.af[1] + my * SJ.af[4] + mz * SJ.af[7];
         const float Tz = mx * SJ.af[2] + my * SJ.af[5] + mz * SJ.af[8];
@@ -569,7 +586,7 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
       }
       n += __popcll(bal);
     }
-    // the next tuple chunk overwrites the tables read above
+    // the next column chunk overwrites the tables read above
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
