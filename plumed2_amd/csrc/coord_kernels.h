@@ -749,6 +749,12 @@ __device__ double exact_r2(const PbcArg* __restrict__ P, const double* __restric
   return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
 }
 
+// a staged j atom in LDS: position in the i cluster's frame (12 B); its
+// origin (entry << 3 | slot, -1 for padding) is kept in a separate short
+struct P3 {
+  float x, y, z;
+};
+
 struct PairAcc {
   float cv, fx, fy, fz, vxx, vxy, vxz, vyy, vyz, vzz;
   int npair;
@@ -762,13 +768,14 @@ struct PairAcc {
 // decision: EXACT=false only raises `band_hit` (the item is then redone by
 // the EXACT=true kernel), EXACT=true decides it on the spot.
 template <int P, bool GENERIC, bool TWO, bool VO, int KIND, bool EXACT>
-__device__ __forceinline__ void eval_group(const PairArgs& A, const float4* __restrict__ sjw,
-                                           const int* __restrict__ scw, const int* __restrict__ saw, int g,
+__device__ __forceinline__ void eval_group(const PairArgs& A, const P3* __restrict__ sjw,
+                                           const short* __restrict__ smw, const int* __restrict__ scw,
+                                           const int* __restrict__ saw, int g,
                                            int ii, int jj, const float4& pi, int reqi, int ai,
                                            const float4* __restrict__ posJ, const PbcArg* __restrict__ pbc,
                                            const double* __restrict__ pos64, float band, bool& band_hit,
                                            PairAcc& acc) {
-  const float4 pj = sjw[g + jj];
+  const P3 pj = sjw[g + jj];
   const float dx = pj.x - pi.x;
   const float dy = pj.y - pi.y;
   const float dz = pj.z - pi.z;
@@ -776,7 +783,7 @@ __device__ __forceinline__ void eval_group(const PairArgs& A, const float4* __re
   bool cut = r2 <= A.dmax2;
   if (__builtin_expect(__ballot(fabsf(r2 - A.dmax2) < band) != 0ull, 0)) {
     if constexpr (EXACT) {
-      const int meta = __float_as_int(pj.w);  // (entry << 3 | slot), -1 for padding
+      const int meta = smw[g + jj];  // (entry << 3 | slot), -1 for padding
       if (fabsf(r2 - A.dmax2) < band && reqi >= 0 && meta >= 0) {
         const int cj = scw[meta >> 3] & kCjMask;
         const int reqj = __float_as_int(posJ[cj * CL + (meta & 7)].w);
@@ -860,7 +867,7 @@ __device__ __forceinline__ void eval_group(const PairArgs& A, const float4* __re
 // redoes those items with the FP64 cut decision inline (it needs many more
 // registers, which is why it is a separate kernel).
 template <int P, bool GENERIC, bool TWO, bool VO, int CH, bool EXACT>
-__global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int4* __restrict__ items,
+__global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : 5))) void k_pairs(PairArgs A, const int4* __restrict__ items,
                                                        const Entry* __restrict__ ents,
                                                        const float4* __restrict__ posI,
                                                        const float4* __restrict__ posJ,
@@ -870,7 +877,8 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int4* __
                                                        double* __restrict__ fpart, double* __restrict__ partial) {
   constexpr int NW = PAIR_BLOCK / WAVE;
   constexpr int CAP = CH * CL + 3 * CL;
-  __shared__ float4 sj[NW][CAP];
+  __shared__ P3 sj[NW][CAP];
+  __shared__ short sm[NW][CAP];
   __shared__ int scode[NW][CH];
   __shared__ int sabs[TWO ? NW : 1][TWO ? CAP : 1];
   const int lane = threadIdx.x & 63;
@@ -878,7 +886,8 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int4* __
   const int ii = lane & 7, jj = lane >> 3;
   const int nitems = EXACT ? *A.redo_count : (A.nitems_dev ? min(*A.nitems_dev, A.items_cap) : A.nitems);
   const float band = 4.0e-6f * A.dmax2;
-  float4* sjw = sj[w];
+  P3* sjw = sj[w];
+  short* smw = sm[w];
   int* scw = scode[w];
   int* saw = TWO ? sabs[w] : nullptr;
 
@@ -968,7 +977,8 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int4* __
 #pragma unroll
       for (int t = 0; t < CL; t++) {
         if ((m8 >> t) & 1) {
-          sjw[p] = make_float4(q[t].x, q[t].y, q[t].z, __int_as_float((lane << 3) | t));
+          sjw[p] = P3{q[t].x, q[t].y, q[t].z};
+          smw[p] = (short)((lane << 3) | t);
           if constexpr (TWO) saw[p] = absJ[cj * CL + t];
           p++;
         }
@@ -976,13 +986,15 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int4* __
     }
     // padding of each kind up to a multiple of 8: far away, never within D_MAX
     if (lane < CL) {
-      const float4 far = make_float4(1.0e5f, 1.0e5f, 1.0e5f, __int_as_float(-1));
+      const P3 far = {1.0e5f, 1.0e5f, 1.0e5f};
       if (tot1 + lane < n1) {
         sjw[b1 + tot1 + lane] = far;
+        smw[b1 + tot1 + lane] = -1;
         if constexpr (TWO) saw[b1 + tot1 + lane] = -1;
       }
       if (tot2 + lane < n2) {
         sjw[b2 + tot2 + lane] = far;
+        smw[b2 + tot2 + lane] = -1;
         if constexpr (TWO) saw[b2 + tot2 + lane] = -1;
       }
     }
@@ -997,14 +1009,14 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_pairs(PairArgs A, const int4* __
     PairAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0};
     bool band_hit = false;  // uniform
     if (n0)
-      eval_group<P, GENERIC, TWO, VO, 0, EXACT>(A, sjw, scw, saw, 0, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band,
+      eval_group<P, GENERIC, TWO, VO, 0, EXACT>(A, sjw, smw, scw, saw, 0, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band,
                                                 band_hit, acc);
     for (int g = b1; g < b1 + n1; g += CL)
-      eval_group<P, GENERIC, TWO, VO, 1, EXACT>(A, sjw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band,
+      eval_group<P, GENERIC, TWO, VO, 1, EXACT>(A, sjw, smw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band,
                                                 band_hit, acc);
     if constexpr (!VO) {
       for (int g = b2; g < b2 + n2; g += CL)
-        eval_group<P, GENERIC, TWO, VO, 2, EXACT>(A, sjw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band,
+        eval_group<P, GENERIC, TWO, VO, 2, EXACT>(A, sjw, smw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band,
                                                   band_hit, acc);
     }
     if (!EXACT && band_hit) {
