@@ -55,6 +55,7 @@ struct p2a_batch {
   p2a::DBuf<float4> bb0, bb, p;
   p2a::DBuf<float2> kr;
   p2a::DBuf<int> ecount, estart, istart, totals, redo, scan_bsum;
+  p2a::DBuf<unsigned> ell;  // candidates the list count pass keeps for the fill pass
   p2a::DBuf<p2a::Entry> ents;
   p2a::DBuf<int4> items;
   p2a::DBuf<double> fpart, partial, npairs, out_values, out_virials, out_deriv;
@@ -119,11 +120,11 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
   const int wpb = 256 / WAVE;
   hipLaunchKernelGGL(k_list<0>, dim3(grid1(clc, wpb)), dim3(256), 0, s, L, b->cl_col.p, (const int*)b->cl_sys.p,
                      (const SysArg*)b->sys.p, (const SysArg*)b->sys.p, b->bb0.p, b->bb0.p, b->kr.p, b->kr.p,
-                     b->col_cl.p, b->origin.p, b->origin.p, b->ecount.p, (const int*)nullptr, (Entry*)nullptr);
+                     b->col_cl.p, b->origin.p, b->origin.p, b->ecount.p, (const int*)nullptr, (Entry*)nullptr, b->ell.p);
   launch_list_scans(s, clc, ncl_dev, b->ecount.p, chunk, b->scan_bsum.p, b->estart.p, b->istart.p, b->totals.p, 2, 3);
   hipLaunchKernelGGL(k_list<1>, dim3(grid1(clc, wpb)), dim3(256), 0, s, L, b->cl_col.p, (const int*)b->cl_sys.p,
                      (const SysArg*)b->sys.p, (const SysArg*)b->sys.p, b->bb0.p, b->bb0.p, b->kr.p, b->kr.p,
-                     b->col_cl.p, b->origin.p, b->origin.p, b->ecount.p, (const int*)b->estart.p, b->ents.p);
+                     b->col_cl.p, b->origin.p, b->origin.p, b->ecount.p, (const int*)b->estart.p, b->ents.p, b->ell.p);
   hipLaunchKernelGGL(k_items, dim3(grid1(clc, 256)), dim3(256), 0, s, clc, ncl_dev, b->ecount.p, b->estart.p,
                      b->istart.p, b->items.p, chunk, b->items_cap);
   // totals[4..5] serve as the pair kernels' redo counters
@@ -345,7 +346,8 @@ int p2a_batch_set_boxes(p2a_batch* b, const double* boxes) {
       (rc = b->origin.reserve(clc)) || (rc = b->bb0.reserve(2 * (size_t)clc)) || (rc = b->bb.reserve(2 * (size_t)clc)) ||
       (rc = b->p.reserve((size_t)clc * CL)) || (rc = b->kr.reserve(clc)) || (rc = b->ecount.reserve(clc + 1)) ||
       (rc = b->estart.reserve(clc + 1)) || (rc = b->istart.reserve(clc + 1)) ||
-      (rc = b->scan_bsum.reserve(2 * (size_t)grid1(clc, kScanBlock) + 2)))
+      (rc = b->scan_bsum.reserve(2 * (size_t)grid1(clc, kScanBlock) + 2)) ||
+      (rc = b->ell.reserve((size_t)clc * kEll)))
     return rc;
   P2A_HIP(hipMemcpy(b->sys.p, S.data(), S.size() * sizeof(SysArg), hipMemcpyHostToDevice));
   P2A_HIP(hipMemcpy(b->pbc.p, Pb.data(), Pb.size() * sizeof(PbcArg), hipMemcpyHostToDevice));

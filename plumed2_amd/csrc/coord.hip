@@ -24,6 +24,7 @@ struct Dir {  // one cluster-pair list: i clusters of group I against group J
   DBuf<int4> items;
   DBuf<double> fpart;  // per item: derivative partials of the 8 i atoms
   DBuf<int> redo;      // items redone with FP64 cut decisions
+  DBuf<unsigned> ell;  // candidates the list count pass keeps for the fill pass
   int nitems = 0, item_offset = 0;
   int64_t nentries = 0;
 };
@@ -199,11 +200,12 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     L.ent_cap = 0;
     L.rlist2 = (float)(c->rlist * c->rlist);
     if ((rc = D.ecount.reserve(GI.maxcl + 1)) || (rc = D.estart.reserve(GI.maxcl + 1)) ||
-        (rc = D.istart.reserve(GI.maxcl + 1)) || (rc = c->scan_bsum.reserve(2 * (size_t)grid1(GI.maxcl, kScanBlock) + 2)))
+        (rc = D.istart.reserve(GI.maxcl + 1)) || (rc = c->scan_bsum.reserve(2 * (size_t)grid1(GI.maxcl, kScanBlock) + 2)) ||
+        (rc = D.ell.reserve((size_t)GI.maxcl * kEll)))
       return rc;
     hipLaunchKernelGGL(k_list<0>, dim3(grid1(GI.maxcl, wpb)), dim3(256), 0, s, L, GI.cl_col.p, (const int*)nullptr,
                        (const SysArg*)GI.sys.p, (const SysArg*)GJ.sys.p, GI.bb0.p, GJ.bb0.p, GI.kr.p, GJ.kr.p,
-                       GJ.col_cl.p, GI.origin.p, GJ.origin.p, D.ecount.p, (const int*)nullptr, (Entry*)nullptr);
+                       GJ.col_cl.p, GI.origin.p, GJ.origin.p, D.ecount.p, (const int*)nullptr, (Entry*)nullptr, D.ell.p);
     launch_list_scans(s, GI.maxcl, ncl_dev, D.ecount.p, chunk, c->scan_bsum.p, D.estart.p, D.istart.p, c->totals.p,
                       8 + 2 * d, 9 + 2 * d);
   }
@@ -230,7 +232,7 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
       hipLaunchKernelGGL(k_list<1>, dim3(grid1(GI.ncl, wpb)), dim3(256), 0, s, Ls[d], GI.cl_col.p,
                          (const int*)nullptr, (const SysArg*)GI.sys.p, (const SysArg*)GJ.sys.p, GI.bb0.p, GJ.bb0.p,
                          GI.kr.p, GJ.kr.p, GJ.col_cl.p, GI.origin.p, GJ.origin.p, D.ecount.p,
-                         (const int*)D.estart.p, D.ents.p);
+                         (const int*)D.estart.p, D.ents.p, D.ell.p);
       hipLaunchKernelGGL(k_items, dim3(grid1(GI.ncl, 256)), dim3(256), 0, s, GI.ncl, (const int*)nullptr,
                          D.ecount.p, D.estart.p, D.istart.p, D.items.p, chunk, (int)D.items.n);
     }

@@ -435,6 +435,30 @@ __global__ void k_origin(int ncl, const int* __restrict__ ncl_dev, const int* __
   kr[c] = make_float2((float)klo - 1.0e-6f, (float)khi + 1.0e-6f);
 }
 
+// candidates kept per i cluster between the count and fill passes of the
+// list search (j cluster in bits 0-25, image code (mx+1)+3(my+1)+9(mz+1) above)
+constexpr int kEll = 384;
+constexpr unsigned kEllCj = 0x3FFFFFFu;
+
+// list entry of i cluster ci for j cluster cj seen through image (mx,my,mz)
+__device__ __forceinline__ Entry make_entry(int mode, int ci, int cj, int mx, int my, int mz, const SysArg& SJ,
+                                            const double4* __restrict__ orgJ, const double4& oi) {
+  const double4 oj = orgJ[cj];
+  const double Tx = mx * SJ.a[0] + my * SJ.a[3] + mz * SJ.a[6];
+  const double Ty = mx * SJ.a[1] + my * SJ.a[4] + mz * SJ.a[7];
+  const double Tz = mx * SJ.a[2] + my * SJ.a[5] + mz * SJ.a[8];
+  const bool zero = (mx == 0 && my == 0 && mz == 0);
+  const bool diag = mode == 0 && cj == ci && zero;
+  const bool positive = (mx > 0) || (mx == 0 && (my > 0 || (my == 0 && mz > 0)));
+  const bool counted = mode == 1 || (mode == 0 && (cj > ci || (cj == ci && positive)));
+  Entry en;
+  en.code = cj | (counted ? kCountBit : 0) | (diag ? (int)0x80000000u : 0);
+  en.dx = (float)(oj.x + Tx - oi.x);
+  en.dy = (float)(oj.y + Ty - oi.y);
+  en.dz = (float)(oj.z + Tz - oi.z);
+  return en;
+}
+
 struct ListArgs {
   int nclI;
   const int* nclI_dev;  // device count of i clusters (null: nclI)
@@ -443,7 +467,10 @@ struct ListArgs {
   float rlist2;
 };
 
-// Cluster-pair search: one wave per i cluster.  FILL=0 counts, FILL=1 writes.
+// Cluster-pair search: one wave per i cluster.  FILL=0 counts and keeps the
+// first kEll passing candidates of the cluster (j cluster | image code) in
+// `ell`; FILL=1 writes the entries, streaming them from `ell` when the
+// cluster's count fits there and searching again otherwise.
 // Candidates are the clusters of the (column, image) window around the i
 // cluster whose range of fractional c overlaps the i cluster's range widened
 // by the list radius: a pair closer than r differs by at most r*|c*| in
@@ -457,7 +484,7 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
                                               const float2* __restrict__ krJ, const int* __restrict__ col_clJ,
                                               const double4* __restrict__ orgI, const double4* __restrict__ orgJ,
                                               int* __restrict__ count, const int* __restrict__ start,
-                                              Entry* __restrict__ out) {
+                                              Entry* __restrict__ out, unsigned* __restrict__ ell) {
   __shared__ int s_first[4][3][WAVE], s_n[4][2][WAVE], s_off[4][WAVE], s_img[4][WAVE];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x / WAVE;
@@ -485,6 +512,18 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
   const double4 oi = orgI[ci];
   int n = 0;
   const int base = FILL ? start[ci] : 0;
+  if (FILL) {
+    const int nc = count[ci];
+    if (nc <= kEll) {
+      for (int e = lane; e < nc; e += WAVE) {
+        const unsigned pk = ell[(size_t)ci * kEll + e];
+        const int cj = (int)(pk & kEllCj), im = (int)(pk >> 26);
+        if (base + e < L.ent_cap)
+          out[base + e] = make_entry(L.mode, ci, cj, im % 3 - 1, (im / 3) % 3 - 1, im / 9 - 1, SJ, orgJ, oi);
+      }
+      return;
+    }
+  }
   // one lane per window column; it covers the images -1, 0, +1 along c, of
   // which only those whose shifted window meets [0, 1) are searched
   for (int t0 = 0; t0 < ncolw; t0 += WAVE) {
@@ -564,24 +603,12 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
         pass = gx * gx + gy * gy + gz * gz < L.rlist2;
       }
       const unsigned long long bal = __ballot(pass);
-      if (FILL && pass) {
-        const int k = __popcll(bal & ((1ull << lane) - 1));
-        const int idx = base + n + k;
-        if (idx < L.ent_cap) {
-          const double4 oj = orgJ[cj];
-          const double Tx = mx * SJ.a[0] + my * SJ.a[3] + mz * SJ.a[6];
-          const double Ty = mx * SJ.a[1] + my * SJ.a[4] + mz * SJ.a[7];
-          const double Tz = mx * SJ.a[2] + my * SJ.a[5] + mz * SJ.a[8];
-          const bool zero = (mx == 0 && my == 0 && mz == 0);
-          const bool diag = L.mode == 0 && cj == ci && zero;
-          const bool positive = (mx > 0) || (mx == 0 && (my > 0 || (my == 0 && mz > 0)));
-          const bool counted = L.mode == 1 || (L.mode == 0 && (cj > ci || (cj == ci && positive)));
-          Entry en;
-          en.code = cj | (counted ? kCountBit : 0) | (diag ? (int)0x80000000u : 0);
-          en.dx = (float)(oj.x + Tx - oi.x);
-          en.dy = (float)(oj.y + Ty - oi.y);
-          en.dz = (float)(oj.z + Tz - oi.z);
-          out[idx] = en;
+      if (pass) {
+        const int k = n + __popcll(bal & ((1ull << lane) - 1));
+        if (FILL) {
+          if (base + k < L.ent_cap) out[base + k] = make_entry(L.mode, ci, cj, mx, my, mz, SJ, orgJ, oi);
+        } else if (k < kEll) {
+          ell[(size_t)ci * kEll + k] = (unsigned)cj | ((unsigned)((mx + 1) + 3 * (my + 1) + 9 * (mz + 1)) << 26);
         }
       }
       n += __popcll(bal);

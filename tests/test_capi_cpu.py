@@ -111,3 +111,24 @@ def test_context_creation_fails_loudly_without_gpu():
         pytest.skip("a GPU is visible")
     with pytest.raises(P.Plumed2AmdError, match="no HIP device"):
         P.Coordination("c: COORDINATION GROUPA=1-10 R_0=1")
+
+
+def test_batch_validates_arguments_and_needs_a_device():
+    import ctypes as C
+
+    import numpy as np
+
+    import plumed2_amd as P
+    from plumed2_amd import _capi as A
+    sw = P.parse_switch("RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2")
+    b = C.c_void_p()
+    nat = np.array([3000, 0], dtype=np.int64)
+    assert A.lib().p2a_batch_create(C.byref(b), 0, 2, nat.ctypes.data, C.byref(sw), 1.2) == A.P2A_ERR_ARG
+    assert b"system 1 has no atoms" in A.lib().p2a_last_error()
+    nat = np.array([3000, 4000], dtype=np.int64)
+    assert A.lib().p2a_batch_create(C.byref(b), 0, 2, nat.ctypes.data, C.byref(sw), 1.0) == A.P2A_ERR_UNSUPPORTED
+    assert b"smaller than D_MAX" in A.lib().p2a_last_error()
+    if P.lib().p2a_device_count() > 0:
+        return
+    with pytest.raises(P.Plumed2AmdError, match="no HIP device"):
+        P.CoordinationBatch([3000, 4000], "RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2", 1.2)

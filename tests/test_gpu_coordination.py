@@ -334,3 +334,20 @@ def test_lattice_pairs_exactly_at_dmax(chunk):
     assert act.stats()["contributing_pairs"] == npair
     check(v, d, w, ov, od, ow)
     act.close()
+
+
+def test_dense_long_list_radius():
+    """100 atoms/nm^3 with NL_CUTOFF=1.6: most i clusters have more list
+    candidates than the count pass keeps, so the fill pass searches again
+    for them; exact pair count, CV, forces against the oracle."""
+    P = _P()
+    x, L = uniform(20000, rho=100.0, seed=31)
+    box = np.diag([L] * 3)
+    defn = "RATIONAL R_0=0.35 NN=8 MM=16 D_MAX=1.0"
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-20000 SWITCH={{{defn}}} NLISTCELLS NL_CUTOFF=1.6 NL_STRIDE=5")
+    v, d, w = act.calculate(x, box, 0)
+    assert act.stats()["list_entries"] > 400 * act.stats()["clusters_i"]
+    ov, od, ow, npair = O.coordination(O.Switch(defn), box, x, 20000, 0)
+    assert act.stats()["contributing_pairs"] == npair
+    check(v, d, w, ov, od, ow)
+    act.close()
