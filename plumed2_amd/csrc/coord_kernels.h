@@ -40,6 +40,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -148,31 +149,73 @@ __device__ __forceinline__ int nsys_of(const int* idx, int i) { return idx ? idx
 // rebuild kernels
 // ------------------------------------------------------------------------
 
-// fractional coordinates, image and column of every atom (atom i of the
-// arrays is requested atom off + i)
-__global__ void k_bin(int n, const double* __restrict__ pos, int off, const SysArg* __restrict__ sys,
-                      const int* __restrict__ atom_sys, int4* __restrict__ info, double* __restrict__ key,
-                      int* __restrict__ col_count) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const SysArg& S = sys[nsys_of(atom_sys, i)];
-  const double* x = pos + 3 * (size_t)(off + i);
-  double s[3];
-  int m[3];
-  for (int j = 0; j < 3; j++) {
-    s[j] = x[0] * S.inv[0 * 3 + j] + x[1] * S.inv[1 * 3 + j] + x[2] * S.inv[2 * 3 + j];
-    double fl = floor(s[j]);
-    m[j] = (int)fl;
-    s[j] -= fl;
+// Column counting is privatised per workgroup: the workgroup's atoms fall in
+// a narrow window of columns (consecutive atoms of one system), counted in
+// LDS and added to the global counts once per (workgroup, column).  A window
+// wider than kBinLds falls back to one global atomic per atom.
+constexpr int kBinLds = 4096;
+
+__device__ __forceinline__ void block_col_range(int col, int* sh, int& lo, int& hi) {
+  // col < 0 marks a thread without an atom
+  const int t = threadIdx.x;
+  sh[t] = col < 0 ? INT_MAX : col;
+  sh[256 + t] = col;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) {
+      sh[t] = min(sh[t], sh[t + o]);
+      sh[256 + t] = max(sh[256 + t], sh[256 + t + o]);
+    }
+    __syncthreads();
   }
-  int cx = min((int)(s[0] * S.ncx), S.ncx - 1);
-  int cy = min((int)(s[1] * S.ncy), S.ncy - 1);
-  cx = max(cx, 0);
-  cy = max(cy, 0);
-  int col = S.colbase + cx + S.ncx * cy;
-  info[i] = make_int4(m[0], m[1], m[2], col);
-  key[i] = s[2];
-  atomicAdd(&col_count[col], 1);
+  lo = sh[0];
+  hi = sh[256];
+  __syncthreads();
+}
+
+// fractional coordinates, image and column of every atom (atom i of the
+// arrays is requested atom off + i); column counts
+__global__ __launch_bounds__(256) void k_bin(int n, const double* __restrict__ pos, int off,
+                                             const SysArg* __restrict__ sys, const int* __restrict__ atom_sys,
+                                             int4* __restrict__ info, double* __restrict__ key,
+                                             int* __restrict__ col_count) {
+  __shared__ int sh[512];
+  __shared__ int hist[kBinLds];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int col = -1;
+  if (i < n) {
+    const SysArg& S = sys[nsys_of(atom_sys, i)];
+    const double* x = pos + 3 * (size_t)(off + i);
+    double sf[3];
+    int m[3];
+    for (int j = 0; j < 3; j++) {
+      sf[j] = x[0] * S.inv[0 * 3 + j] + x[1] * S.inv[1 * 3 + j] + x[2] * S.inv[2 * 3 + j];
+      double fl = floor(sf[j]);
+      m[j] = (int)fl;
+      sf[j] -= fl;
+    }
+    int cx = min((int)(sf[0] * S.ncx), S.ncx - 1);
+    int cy = min((int)(sf[1] * S.ncy), S.ncy - 1);
+    cx = max(cx, 0);
+    cy = max(cy, 0);
+    col = S.colbase + cx + S.ncx * cy;
+    info[i] = make_int4(m[0], m[1], m[2], col);
+    key[i] = sf[2];
+  }
+  int lo, hi;
+  block_col_range(col, sh, lo, hi);
+  if (hi < 0) return;  // no atom in this workgroup (uniform)
+  if (hi - lo >= kBinLds) {
+    if (col >= 0) atomicAdd(&col_count[col], 1);
+    return;
+  }
+  const int range = hi - lo + 1;
+  for (int j = threadIdx.x; j < range; j += blockDim.x) hist[j] = 0;
+  __syncthreads();
+  if (col >= 0) atomicAdd(&hist[col - lo], 1);
+  __syncthreads();
+  for (int j = threadIdx.x; j < range; j += blockDim.x)
+    if (hist[j]) atomicAdd(&col_count[lo + j], hist[j]);
 }
 
 // exclusive scans of column counts (atoms) and of ceil(count/8) (clusters),
@@ -338,13 +381,35 @@ __global__ void k_fill_double(double* p, size_t n) {
   for (; i < n; i += stride) p[i] = 0.0;
 }
 
-__global__ void k_scatter(int n, const int4* __restrict__ info, const int* __restrict__ col_start,
-                          int* __restrict__ fill, int* __restrict__ tmp) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int col = info[i].w;
-  int k = atomicAdd(&fill[col], 1);
-  tmp[col_start[col] + k] = i;
+// atoms listed by column (any order inside a column; k_rank orders them);
+// slots within a column are reserved once per (workgroup, column) as in k_bin
+__global__ __launch_bounds__(256) void k_scatter(int n, const int4* __restrict__ info,
+                                                 const int* __restrict__ col_start, int* __restrict__ fill,
+                                                 int* __restrict__ tmp) {
+  __shared__ int sh[512];
+  __shared__ int hist[kBinLds];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int col = (i < n) ? info[i].w : -1;
+  int lo, hi;
+  block_col_range(col, sh, lo, hi);
+  if (hi < 0) return;  // uniform
+  if (hi - lo >= kBinLds) {
+    if (col >= 0) {
+      const int k = atomicAdd(&fill[col], 1);
+      tmp[col_start[col] + k] = i;
+    }
+    return;
+  }
+  const int range = hi - lo + 1;
+  for (int j = threadIdx.x; j < range; j += blockDim.x) hist[j] = 0;
+  __syncthreads();
+  const int local = (col >= 0) ? atomicAdd(&hist[col - lo], 1) : 0;
+  __syncthreads();
+  // hist[j] becomes this workgroup's first slot in column lo + j
+  for (int j = threadIdx.x; j < range; j += blockDim.x)
+    if (hist[j]) hist[j] = atomicAdd(&fill[lo + j], hist[j]);
+  __syncthreads();
+  if (col >= 0) tmp[col_start[col] + hist[col - lo] + local] = i;
 }
 
 // rank of each atom inside its column by (key, index): deterministic order.
