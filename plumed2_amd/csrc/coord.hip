@@ -241,17 +241,11 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
   if ((rc = c->partial.reserve(8 * (size_t)std::max(c->nitems, 1))) || (rc = c->redo_count.reserve(2))) return rc;
   // absolute index per slot (two groups: pairs of one atom with itself are skipped)
   if (ngroups == 2) {
-    std::vector<int> loc[2];
     for (int gi = 0; gi < 2; gi++) {
       Group& G = c->g[gi];
-      std::vector<int> sl((size_t)G.ncl * CL);
       if (G.ncl > 0)
-        P2A_HIP(hipMemcpyAsync(sl.data(), G.slot_loc.p, sl.size() * sizeof(int), hipMemcpyDeviceToHost, s));
-      P2A_HIP(hipStreamSynchronize(s));
-      for (auto& v : sl) v = (v >= 0) ? (int)c->abs[G.off + v] : -1 - (int)(&v - sl.data());
-      if (G.ncl > 0)
-        P2A_HIP(hipMemcpyAsync(G.abs_slot.p, sl.data(), sl.size() * sizeof(int), hipMemcpyHostToDevice, s));
-      P2A_HIP(hipStreamSynchronize(s));
+        hipLaunchKernelGGL(k_abs_slot, dim3(grid1((size_t)G.ncl * CL, 256)), dim3(256), 0, s, G.ncl * CL,
+                           (const int*)nullptr, G.slot_loc.p, G.off, (const long long*)c->abs_dev.p, G.abs_slot.p);
     }
   }
   if ((rc = launch_check("rebuild: list fill"))) return rc;

@@ -456,6 +456,17 @@ __global__ __launch_bounds__(256) void k_rank(int ncol, const double* __restrict
   }
 }
 
+// absolute atom index of every slot of a group (two groups: an atom listed in
+// both is not paired with itself); padding slots get distinct negative tags
+__global__ void k_abs_slot(int nslot, const int* __restrict__ ncl_dev, const int* __restrict__ slot_loc, int off,
+                           const long long* __restrict__ absidx, int* __restrict__ abs_slot) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ncl_dev) nslot = *ncl_dev * CL;
+  if (s >= nslot) return;
+  const int a = slot_loc[s];
+  abs_slot[s] = (a >= 0) ? (int)absidx[off + a] : -1 - s;
+}
+
 __global__ void k_cl_col(int ncol, const int* __restrict__ col_cl, int* __restrict__ cl_col) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncol) return;
