@@ -871,14 +871,12 @@ struct PairAcc {
 // decision: EXACT=false only raises `band_hit` (the item is then redone by
 // the EXACT=true kernel), EXACT=true decides it on the spot.
 template <int P, bool GENERIC, bool TWO, bool VO, int KIND, bool EXACT>
-__device__ __forceinline__ void eval_group(const PairArgs& A, const P3* __restrict__ sjw,
-                                           const short* __restrict__ smw, const int* __restrict__ scw,
-                                           const int* __restrict__ saw, int g,
+__device__ __forceinline__ void eval_group(const PairArgs& A, const P3& pj, const short* __restrict__ smw,
+                                           const int* __restrict__ scw, const int* __restrict__ saw, int g,
                                            int ii, int jj, const float4& pi, int reqi, int ai,
                                            const float4* __restrict__ posJ, const PbcArg* __restrict__ pbc,
                                            const double* __restrict__ pos64, float band, bool& band_hit,
                                            PairAcc& acc) {
-  const P3 pj = sjw[g + jj];
   const float dx = pj.x - pi.x;
   const float dy = pj.y - pi.y;
   const float dz = pj.z - pi.z;
@@ -979,7 +977,7 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(EXAC
                                                        const double* __restrict__ pos64,
                                                        double* __restrict__ fpart, double* __restrict__ partial) {
   constexpr int NW = PAIR_BLOCK / WAVE;
-  constexpr int CAP = CH * CL + 3 * CL;
+  constexpr int CAP = CH * CL + 4 * CL;  // + padding of 3 kinds + the read-ahead slots
   __shared__ P3 sj[NW][CAP];
   __shared__ short sm[NW][CAP];
   __shared__ int scode[NW][CH];
@@ -1111,16 +1109,28 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(EXAC
     }
     PairAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0};
     bool band_hit = false;  // uniform
+    // each group's staged atom is read from LDS one group ahead (the staging
+    // area has 8 spare slots for the read past the last group)
     if (n0)
-      eval_group<P, GENERIC, TWO, VO, 0, EXACT>(A, sjw, smw, scw, saw, 0, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band,
-                                                band_hit, acc);
-    for (int g = b1; g < b1 + n1; g += CL)
-      eval_group<P, GENERIC, TWO, VO, 1, EXACT>(A, sjw, smw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band,
-                                                band_hit, acc);
+      eval_group<P, GENERIC, TWO, VO, 0, EXACT>(A, sjw[jj], smw, scw, saw, 0, ii, jj, pi, reqi, ai, posJ, pbc, pos64,
+                                                band, band_hit, acc);
+    {
+      P3 pj = sjw[b1 + jj];
+      for (int g = b1; g < b1 + n1; g += CL) {
+        const P3 pn = sjw[g + CL + jj];
+        eval_group<P, GENERIC, TWO, VO, 1, EXACT>(A, pj, smw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc, pos64,
+                                                  band, band_hit, acc);
+        pj = pn;
+      }
+    }
     if constexpr (!VO) {
-      for (int g = b2; g < b2 + n2; g += CL)
-        eval_group<P, GENERIC, TWO, VO, 2, EXACT>(A, sjw, smw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc, pos64, band,
-                                                  band_hit, acc);
+      P3 pj = sjw[b2 + jj];
+      for (int g = b2; g < b2 + n2; g += CL) {
+        const P3 pn = sjw[g + CL + jj];
+        eval_group<P, GENERIC, TWO, VO, 2, EXACT>(A, pj, smw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc, pos64,
+                                                  band, band_hit, acc);
+        pj = pn;
+      }
     }
     if (!EXACT && band_hit) {
       // the item is redone with FP64 cut decisions; its outputs come from there
