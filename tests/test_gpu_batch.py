@@ -184,3 +184,42 @@ def test_batch_lattice_pairs_exactly_at_dmax():
     for k in range(2):
         check_system(k, natoms, boxes, x, v, d, w, counts)
     b.close()
+
+
+def test_one_system_batch_equals_single_context():
+    """A batch of one system against the single-system action on the same
+    input (different list radius bookkeeping, same pairs)."""
+    import torch
+
+    import plumed2_amd as P
+    natoms, boxes, x = small_batch(seed=15, nsys=1)
+    b, _, v, d, w = run(P, torch, natoms, boxes, x)
+    n = int(natoms[0])
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{n} SWITCH={{{SWITCH}}} NLISTCELLS NL_CUTOFF=1.2 NL_STRIDE=1")
+    v1, d1, w1 = act.calculate(x, boxes[0], 0)
+    assert b.pair_counts()[0] == act.stats()["contributing_pairs"]
+    assert abs(v[0] - v1) <= 1e-6 * abs(v1)
+    assert np.max(np.abs(d - d1)) <= 1e-5 * np.max(np.abs(d1))
+    act.close()
+    b.close()
+
+
+def test_batch_with_generic_switch():
+    """EXP switching function (evaluated in FP64 per pair) through the batch."""
+    import torch
+
+    import plumed2_amd as P
+    natoms, boxes, x = small_batch(seed=16, nsys=3)
+    defn = "EXP R_0=0.2 D_0=0.1 D_MAX=1.2"
+    b = P.CoordinationBatch(natoms, defn, 1.2)
+    b.set_boxes(boxes)
+    xt = torch.tensor(x, dtype=torch.float64, device="cuda")
+    v, d, w = (t.cpu().numpy() for t in b.calculate_torch(xt))
+    counts = b.pair_counts()
+    o = np.concatenate([[0], np.cumsum(natoms)])
+    for k in range(3):
+        ov, od, ow, npair = O.coordination(O.Switch(defn), boxes[k], x[o[k]:o[k + 1]], int(natoms[k]), 0)
+        assert counts[k] == npair
+        assert abs(v[k] - ov) <= CV_TOL * abs(ov)
+        assert np.max(np.abs(d[o[k]:o[k + 1]] - od)) <= F_TOL * np.max(np.abs(od))
+    b.close()

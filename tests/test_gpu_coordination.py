@@ -351,3 +351,26 @@ def test_dense_long_list_radius():
     assert act.stats()["contributing_pairs"] == npair
     check(v, d, w, ov, od, ow)
     act.close()
+
+
+@pytest.mark.parametrize("defn", ["EXP R_0=0.2 D_0=0.1 D_MAX=1.0", "RATIONAL R_0=0.3 NN=7 MM=13 D_MAX=1.0"])
+def test_value_only_generic_switch(defn):
+    """Value-only kernel (no derivative and no virial pointers) with switches
+    outside fixedRational: the value equals the full kernel's and the oracle's."""
+    import torch
+
+    P = _P()
+    x, L = uniform(8000, seed=41)
+    box = np.diag([L] * 3)
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-8000 SWITCH={{{defn}}}")
+    v_full, _, _ = act.calculate(x, box, 0)
+    xt = torch.tensor(x, dtype=torch.float64, device="cuda")
+    val = torch.empty(1, dtype=torch.float64, device="cuda")
+    act.set_stream(torch.cuda.current_stream().cuda_stream)
+    act.calculate_device(xt.data_ptr(), box, 1, val.data_ptr(), 0, 0)
+    torch.cuda.synchronize()
+    ov, _, _, npair = O.coordination(O.Switch(defn), box, x, 8000, 0)
+    assert abs(float(val.item()) - ov) <= 1e-6 * abs(ov)
+    assert abs(v_full - ov) <= 1e-6 * abs(ov)
+    assert act.stats()["contributing_pairs"] == npair
+    act.close()
