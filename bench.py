@@ -298,8 +298,8 @@ def main():
                     help="run only a plain 256 MiB device copy (reference size for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it "
-                                   "(see DESIGN.md for how the path would shard)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
 
     import torch

@@ -10,7 +10,7 @@
 // src/colvar/CoordinationBase.cpp:30-131); the arithmetic goes through the C
 // ABI in include/plumed2_amd.h.  This file is host C++ only: it is compiled
 // against an installed PLUMED (`plumed info --include-dir`) and linked with
-// libplumed2amd.so, see INTEGRATION.md.
+// libplumed2amd.so.
 #include "core/ActionRegister.h"
 #include "core/Colvar.h"
 #include "tools/AtomNumber.h"
