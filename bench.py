@@ -298,7 +298,7 @@ def main():
                     help="run only a plain 256 MiB device copy (reference size for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "plumed2_amd runs on one GPU; runs on several GPUs are not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

@@ -37,7 +37,7 @@ extern "C" {
 #define P2A_ERR_ARG (-1)      /* invalid argument or input syntax          */
 #define P2A_ERR_HIP (-2)      /* HIP runtime error                          */
 #define P2A_ERR_NOMEM (-3)    /* device allocation failed                   */
-#define P2A_ERR_UNSUPPORTED (-4) /* valid input this build does not handle  */
+#define P2A_ERR_UNSUPPORTED (-4) /* valid input the library does not handle */
 #define P2A_ERR_GRID (-5)     /* lookup outside the METAD grid (Grid.cpp:108-118) */
 
 /* Library version string. */
