@@ -23,6 +23,7 @@
 // column-major order of GridBase::getIndex (first CV fastest).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -280,6 +281,140 @@ __global__ __launch_bounds__(kSeqThreads) void k_deposit_seq(GridArg G, double* 
   }
 }
 
+// Two non-periodic CVs: the same sequential deposition with the grid around
+// the walk held in LDS.  A tile of kTileX x kTileY nodes (value, two
+// gradient components) serves every hill whose support and spline corners
+// lie inside it; a hill outside writes the tile back and loads the one
+// centred on it.  Every node receives the same FP64 additions in the same
+// order as in k_deposit_seq, so the grids are identical.
+constexpr int kTileX = 64, kTileY = 80;
+
+__device__ __forceinline__ double spline_corner_vals(const GridArg& G, const double* x, const unsigned* ind,
+                                                     const unsigned* ni, double grid, const double* der) {
+  double ff = 1.0;
+  for (int j = 0; j < 2; j++) {
+    const int x0 = (ni[j] == ind[j]) ? 0 : 1;
+    const double dx = G.dx[j];
+    const double xfloor = G.min[j] + (double)ind[j] * dx;
+    const double X = fabs((x[j] - xfloor) / dx - (double)x0);
+    const double X2 = X * X, X3 = X2 * X;
+    const double yy = fabs(grid) < 0.0000001 ? 0.0 : -der[j] / grid;
+    const double sg = x0 ? -1.0 : 1.0;
+    ff *= (1.0 - 3.0 * X2 + 2.0 * X3) - sg * yy * (X - 2.0 * X2 + X3) * dx;
+  }
+  return grid * ff;
+}
+
+__global__ __launch_bounds__(kSeqThreads) void k_deposit_seq_tile2(GridArg G, double* __restrict__ val,
+                                                                   double* __restrict__ der, long long nhills,
+                                                                   const double* __restrict__ centers,
+                                                                   double* __restrict__ heights,
+                                                                   int* __restrict__ err) {
+  __shared__ double tv[kTileX * kTileY], tdx[kTileX * kTileY], tdy[kTileX * kTileY];
+  __shared__ double sh_h, sh_c[2];
+  __shared__ unsigned sh_ind[2];
+  __shared__ int sh_ok, sh_fits, sh_nox, sh_noy;
+  const int t = threadIdx.x;
+  const int nx = (int)G.nbin[0], ny = (int)G.nbin[1];
+  const int tw = min(kTileX, nx), th = min(kTileY, ny);
+  int ox = -1, oy = -1;  // tile origin in nodes (-1: no tile loaded)
+  auto flush = [&]() {
+    if (ox < 0) return;
+    for (int j = t; j < tw * th; j += blockDim.x) {
+      const int lx = j % tw, ly = j / tw;
+      const long long node = (long long)(oy + ly) * nx + (ox + lx);
+      const int l = lx + kTileX * ly;
+      val[node] = tv[l];
+      der[2 * node] = tdx[l];
+      der[2 * node + 1] = tdy[l];
+    }
+  };
+  for (long long k = 0; k < nhills; k++) {
+    if (t == 0) {
+      double c[2] = {centers[2 * k], centers[2 * k + 1]};
+      unsigned ind[2] = {0u, 0u};
+      const int ok = grid_index_of(G, c, ind) >= 0;
+      // nodes the hill touches: support, and the cell's upper spline corner
+      const int xl = max(0, (int)ind[0] - G.nneigh[0]), xh = min(nx - 1, (int)ind[0] + G.nneigh[0] + 1);
+      const int yl = max(0, (int)ind[1] - G.nneigh[1]), yh = min(ny - 1, (int)ind[1] + G.nneigh[1] + 1);
+      sh_fits = ox >= 0 && xl >= ox && xh < ox + tw && yl >= oy && yh < oy + th;
+      // a tile centred on the hill holds it whenever nneigh + 2 <= tile / 2,
+      // which the launch guarantees; checked anyway before any LDS index
+      const int nox = max(0, min(nx - tw, (int)ind[0] - tw / 2));
+      const int noy = max(0, min(ny - th, (int)ind[1] - th / 2));
+      const bool fits_new = xl >= nox && xh < nox + tw && yl >= noy && yh < noy + th;
+      sh_nox = nox;
+      sh_noy = noy;
+      sh_ok = ok && (sh_fits || fits_new);
+      sh_ind[0] = ind[0];
+      sh_ind[1] = ind[1];
+      sh_c[0] = c[0];
+      sh_c[1] = c[1];
+      if (!sh_ok) atomicCAS(err, 0, (int)(1 + (k < 0x3fffffff ? k : 0x3fffffff)));
+    }
+    __syncthreads();
+    if (!sh_ok) {  // uniform
+      flush();
+      return;
+    }
+    if (!sh_fits) {
+      flush();
+      // the tile written above may overlap the one read below
+      __threadfence();
+      __syncthreads();
+      ox = sh_nox;
+      oy = sh_noy;
+      for (int j = t; j < tw * th; j += blockDim.x) {
+        const int lx = j % tw, ly = j / tw;
+        const long long node = (long long)(oy + ly) * nx + (ox + lx);
+        const int l = lx + kTileX * ly;
+        tv[l] = val[node];
+        tdx[l] = der[2 * node];
+        tdy[l] = der[2 * node + 1];
+      }
+      __syncthreads();
+    }
+    const unsigned ind[2] = {sh_ind[0], sh_ind[1]};
+    const double c[2] = {sh_c[0], sh_c[1]};
+    if (t < WAVE_M) {
+      double v = 0.0;
+      if (G.welltemp && t < 4) {
+        // corner ip = t of the cell (Grid.cpp:442-505): skipped past the last node
+        const unsigned n0 = ind[0] + (t & 1), n1 = ind[1] + (t >> 1);
+        if (n0 < G.nbin[0] && n1 < G.nbin[1]) {
+          const unsigned ni[2] = {n0, n1};
+          const int l = (int)(n0 - ox) + kTileX * (int)(n1 - oy);
+          const double dv[2] = {tdx[l], tdy[l]};
+          v = spline_corner_vals(G, c, ind, ni, tv[l], dv);
+        }
+      }
+      double tot = 0.0;
+      for (int q = 0; q < 4; q++) tot += __shfl(v, q, WAVE_M);
+      if (t == 0) {
+        const double h = G.welltemp ? wt_height(G, tot) : G.height0;
+        sh_h = h;
+        if (heights) heights[k] = h;
+      }
+    }
+    __syncthreads();
+    const double h = sh_h;
+    for (int sidx = t; sidx < G.support; sidx += blockDim.x) {
+      unsigned ni[kMaxCv];
+      if (!support_node(G, ind, sidx, ni)) continue;
+      double xx[2], d[2];
+      xx[0] = G.min[0] + (double)ni[0] * G.dx[0];
+      xx[1] = G.min[1] + (double)ni[1] * G.dx[1];
+      const double b = gaussian(G, xx, c, h, d);
+      const int l = (int)(ni[0] - ox) + kTileX * (int)(ni[1] - oy);
+      tv[l] += b;
+      tdx[l] += d[0];
+      tdy[l] += d[1];
+    }
+    __syncthreads();
+  }
+  flush();
+}
+
 // hills of given heights, in parallel (one block per hill)
 __global__ __launch_bounds__(kDepositThreads) void k_add_hills(GridArg G, double* __restrict__ val,
                                                                 double* __restrict__ der, long long nhills,
@@ -521,8 +656,16 @@ int p2a_metad_deposit_device(p2a_metad* m, int64_t nhills, const double* centers
   if (!m || (nhills > 0 && !centers_dev)) return fail(P2A_ERR_ARG, "null argument");
   if (nhills <= 0) return P2A_OK;
   P2A_HIPM(hipSetDevice(m->device));
-  hipLaunchKernelGGL(k_deposit_seq, dim3(1), dim3(kSeqThreads), 0, m->stream, m->G, m->val, m->der,
-                     (long long)nhills, centers_dev, heights_dev, m->err);
+  const GridArg& G = m->G;
+  const bool tile2 = G.ncv == 2 && !G.pbc[0] && !G.pbc[1] && !G.may_alias &&
+                     2 * G.nneigh[0] + 4 <= std::min<int>(kTileX, (int)G.nbin[0]) &&
+                     2 * G.nneigh[1] + 4 <= std::min<int>(kTileY, (int)G.nbin[1]);
+  if (tile2)
+    hipLaunchKernelGGL(k_deposit_seq_tile2, dim3(1), dim3(kSeqThreads), 0, m->stream, m->G, m->val, m->der,
+                       (long long)nhills, centers_dev, heights_dev, m->err);
+  else
+    hipLaunchKernelGGL(k_deposit_seq, dim3(1), dim3(kSeqThreads), 0, m->stream, m->G, m->val, m->der,
+                       (long long)nhills, centers_dev, heights_dev, m->err);
   P2A_HIPM(hipGetLastError());
   return P2A_OK;
 }

@@ -170,3 +170,20 @@ def test_add_hills_equals_sequential_with_same_heights():
     v2, d2 = m2.grid()
     np.testing.assert_allclose(v1, v2, atol=1e-12 * np.max(np.abs(v1)))
     np.testing.assert_allclose(d1, d2, atol=1e-12 * np.max(np.abs(d1)))
+
+
+def test_deposition_at_grid_edges_matches_oracle():
+    """A walk that starts in a corner of the grid: hill supports are cut by the
+    grid edges and the LDS tile of the deposition sits against them."""
+    P = _P()
+    m = P.MetaD("m: METAD ARG=c,d SIGMA=20,0.05 HEIGHT=1.2 PACE=500 BIASFACTOR=10 TEMP=300 "
+                "GRID_MIN=0,0 GRID_MAX=5000,6 GRID_BIN=512,512", periodic={})
+    centers = random_walk(600, [5.0, 0.02], [30.0, 0.08], [0, 0], [4999.9, 5.999], 9)
+    centers = np.concatenate([centers, random_walk(300, [4995.0, 5.98], [30.0, 0.08], [0, 0], [4999.9, 5.999], 10)])
+    h = m.deposit(centers)
+    g = O.Grid([0.0, 0.0], [5000.0, 6.0], [512, 512], [0, 0])
+    oh = g.deposit_wt(centers, [20.0, 0.05], 1.2, 10.0, 300 * KB)
+    np.testing.assert_allclose(h, oh, rtol=TOL)
+    v, d = m.grid()
+    assert np.max(np.abs(v - g.values())) <= TOL * np.max(np.abs(g.values()))
+    assert np.max(np.abs(d - g.derivs())) <= TOL * np.max(np.abs(g.derivs()))
