@@ -147,7 +147,7 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
   A.redo_count = b->totals.p + 4;
   int grid = grid1(std::max(b->items_cap, 1), PAIR_BLOCK / WAVE);
   if (b->tune.blocks_per_cu > 0) grid = std::min(grid, b->tune.blocks_per_cu * b->cu_count);
-  launch_pairs(A, grid, std::min(grid, b->cu_count), s, false, chunk, b->items.p, b->ents.p, b->p.p, b->p.p, b->bb.p, b->bb.p,
+  launch_pairs(A, grid, std::min(grid, b->cu_count * kExactBlocksPerCu), s, false, chunk, b->items.p, b->ents.p, b->p.p, b->p.p, b->bb.p, b->bb.p,
                (const int*)nullptr, (const int*)nullptr, pos, b->fpart.p, b->partial.p);
   if ((rc = launch_check("batch: pair kernel"))) return rc;
   hipLaunchKernelGGL(k_gather_reduce, dim3(grid1(nslot * 3, 256)), dim3(256), 0, s, grid1(nslot * 3, 256), clc,

@@ -352,7 +352,7 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     if (c->tune.blocks_per_cu > 0) grid = std::min(grid, c->tune.blocks_per_cu * c->cu_count);
     A.redo = D.redo.p;
     A.redo_count = c->redo_count.p + d;
-    launch_pairs(A, grid, std::min(grid, c->cu_count), s, value_only, c->tune.chunk, D.items.p, D.ents.p, GI.p.p, GJ.p.p, GI.bb.p,
+    launch_pairs(A, grid, std::min(grid, c->cu_count * kExactBlocksPerCu), s, value_only, c->tune.chunk, D.items.p, D.ents.p, GI.p.p, GJ.p.p, GI.bb.p,
                  GJ.bb.p, GI.abs_slot.p, GJ.abs_slot.p, pos_dev, D.fpart.p, c->partial.p + 8 * (size_t)D.item_offset);
   }
   if (c->profiling) {

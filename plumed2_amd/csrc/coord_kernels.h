@@ -966,7 +966,9 @@ __device__ __forceinline__ void eval_group(const PairArgs& A, const P3& pj, cons
 // EXACT=false is the FP32 pass over every item; an item with a pair in the
 // band around D_MAX^2 writes nothing and is appended to A.redo.  EXACT=true
 // redoes those items with the FP64 cut decision inline (it needs many more
-// registers, which is why it is a separate kernel).
+// registers, which is why it is a separate kernel); there the four waves of
+// a workgroup share one item, each evaluating every fourth group, and their
+// partials are summed in wave order.
 template <int P, bool GENERIC, bool TWO, bool VO, int CH, bool EXACT>
 __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : 5))) void k_pairs(PairArgs A, const int4* __restrict__ items,
                                                        const Entry* __restrict__ ents,
@@ -982,6 +984,9 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(EXAC
   __shared__ short sm[NW][CAP];
   __shared__ int scode[NW][CH];
   __shared__ int sabs[TWO ? NW : 1][TWO ? CAP : 1];
+  // EXACT: per-wave partials of the shared item (i derivatives, then value,
+  // virial and pair count)
+  __shared__ double sred[EXACT ? NW : 1][EXACT ? 3 * CL + 8 : 1];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int ii = lane & 7, jj = lane >> 3;
@@ -993,9 +998,11 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(EXAC
   int* saw = TWO ? sabs[w] : nullptr;
 
   // the next item's descriptor and entries are loaded while the current item
-  // computes (a wave handles several items when the grid is persistent)
-  const int kstride = gridDim.x * NW;
-  int k = blockIdx.x * NW + w;
+  // computes (a wave handles several items when the grid is persistent).
+  // EXACT: the waves of a workgroup share one item (the few items of this
+  // pass are latency-bound chains), so k is uniform over the workgroup.
+  const int kstride = EXACT ? gridDim.x : gridDim.x * NW;
+  int k = EXACT ? blockIdx.x : blockIdx.x * NW + w;
   int item_n = 0;
   int4 it_n = make_int4(0, 0, 0, 0);
   Entry my_n = {0, 0.f, 0.f, 0.f};
@@ -1109,6 +1116,59 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(EXAC
     }
     PairAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0};
     bool band_hit = false;  // uniform
+    if constexpr (EXACT) {
+      // every wave staged the whole item; wave w evaluates groups w, w + NW, ...
+      if (n0 && w == 0)
+        eval_group<P, GENERIC, TWO, VO, 0, true>(A, sjw[jj], smw, scw, saw, 0, ii, jj, pi, reqi, ai, posJ, pbc, pos64,
+                                                 band, band_hit, acc);
+      int gi = n0 ? 1 : 0;
+      for (int g = b1; g < b1 + n1; g += CL, gi++)
+        if (gi % NW == w)
+          eval_group<P, GENERIC, TWO, VO, 1, true>(A, sjw[g + jj], smw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc,
+                                                   pos64, band, band_hit, acc);
+      if constexpr (!VO)
+        for (int g = b2; g < b2 + n2; g += CL, gi++)
+          if (gi % NW == w)
+            eval_group<P, GENERIC, TWO, VO, 2, true>(A, sjw[g + jj], smw, scw, saw, g, ii, jj, pi, reqi, ai, posJ,
+                                                     pbc, pos64, band, band_hit, acc);
+      double* red = sred[w];
+      if constexpr (!VO) {
+        double dfx = acc.fx, dfy = acc.fy, dfz = acc.fz;
+        for (int o = 8; o < 64; o <<= 1) {
+          dfx += __shfl_xor(dfx, o, 64);
+          dfy += __shfl_xor(dfy, o, 64);
+          dfz += __shfl_xor(dfz, o, 64);
+        }
+        if (jj == 0) {
+          red[ii * 3] = dfx;
+          red[ii * 3 + 1] = dfy;
+          red[ii * 3 + 2] = dfz;
+        }
+      }
+      double r[7] = {acc.cv, acc.vxx, acc.vxy, acc.vxz, acc.vyy, acc.vyz, acc.vzz};
+      for (int q2 = 0; q2 < 7; q2++) r[q2] = wave_sum(r[q2]);
+      if (lane == 0) {
+        for (int q2 = 0; q2 < 7; q2++) red[3 * CL + q2] = r[q2];
+        red[3 * CL + 7] = (double)acc.npair;
+      }
+      __syncthreads();
+      // the waves' partials summed in wave order
+      if (w == 0) {
+        if (!VO && lane < 3 * CL) {
+          double s = 0.0;
+          for (int v = 0; v < NW; v++) s += sred[v][lane];
+          fpart[(size_t)item * (CL * 3) + lane] = -s;
+        }
+        if (lane < 8) {
+          double s = 0.0;
+          for (int v = 0; v < NW; v++) s += sred[v][3 * CL + lane];
+          partial[(size_t)item * 8 + lane] = s;
+        }
+      }
+      // the next item overwrites the staging and sred
+      __syncthreads();
+      continue;
+    }
     // each group's staged atom is read from LDS one group ahead (the staging
     // area has 8 spare slots for the read past the last group)
     if (n0)
@@ -1455,6 +1515,9 @@ struct PairTuning {
   int chunk = 64;        // 32 or 64
   int blocks_per_cu = 0; // 0: one wave per item; > 0: persistent grid of that many blocks per CU
 };
+
+// workgroups per CU of the FP64-decision pass (its register use allows 3)
+constexpr int kExactBlocksPerCu = 3;
 
 // launches of k_pairs for a switch, group mode and tuning: the FP32 pass
 // over all items, then the FP64-decision pass over the items it set aside
