@@ -62,3 +62,18 @@ def read_deriv(fname):
 def read_hills(fname):
     names, arr = read_colvar(fname)
     return names, arr
+
+
+def rt_backup_grid_cvs():
+    """d1 = DISTANCE ATOMS=1,2 and d2 = DISTANCE ATOMS=3,4 (minimum image in
+    the cubic box) over the frames of regtest/basic/rt-backup-grid."""
+    out = []
+    for pos, box in read_xyz(path("rt-backup-grid", "trajectory.xyz")):
+        L = np.diag(box)
+        c = []
+        for a, b in [(0, 1), (2, 3)]:
+            d = pos[b] - pos[a]
+            d -= L * np.round(d / L)
+            c.append(float(np.sqrt(d @ d)))
+        out.append(c)
+    return out

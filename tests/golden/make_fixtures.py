@@ -25,6 +25,8 @@ for name in ["rational", "rational-MMeq2NN", "fastrational", "fastrational-MMeq2
                                     f"basic/rt20-switch-{name}/deriv.reference"]
 FILES["rt11"] = ["basic/rt11/COLVAR.reference", "basic/rt11/HILLS.reference", "basic/rt11/forces.reference"]
 FILES["rt67"] = ["basic/rt67/COLVAR.reference", "basic/rt67/traj.gro"]
+FILES["rt-backup-grid"] = ["basic/rt-backup-grid/trajectory.xyz", "basic/rt-backup-grid/grid.reference",
+                            "basic/rt-backup-grid/bck.last.grid.reference"]
 
 for d, files in FILES.items():
     os.makedirs(os.path.join(HERE, d), exist_ok=True)

@@ -1,5 +1,5 @@
 """GPU parity of the METAD grid (HIP library through the C ABI) against the FP64
-oracle and the rt11/rt67 golden files.  The grid accumulator is FP64 on both
+oracle and the rt11/rt67/rt-backup-grid golden files.  The grid accumulator is FP64 on both
 sides: values, gradients and heights are held to 1e-10 relative."""
 import math
 import os
@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from fixtures import path, read_colvar
+from fixtures import path, read_colvar, rt_backup_grid_cvs
 
 pytestmark = pytest.mark.gpu
 KB = 0.0083144621
@@ -187,3 +187,49 @@ def test_deposition_at_grid_edges_matches_oracle():
     v, d = m.grid()
     assert np.max(np.abs(v - g.values())) <= TOL * np.max(np.abs(g.values()))
     assert np.max(np.abs(d - g.derivs())) <= TOL * np.max(np.abs(g.derivs()))
+
+
+def test_rt_backup_grid_files_through_gpu(tmp_path):
+    """regtest/basic/rt-backup-grid: the GRID_WFILE dumps of the GPU grid, the
+    last one and the one it replaced, equal the reference files."""
+    P = _P()
+    g = tmp_path / "grid"
+    m = P.MetaD(f"METAD ARG=d1,d2 SIGMA=0.1,0.1 PACE=2 HEIGHT=0.5 FMT=%6.2f GRID_MIN=0,0 GRID_MAX=2,2 "
+                f"GRID_BIN=4,4 GRID_WFILE={g} GRID_WSTRIDE=2 LABEL=meta1 FILE={tmp_path / 'HILLS'}", timestep=0.05)
+    for step, cv in enumerate(rt_backup_grid_cvs()):
+        m.calculate(cv)
+        m.update(step, cv)
+    m.close()
+    assert g.read_text() == open(path("rt-backup-grid", "grid.reference")).read()
+    assert (tmp_path / "bck.last.grid").read_text() == open(path("rt-backup-grid", "bck.last.grid.reference")).read()
+
+
+def test_grid_rfile_restart(tmp_path):
+    """GRID_RFILE: a run restarted from a written grid holds the file's values
+    (9 decimals) and its bias is the spline of that grid."""
+    P = _P()
+    line = ("m: METAD ARG=c,d SIGMA=20,0.05 HEIGHT=1.2 PACE=500 BIASFACTOR=10 TEMP=300 "
+            "GRID_MIN=0,0 GRID_MAX=5000,6 GRID_BIN=128,96 ")
+    m = P.MetaD(line + f"FILE={tmp_path / 'H1'}")
+    m.deposit(random_walk(200, [2500.0, 3.0], [40.0, 0.1], [0, 0], [4999.9, 5.999], 21))
+    m.write_grid(str(tmp_path / "g"))
+    v0, d0 = m.grid()
+    m.close()
+    # the restart names the grid file, and a different GRID_BIN that the file overrides
+    r = P.MetaD(line.replace("GRID_BIN=128,96", "GRID_BIN=10,10") + f"GRID_RFILE={tmp_path / 'g'} "
+                f"FILE={tmp_path / 'H2'}")
+    assert r.gbin == [128, 96]
+    v1, d1 = r.grid()
+    assert np.max(np.abs(v1 - v0)) <= 5.1e-10 and np.max(np.abs(d1 - d0)) <= 5.1e-10
+    og = O.Grid([0.0, 0.0], [5000.0, 6.0], [128, 96], [0, 0])
+    og.set(v1, d1)
+    pts = np.column_stack([np.linspace(10, 4990, 37), np.linspace(0.01, 5.99, 37)])
+    b, db = r.bias(pts)
+    for k, x in enumerate(pts):
+        ob, od = og.spline(x)
+        assert abs(b[k] - ob) <= TOL * max(1.0, abs(ob))
+        np.testing.assert_allclose(db[k], od, rtol=TOL, atol=TOL)
+    # the restarted grid written again is the same text
+    r.write_grid(str(tmp_path / "g2"))
+    assert (tmp_path / "g2").read_text() == (tmp_path / "g").read_text()
+    r.close()

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from fixtures import path, read_colvar
+from fixtures import path, read_colvar, rt_backup_grid_cvs
 
 needs_ref = pytest.mark.skipif(O.ref() is None, reason="compiled reference subset (oracle/_ref) not built here")
 
@@ -161,3 +161,77 @@ def test_rt67_golden_periodic_bias():
             g.deposit_wt([c], [0.2, 0.2], 1.2, biasf, kbt)
     # CVs are printed with 4 decimals: the bias is reproduced to ~1e-3
     assert worst < 2e-3, worst
+
+
+# ---- grid files: regtest/basic/rt-backup-grid -----------------------------------
+# METAD ARG=d1,d2 SIGMA=0.1,0.1 PACE=2 HEIGHT=0.5 GRID_MIN=0,0 GRID_MAX=2,2
+# GRID_BIN=4,4 GRID_WFILE=grid GRID_WSTRIDE=2 LABEL=meta1, d1 = DISTANCE 1,2,
+# d2 = DISTANCE 3,4 over the 10 frames of its trajectory.  grid.reference is
+# the dump of step 8 (hills of steps 2, 4, 6, 8), bck.last.grid.reference the
+# one of step 6 it replaced.
+
+def backup_grid_oracle_texts():
+    from plumed2_amd.metad import format_grid
+    g = O.Grid([0.0, 0.0], [2.0, 2.0], [4, 4], [0, 0])
+    texts = {}
+    for s, c in enumerate(rt_backup_grid_cvs()):
+        if s % 2 == 0 and s > 0:
+            g.add_gaussian(c, [0.1, 0.1], 0.5)
+        if s % 2 == 0:
+            texts[s] = (format_grid("meta1.bias", ["d1", "d2"], ["0", "0"], ["2", "2"], [0.0, 0.0], [2.0, 2.0],
+                                    [4, 4], [False, False], g.values(), g.derivs()), g.values(), g.derivs())
+    return texts
+
+
+def test_grid_file_text_matches_rt_backup_grid():
+    texts = backup_grid_oracle_texts()
+    assert texts[8][0] == open(path("rt-backup-grid", "grid.reference")).read()
+    assert texts[6][0] == open(path("rt-backup-grid", "bck.last.grid.reference")).read()
+
+
+def test_grid_file_read_back():
+    from plumed2_amd.metad import format_grid, read_grid_file
+    _, v, d = backup_grid_oracle_texts()[8]
+    r = read_grid_file(path("rt-backup-grid", "grid.reference"), "meta1.bias", ["d1", "d2"])
+    assert r["nbin"] == [4, 4] and r["periodic"] == [False, False]
+    assert r["gmin"] == [0.0, 0.0] and r["gmax"] == [2.0, 2.0]
+    # the file holds 9 decimals
+    np.testing.assert_allclose(r["values"], v, rtol=0, atol=5.1e-10)
+    np.testing.assert_allclose(r["derivs"], d, rtol=0, atol=5.1e-10)
+    assert np.count_nonzero(r["values"]) == 3
+    text = format_grid("meta1.bias", ["d1", "d2"], r["gmin_s"], r["gmax_s"], r["gmin"], r["gmax"], r["nbin"],
+                       r["periodic"], r["values"], r["derivs"])
+    assert text == open(path("rt-backup-grid", "grid.reference")).read()
+
+
+def test_periodic_grid_file_round_trip(tmp_path):
+    from plumed2_amd.metad import format_grid, read_grid_file
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=12 * 7)
+    d = rng.normal(size=(12 * 7, 2))
+    text = format_grid("m.bias", ["phi", "x"], ["-pi", "0"], ["pi", "3"], [-math.pi, 0.0], [math.pi, 3.0], [12, 6],
+                       [True, False], v, d)
+    assert "#! SET nbins_phi  12" in text and "#! SET nbins_x  7" in text
+    assert "#! SET periodic_phi true" in text and "#! SET min_phi -pi" in text
+    assert text.count("\n\n") == 6
+    p = tmp_path / "g"
+    p.write_text(text)
+    r = read_grid_file(str(p), "m.bias", ["phi", "x"])
+    assert r["nbin"] == [12, 6] and r["periodic"] == [True, False]
+    np.testing.assert_allclose(r["values"], v, atol=5.1e-10)
+    np.testing.assert_allclose(r["derivs"], d, atol=5.1e-10)
+    with pytest.raises(Exception, match="in grid input"):
+        read_grid_file(str(p), "other.bias", ["phi", "x"])
+    with pytest.raises(Exception, match="same order"):
+        read_grid_file(str(p), "m.bias", ["x", "phi"])
+
+
+def test_grid_keywords_are_checked_on_the_host():
+    from plumed2_amd import MetaD, Plumed2AmdError
+    base = "m: METAD ARG=a,b SIGMA=0.1,0.1 PACE=2 HEIGHT=0.5 GRID_MIN=0,0 GRID_MAX=2,2 GRID_BIN=4,4 "
+    with pytest.raises(Plumed2AmdError, match="GRID_WSTRIDE"):
+        MetaD(base + "GRID_WFILE=g")
+    with pytest.raises(Plumed2AmdError, match="GRID_WFILE"):
+        MetaD(base + "GRID_WSTRIDE=2")
+    with pytest.raises(Plumed2AmdError, match="cannot be found"):
+        MetaD(base + "GRID_RFILE=/nonexistent/grid")
