@@ -106,38 +106,57 @@ def _events(torch):
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
 
-def bench_autograd(P, torch, steps=10, warm=2):
+def bench_autograd(P, torch, steps=20, warm=2, skin=0.1):
     """Differentiable COORDINATION: GROUPA=10,000, GROUPB=490,000 at 100 /nm^3,
-    RATIONAL R_0=0.35 NN=8 MM=16 D_MAX=1.0, float32 positions; one step = the
-    value-only forward plus the recomputing backward."""
+    RATIONAL R_0=0.35 NN=8 MM=16 D_MAX=1.0, float32 positions on a random walk
+    (sigma 0.005 nm per step, frames generated on the GPU before timing); one
+    step = the value-only forward plus the recomputing backward on the next
+    frame.  Timed twice: with the pair list kept while no atom has moved more
+    than skin/2, and rebuilt at every call."""
     from plumed2_amd.autograd import coordination
 
     na, nb = 10_000, 490_000
     n = na + nb
     L = (n / 100.0) ** (1.0 / 3.0)
     box = np.diag([L] * 3)
-    x = np.random.default_rng(3).uniform(0.0, L, (n, 3))
-    act = P.Coordination(f"c: COORDINATION GROUPA=1-{na} GROUPB={na + 1}-{n} "
-                         "SWITCH={RATIONAL R_0=0.35 NN=8 MM=16 D_MAX=1.0}")
-    xt = torch.tensor(x, dtype=torch.float32, device="cuda", requires_grad=True)
-    for s in range(warm):
-        coordination(xt, act, box, s).backward()
+    line = (f"c: COORDINATION GROUPA=1-{na} GROUPB={na + 1}-{n} "
+            "SWITCH={RATIONAL R_0=0.35 NN=8 MM=16 D_MAX=1.0}")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    x = torch.rand((n, 3), generator=g, device="cuda", dtype=torch.float64) * L
+    frames = []
+    for _ in range(warm + steps):
+        frames.append(x.to(torch.float32).requires_grad_(True))
+        x = x + 0.005 * torch.randn((n, 3), generator=g, device="cuda", dtype=torch.float64)
     torch.cuda.synchronize()
-    st = act.stats()
-    pairs = st["contributing_pairs"]
-    a, b = _events(torch)
-    a.record()
-    for s in range(warm, warm + steps):
-        xt.grad = None
-        coordination(xt, act, box, s).backward()
-    b.record()
-    torch.cuda.synchronize()
-    ms = a.elapsed_time(b) / steps
-    act.close()
+
+    def run(sk):
+        act = P.Coordination(line)
+        act.set_skin(sk)
+        for s in range(warm):
+            coordination(frames[s], act, box, s).backward()
+        torch.cuda.synchronize()
+        r0 = act.stats()["rebuilds"]
+        pairs = 0
+        a, b = _events(torch)
+        a.record()
+        for s in range(warm, warm + steps):
+            coordination(frames[s], act, box, s).backward()
+        b.record()
+        torch.cuda.synchronize()
+        st = act.stats()
+        act.close()
+        return a.elapsed_time(b) / steps, st["contributing_pairs"], st["rebuilds"] - r0
+
+    ms_skin, pairs, rb_skin = run(skin)
+    ms_every, _, rb_every = run(0.0)
     return {"workload": f"GROUPA={na} GROUPB={nb}, rho=100/nm^3, L={L:.4f} nm, RATIONAL R_0=0.35 NN=8 MM=16 "
-                        "D_MAX=1.0, float32 positions, torch.autograd forward+backward",
-            "ms_per_step": ms, "contributing_pairs_per_step": pairs,
-            "pair_evals_per_s": pairs / (ms / 1e3)}
+                        "D_MAX=1.0, float32 positions on a random walk (0.005 nm/step), torch.autograd "
+                        "forward+backward",
+            "ms_per_step": ms_skin, "skin_nm": skin, "rebuilds_in_timed_steps": rb_skin,
+            "ms_per_step_rebuild_every_call": ms_every, "rebuilds_every_call": rb_every,
+            "steps": steps, "contributing_pairs_last_step": pairs,
+            "pair_evals_per_s": pairs / (ms_skin / 1e3)}
 
 
 def bench_metad(P, torch, nhills=10_000, steps=2000):

@@ -193,6 +193,17 @@ int p2a_coord_set_profiling(p2a_coord *ctx, int on);
  * counterpart (the reference's OpenMP loop has no such knob). */
 int p2a_coord_set_tuning(p2a_coord *ctx, int chunk, int blocks_per_cu);
 
+/* Without a neighbour list (no NLIST/NLISTCELLS): build the pair list at
+ * D_MAX + skin and keep it while no atom has moved more than skin/2 from its
+ * position at the build; each call then checks the displacements on the
+ * device (one host synchronisation) instead of rebuilding.  The pairs
+ * evaluated are those of a list rebuilt at every call, as the reference
+ * computes every pair at every step (CoordinationBase.cpp:142-232).  skin 0
+ * (the default) rebuilds at every call; a cell too narrow for D_MAX + skin
+ * also does.  No reference counterpart: the reference has no list without
+ * NLIST. */
+int p2a_coord_set_skin(p2a_coord *ctx, double skin);
+
 /* ------------------------------------------------------------------------
  * Batched COORDINATION: many independent single-group systems (frames or
  * walkers, each with its own atom count and periodic cell) sharing one

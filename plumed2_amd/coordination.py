@@ -151,3 +151,8 @@ class Coordination:
     def set_tuning(self, chunk: int = 64, blocks_per_cu: int = 0):
         """Entries per wave (32/64) and persistent grid of the pair kernel."""
         A.check(A.lib().p2a_coord_set_tuning(self._ctx, int(chunk), int(blocks_per_cu)))
+
+    def set_skin(self, skin: float):
+        """Without NLIST/NLISTCELLS: keep the pair list built at D_MAX + skin
+        while no atom has moved more than skin/2 (0: rebuild every call)."""
+        A.check(A.lib().p2a_coord_set_skin(self._ctx, float(skin)))

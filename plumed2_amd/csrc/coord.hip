@@ -86,6 +86,12 @@ struct p2a_coord {
   bool pbc_valid = false;
   double path_box[9] = {0};  // box of the last path decision
   int path_cached = -1;
+  // without a neighbour list: list built at D_MAX + skin and kept while no
+  // atom has moved more than skin/2 (skin_active: 0 when the cell is too
+  // narrow for the larger radius)
+  double skin = 0.0, skin_active = 0.0;
+  p2a::DBuf<double> pos_ref;  // positions at the last build
+  p2a::DBuf<int> moved;
 };
 
 namespace p2a {
@@ -113,6 +119,9 @@ static int choose_path_uncached(p2a_coord* c, const double* box9) {
   pbc_widths(box9, w);
   double wmin = std::min(w[0], std::min(w[1], w[2]));
   if (wmin <= 2.0 * rl * 1.0001 + 1e-6) return 1;
+  c->skin_active = 0.0;
+  if (c->spec.nl_mode == P2A_NL_NONE && c->skin > 0.0 && wmin > 2.0 * (sw.dmax + c->skin) * 1.0001 + 1e-6)
+    c->skin_active = c->skin;
   return 0;
 }
 
@@ -120,8 +129,8 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
   const p2a_switch& sw = c->spec.sw;
   // list radius: NL_CUTOFF when a neighbour list is requested (its candidates
   // must cover every pair that can come within D_MAX before the next rebuild),
-  // D_MAX otherwise (the list is rebuilt every step)
-  c->rlist = (c->spec.nl_mode != P2A_NL_NONE ? std::max(c->spec.nl_cutoff, sw.dmax) : sw.dmax) + 1e-4;
+  // D_MAX + skin otherwise (skin 0: the list is rebuilt every step)
+  c->rlist = (c->spec.nl_mode != P2A_NL_NONE ? std::max(c->spec.nl_cutoff, sw.dmax) : sw.dmax + c->skin_active) + 1e-4;
   const int ngroups = c->spec.nb > 0 ? 2 : 1;
   hipStream_t s = c->stream;
   int rc;
@@ -249,6 +258,11 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     }
   }
   if ((rc = launch_check("rebuild: list fill"))) return rc;
+  if (c->skin_active > 0.0) {
+    const size_t n3 = 3 * (size_t)c->natoms;
+    if ((rc = c->pos_ref.reserve(n3))) return rc;
+    P2A_HIP(hipMemcpyAsync(c->pos_ref.p, pos_dev, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
   std::memcpy(c->box_at_build, box9, sizeof(c->box_at_build));
   c->built = true;
   c->rebuilds++;
@@ -427,12 +441,30 @@ static int calc_device(p2a_coord* c, int64_t step, const double* box9, const dou
   c->path = path;
   if (path != 0) return run_fallback(c, box9, pos_dev, vdev, ddev, wdev);
   // NeighborList::prepare (NeighborList.cpp:433-456): rebuild on the first
-  // step and every NL_STRIDE steps; without a neighbour list every step.
+  // step and every NL_STRIDE steps.  Without a neighbour list every step, or,
+  // with a skin, when some atom has moved more than skin/2 since the build:
+  // every pair within D_MAX now was within D_MAX + skin then, so the pairs
+  // evaluated are those of a list rebuilt now.
   bool need = !c->built || c->firsttime;
-  if (c->spec.nl_mode == P2A_NL_NONE || c->spec.nl_stride <= 1) need = true;
-  else if (step % c->spec.nl_stride == 0) need = true;
   // the images of the list belong to one cell: a new cell needs a new list
   if (!need && std::memcmp(c->box_at_build, box9, sizeof(c->box_at_build)) != 0) need = true;
+  if (c->spec.nl_mode == P2A_NL_NONE) {
+    if (!need && c->skin_active > 0.0) {
+      if ((rc = c->moved.reserve(1))) return rc;
+      const double half = 0.5 * c->skin_active;
+      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->moved.p, 1, 0);
+      hipLaunchKernelGGL(k_moved, dim3(grid1((size_t)c->natoms, 256)), dim3(256), 0, c->stream, c->natoms, pos_dev,
+                         (const double*)c->pos_ref.p, half * half, c->moved.p);
+      int m = 1;
+      P2A_HIP(hipMemcpyAsync(&m, c->moved.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      P2A_HIP(hipStreamSynchronize(c->stream));
+      need = m != 0;
+    } else {
+      need = true;
+    }
+  } else if (c->spec.nl_stride <= 1 || step % c->spec.nl_stride == 0) {
+    need = true;
+  }
   c->firsttime = false;
   if (need && (rc = rebuild(c, box9, pos_dev))) return rc;
   return run_pairs(c, box9, pos_dev, vdev, ddev, wdev, value_only);
@@ -587,6 +619,15 @@ int p2a_coord_calculate(p2a_coord* c, int64_t step, const double* box9, const do
   P2A_HIP(hipStreamSynchronize(c->stream));
   if (value) *value = out[0];
   if (virial) std::memcpy(virial, out + 1, 9 * sizeof(double));
+  return P2A_OK;
+}
+
+int p2a_coord_set_skin(p2a_coord* c, double skin) {
+  if (!c) return fail(P2A_ERR_ARG, "null argument");
+  if (!(skin >= 0.0) || !std::isfinite(skin)) return fail(P2A_ERR_ARG, "the skin must be a finite distance >= 0");
+  c->skin = skin;
+  c->path_cached = -1;  // the radius decides whether the cell is wide enough
+  c->built = false;
   return P2A_OK;
 }
 

@@ -375,6 +375,22 @@ __global__ void k_fill_int(int* p, int n, int v) {
   if (i < n) p[i] = v;
 }
 
+// flag[0] = 1 when some atom lies farther than sqrt(lim2) from its position
+// at the last list build (a NaN coordinate counts as moved); flag[0] must be
+// zero before the launch
+__global__ __launch_bounds__(256) void k_moved(int n, const double* __restrict__ pos,
+                                               const double* __restrict__ ref, double lim2, int* __restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool m = false;
+  if (i < n) {
+    const double dx = pos[3 * (size_t)i] - ref[3 * (size_t)i];
+    const double dy = pos[3 * (size_t)i + 1] - ref[3 * (size_t)i + 1];
+    const double dz = pos[3 * (size_t)i + 2] - ref[3 * (size_t)i + 2];
+    m = !(dx * dx + dy * dy + dz * dz <= lim2);
+  }
+  if (__ballot(m) != 0ull && (threadIdx.x & (WAVE - 1)) == 0) atomicOr(flag, 1);
+}
+
 __global__ void k_fill_double(double* p, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;

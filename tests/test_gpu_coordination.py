@@ -374,3 +374,65 @@ def test_value_only_generic_switch(defn):
     assert abs(v_full - ov) <= 1e-6 * abs(ov)
     assert act.stats()["contributing_pairs"] == npair
     act.close()
+
+
+@pytest.mark.parametrize("groups", ["one", "two"])
+def test_skin_keeps_the_list_and_matches_rebuilding(groups):
+    """Without a neighbour list, a skin keeps the pair list while no atom has
+    moved more than skin/2.  Along a random walk every step equals a context
+    that rebuilds at every call (same contributing pairs; CV, forces and virial
+    within the FP32 tolerances), the list is rebuilt only now and then, and
+    the last step matches the oracle."""
+    P = _P()
+    n = 12000
+    x, L = uniform(n, seed=7)
+    box = np.diag([L] * 3)
+    defn = "RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2"
+    grp = f"GROUPA=1-{n}" if groups == "one" else f"GROUPA=1-2000 GROUPB=2001-{n}"
+    line = f"c: COORDINATION {grp} SWITCH={{{defn}}}"
+    a = P.Coordination(line)
+    a.set_skin(0.1)
+    b = P.Coordination(line)
+    rng = np.random.default_rng(8)
+    steps = 30
+    for s in range(steps):
+        va, da, wa = a.calculate(x, box, step=s)
+        vb, db, wb = b.calculate(x, box, step=s)
+        check(va, da, wa, vb, db, wb, tag=s)
+        assert a.stats()["contributing_pairs"] == b.stats()["contributing_pairs"], s
+        last = x
+        x = x + rng.normal(0.0, 0.003, x.shape)
+    na, nb = (n, 0) if groups == "one" else (2000, n - 2000)
+    ov, od, ow, npair = O.coordination(O.Switch(defn), box, last, na, nb)
+    check(va, da, wa, ov, od, ow)
+    assert a.stats()["contributing_pairs"] == npair
+    # the walk moves some atom beyond skin/2 = 0.05 nm within ~15 steps
+    assert b.stats()["rebuilds"] == steps
+    assert 2 <= a.stats()["rebuilds"] <= steps // 3, a.stats()["rebuilds"]
+    a.close()
+    b.close()
+
+
+def test_skin_in_a_cell_too_narrow_for_it():
+    """A cell wide enough for D_MAX but not for D_MAX + skin rebuilds at every
+    call and stays correct; a negative skin is refused."""
+    P = _P()
+    L = 2.8
+    n = int(33.4 * L ** 3)
+    x = np.random.default_rng(4).uniform(0, L, (n, 3))
+    box = np.diag([L] * 3)
+    defn = "RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2"
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{n} SWITCH={{{defn}}}")
+    with pytest.raises(P.Plumed2AmdError):
+        act.set_skin(-0.1)
+    act.set_skin(0.5)
+    rng = np.random.default_rng(5)
+    for s in range(4):
+        v, d, w = act.calculate(x, box, step=s)
+        ov, od, ow, npair = O.coordination(O.Switch(defn), box, x, n, 0)
+        check(v, d, w, ov, od, ow, tag=s)
+        st = act.stats()
+        assert st["path"] == 0 and st["contributing_pairs"] == npair
+        x = x + rng.normal(0.0, 0.001, x.shape)
+    assert act.stats()["rebuilds"] == 4
+    act.close()
