@@ -69,7 +69,9 @@ struct DBuf {
     if (p) (void)hipFree(p);
     p = nullptr;
     n = 0;
-    size_t want = std::max<size_t>(m, 1);
+    // 1/8 headroom: list sizes drift between rebuilds, and every regrowth
+    // is a hipFree (which waits for the device) plus a hipMalloc
+    size_t want = std::max<size_t>(m + m / 8, 1);
     if (hipMalloc(&p, want * sizeof(T)) != hipSuccess) {
       p = nullptr;
       return fail(P2A_ERR_NOMEM, "hipMalloc of " + std::to_string(want * sizeof(T)) + " bytes failed");
