@@ -182,9 +182,11 @@ typedef struct p2a_coord_stats {
 
 /* Statistics of the last step.  Synchronises the context's stream. */
 int p2a_coord_get_stats(p2a_coord *ctx, p2a_coord_stats *stats);
-/* Record HIP events around every pair-kernel launch (on the launch stream) and
- * accumulate their durations into pair_kernel_ms; on!=0 also resets the sums
- * (kernel time and total_contributing_pairs). */
+/* Record HIP events around the pair-kernel launches (on the launch stream) of
+ * every on-th call (1: every call; 0: off) and accumulate their durations into
+ * pair_kernel_ms / pair_kernel_launches; on>0 also resets those sums and
+ * total_contributing_pairs, which keeps counting at every call.  Each event
+ * record costs a few microseconds of stream time, hence the sampling. */
 int p2a_coord_set_profiling(p2a_coord *ctx, int on);
 
 /* Work decomposition of the pair kernel: `chunk` list entries per wave (32 or

@@ -145,8 +145,9 @@ class Coordination:
         A.check(A.lib().p2a_coord_get_stats(self._ctx, C.byref(st)))
         return {k: getattr(st, k) for k, _ in st._fields_ if k != "pad_"}
 
-    def set_profiling(self, on: bool):
-        A.check(A.lib().p2a_coord_set_profiling(self._ctx, int(bool(on))))
+    def set_profiling(self, on: bool, every: int = 1):
+        """Time the pair kernels of every `every`-th call with HIP events."""
+        A.check(A.lib().p2a_coord_set_profiling(self._ctx, int(every) if on else 0))
 
     def set_tuning(self, chunk: int = 64, blocks_per_cu: int = 0):
         """Entries per wave (32/64) and persistent grid of the pair kernel."""

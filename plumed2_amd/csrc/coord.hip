@@ -74,6 +74,8 @@ struct p2a_coord {
   p2a::DBuf<long long> abs_dev;
   int64_t rebuilds = 0;
   bool profiling = false;
+  int prof_every = 1;       // events around every prof_every-th pair pass
+  int64_t prof_calls = 0;
   double prof_ms = 0.0;
   int64_t prof_launches = 0;
   std::vector<hipEvent_t> ev_pool;
@@ -338,7 +340,8 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
   A.cl_sys = nullptr;
   A.nitems_dev = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (c->profiling) {
+  const bool rec = c->profiling && (c->prof_calls++ % c->prof_every) == 0;
+  if (rec) {
     if (c->ev_pool.size() < 2) {
       hipEvent_t a, b;
       P2A_HIP(hipEventCreate(&a));
@@ -369,7 +372,7 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     launch_pairs(A, grid, std::min(grid, c->cu_count * kExactBlocksPerCu), s, value_only, c->tune.chunk, D.items.p, D.ents.p, GI.p.p, GJ.p.p, GI.bb.p,
                  GJ.bb.p, GI.abs_slot.p, GJ.abs_slot.p, pos_dev, D.fpart.p, c->partial.p + 8 * (size_t)D.item_offset);
   }
-  if (c->profiling) {
+  if (rec) {
     P2A_HIP(hipEventRecord(e1, s));
     c->pending.emplace_back(e0, e1);
   }
@@ -660,8 +663,11 @@ int p2a_coord_set_profiling(p2a_coord* c, int on) {
   P2A_HIP(hipSetDevice(c->device));
   P2A_HIP(hipStreamSynchronize(c->stream));
   collect_profile(c);
+  if (on < 0) return fail(P2A_ERR_ARG, "the profiling interval must be >= 0");
   c->profiling = on != 0;
   if (on) {
+    c->prof_every = on;
+    c->prof_calls = 0;
     c->prof_ms = 0.0;
     c->prof_launches = 0;
     if (c->stats.p) {

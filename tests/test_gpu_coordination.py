@@ -436,3 +436,23 @@ def test_skin_in_a_cell_too_narrow_for_it():
         x = x + rng.normal(0.0, 0.001, x.shape)
     assert act.stats()["rebuilds"] == 4
     act.close()
+
+
+def test_profiling_samples_every_kth_call():
+    """p2a_coord_set_profiling(ctx, k): pair-kernel events on calls 0, k, 2k, ...;
+    the contributing pairs keep summing at every call."""
+    P = _P()
+    x, L = uniform(4000, seed=12)
+    box = np.diag([L] * 3)
+    act = P.Coordination("c: COORDINATION GROUPA=1-4000 SWITCH={RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2}")
+    act.calculate(x, box)
+    act.set_profiling(True, every=3)
+    for s in range(7):
+        act.calculate(x, box, step=s)
+    st = act.stats()
+    assert st["pair_kernel_launches"] == 3 and st["pair_kernel_ms"] > 0.0
+    assert st["total_contributing_pairs"] == 7 * st["contributing_pairs"]
+    with pytest.raises(P.Plumed2AmdError):
+        act.set_profiling(True, every=-1)
+    act.set_profiling(False)
+    act.close()
