@@ -229,6 +229,58 @@ __device__ double spline_corner(const GridArg& G, const double* __restrict__ val
   return grid * ff;
 }
 
+// value and gradient terms of the corner ip of the cell of x: the terms
+// `value += grid * ff` and `dv[j] += grid * fd` of GridBase::getValueAndDerivatives
+// (Grid.cpp:442-505) for one ip, with the arithmetic of spline() above; a
+// corner past the last node of a non-periodic CV gives zeros
+__device__ void spline_corner_grad(const GridArg& G, const double* __restrict__ val, const double* __restrict__ der,
+                                   const double* x, const unsigned* ind, unsigned ip, double& value, double* dv) {
+  value = 0.0;
+#pragma unroll
+  for (int j = 0; j < kMaxCv; j++) dv[j] = 0.0;
+  unsigned ni[kMaxCv];
+  unsigned t = ip;
+  int cnt = 0;
+  for (int j = 0; j < G.ncv; j++) {
+    unsigned i0 = t % 2 + ind[j];
+    t /= 2;
+    if (!G.pbc[j] && i0 == G.nbin[j]) continue;
+    if (G.pbc[j] && i0 == G.nbin[j]) i0 = 0;
+    ni[cnt++] = i0;
+  }
+  if (cnt != G.ncv) return;
+  const long long node = flat(G, ni);
+  const double grid = val[node];
+  double C[kMaxCv], D[kMaxCv], ff = 1.0;
+#pragma unroll
+  for (int j = 0; j < kMaxCv; j++) {
+    C[j] = 1.0;
+    D[j] = 0.0;
+    if (j >= G.ncv) continue;
+    const int x0 = (ni[j] == ind[j]) ? 0 : 1;
+    const double dx = G.dx[j];
+    const double xfloor = G.min[j] + (double)ind[j] * dx;
+    const double X = fabs((x[j] - xfloor) / dx - (double)x0);
+    const double X2 = X * X, X3 = X2 * X;
+    const double yy = fabs(grid) < 0.0000001 ? 0.0 : -der[node * G.ncv + j] / grid;
+    const double sg = x0 ? -1.0 : 1.0;
+    C[j] = (1.0 - 3.0 * X2 + 2.0 * X3) - sg * yy * (X - 2.0 * X2 + X3) * dx;
+    D[j] = (-6.0 * X + 6.0 * X2) - sg * yy * (1.0 - 4.0 * X + 3.0 * X2) * dx;
+    D[j] *= sg / dx;
+    ff *= C[j];
+  }
+#pragma unroll
+  for (int j = 0; j < kMaxCv; j++) {
+    if (j >= G.ncv) continue;
+    double fd = D[j];
+#pragma unroll
+    for (int i = 0; i < kMaxCv; i++)
+      if (i != j && i < G.ncv) fd *= C[i];
+    dv[j] = grid * fd;
+  }
+  value = grid * ff;
+}
+
 // Sequential well-tempered deposition of many hills in one workgroup: each
 // hill's height needs the bias of the grid left by the previous hill
 // (MetaD::update -> getHeight).  Per hill, the first wave evaluates the
@@ -471,27 +523,45 @@ __global__ __launch_bounds__(kDepositThreads) void k_metad_step(GridArg G, doubl
   __shared__ int sh_ok;
   __shared__ unsigned sh_ind[kMaxCv];
   __shared__ double sh_c[kMaxCv];
-  if (threadIdx.x == 0) {
-    double c[kMaxCv], v = 0.0, dv[kMaxCv];
-    unsigned ind[kMaxCv];
+  if (threadIdx.x < WAVE_M) {
+    // lane t < 2^ncv evaluates corner t; the corners are summed in corner
+    // order, as spline() adds them
+    const int t = threadIdx.x;
+    double c[kMaxCv] = {0.0, 0.0, 0.0, 0.0};
+    unsigned ind[kMaxCv] = {0u, 0u, 0u, 0u};
     for (int i = 0; i < G.ncv; i++) c[i] = cv[i];
-    int ok = spline(G, val, der, c, v, dv);
-    if (!ok) atomicCAS(err, 0, 1);
-    // MetaD.cpp:2348-2350: no bias (and no force) when BIASFACTOR=1
-    const bool nobias = (G.biasf == 1.0);
-    if (bias) *bias = nobias ? 0.0 : v;
-    if (force)
-      for (int i = 0; i < G.ncv; i++) force[i] = nobias ? 0.0 : -dv[i];
-    ok = ok && grid_index_of(G, c, ind) >= 0;
-    // getHeight evaluates the bias of the grid before this hill: the same v
-    const double h = wt_height(G, v);
-    for (int i = 0; i < G.ncv; i++) {
-      sh_ind[i] = ind[i];
-      sh_c[i] = c[i];
+    int ok = grid_index_of(G, c, ind) >= 0;
+    const unsigned nn = 1u << G.ncv;
+    double cvv = 0.0, cdv[kMaxCv];
+    if (ok && (unsigned)t < nn) {
+      spline_corner_grad(G, val, der, c, ind, (unsigned)t, cvv, cdv);
+    } else {
+#pragma unroll
+      for (int j = 0; j < kMaxCv; j++) cdv[j] = 0.0;
     }
-    sh_h = h;
-    sh_ok = ok && deposit;
-    if (height) *height = (ok && deposit) ? h : 0.0;
+    double v = 0.0, dv[kMaxCv] = {0.0, 0.0, 0.0, 0.0};
+    for (unsigned q = 0; q < nn; q++) {
+      v += __shfl(cvv, (int)q, WAVE_M);
+#pragma unroll
+      for (int j = 0; j < kMaxCv; j++) dv[j] += __shfl(cdv[j], (int)q, WAVE_M);
+    }
+    if (t == 0) {
+      if (!ok) atomicCAS(err, 0, 1);
+      // MetaD.cpp:2348-2350: no bias (and no force) when BIASFACTOR=1
+      const bool nobias = (G.biasf == 1.0);
+      if (bias) *bias = nobias ? 0.0 : v;
+      if (force)
+        for (int i = 0; i < G.ncv; i++) force[i] = nobias ? 0.0 : -dv[i];
+      // getHeight evaluates the bias of the grid before this hill: the same v
+      const double h = wt_height(G, v);
+      for (int i = 0; i < G.ncv; i++) {
+        sh_ind[i] = ind[i];
+        sh_c[i] = c[i];
+      }
+      sh_h = h;
+      sh_ok = ok && deposit;
+      if (height) *height = (ok && deposit) ? h : 0.0;
+    }
   }
   __syncthreads();
   if (!sh_ok) return;
