@@ -248,6 +248,34 @@ def coordination(sw: Switch, box, pos, nA, nB=0, *, absidx=None, use_pbc=True, d
     return val.value, deriv.reshape(nat, 3), vir.reshape(3, 3), int(nc)
 
 
+def coordination_numbers(sw: Switch, box, pos, species=None, species_a=None, species_b=None, *, use_pbc=True):
+    """COORDINATIONNUMBER restated in FP64: the vector cn_i = sum_j s(r_ij).
+
+    The reference builds it as a shortcut (symfunc/CoordinationNumbers.cpp:326-372):
+    CONTACT_MATRIX rows (adjmat/ContactMatrix.cpp) times a vector of ONES.
+    SPECIES=list gives one row per listed atom over the other listed atoms (the
+    diagonal i == j is not a contact); SPECIESA/SPECIESB give one row per A atom
+    over the B atoms.  Each row equals COORDINATION of GROUPA={i} against that
+    environment, so it is evaluated through `coordination` (same switching
+    function, same minimum-image distance).  Indices are 0-based rows of `pos`.
+    Returns cn[len(rows)]."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    if species is not None:
+        rows = [int(i) for i in species]
+        envs = [[int(j) for j in species if int(j) != i] for i in rows]
+    else:
+        rows = [int(i) for i in species_a]
+        envs = [[int(j) for j in species_b if int(j) != i] for i in rows]
+    pbc = Pbc(box) if not isinstance(box, Pbc) else box
+    cn = np.zeros(len(rows))
+    for k, (i, env) in enumerate(zip(rows, envs)):
+        if not env:
+            continue
+        sub = pos[[i] + env]
+        cn[k] = coordination(sw, pbc, sub, 1, len(env), use_pbc=use_pbc)[0]
+    return cn
+
+
 # --------------------------------------------------------------------------
 # METAD grid
 # --------------------------------------------------------------------------

@@ -28,6 +28,10 @@ FILES["rt67"] = ["basic/rt67/COLVAR.reference", "basic/rt67/traj.gro"]
 FILES["rt-backup-grid"] = ["basic/rt-backup-grid/trajectory.xyz", "basic/rt-backup-grid/grid.reference",
                             "basic/rt-backup-grid/bck.last.grid.reference"]
 
+# per-atom coordination numbers (COORDINATIONNUMBER): SPECIESA/SPECIESB and SPECIES + MORE_THAN
+FILES["symfunc-rt-single-groupb"] = ["trajectories/trajectory.xyz", "symfunc/rt-single-groupb/colvar.reference"]
+FILES["symfunc-rt20"] = ["symfunc/rt20/colv2.reference"]
+
 for d, files in FILES.items():
     os.makedirs(os.path.join(HERE, d), exist_ok=True)
     for f in files:
