@@ -327,6 +327,18 @@ int p2a_metad_set_grid(p2a_metad *ctx, const double *values, const double *deriv
 int p2a_apply_force_device(void *stream, int64_t natoms, const double *f_dev, const double *deriv_dev,
                            const double *virial_cv_dev, double *forces_dev, double *virial_dev);
 
+/* COORDINATIONNUMBER (src/symfunc/CoordinationNumbers.cpp:326-372, a shortcut
+ * for CONTACT_MATRIX (src/adjmat/ContactMatrix.cpp) times ONES): per-atom
+ * coordination numbers cn[k] = sum over env atoms j != rows[k] of
+ * s(|x_j - x_rows[k]|), FP64.  rows/env are 0-based indices into pos_dev
+ * (natoms x 3 doubles); SPECIES=L is rows = env = L, SPECIESA/SPECIESB is
+ * rows = A, env = B.  Device arrays, no derivatives.  A row whose index, or
+ * any of whose env indices, lies outside [0, natoms) comes out NaN.
+ * Asynchronous on `stream` (NULL: default stream). */
+int p2a_coordination_numbers_device(void *stream, const p2a_switch *sw, const double *box9, int use_pbc,
+                                    int64_t natoms, int64_t nrows, const int *rows_dev, int64_t nenv,
+                                    const int *env_dev, const double *pos_dev, double *cn_dev);
+
 #if defined(__GNUC__)
 #pragma GCC visibility pop
 #endif

@@ -75,6 +75,9 @@ _SIGS = {
                                     C.c_void_p], C.c_int),
     "p2a_coord_backward_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "p2a_coord_get_stats": ([C.c_void_p, C.POINTER(p2a_coord_stats)], C.c_int),
+    "p2a_coordination_numbers_device": ([C.c_void_p, C.POINTER(p2a_switch), C.c_void_p, C.c_int, C.c_int64,
+                                         C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p],
+                                        C.c_int),
     "p2a_coord_set_profiling": ([C.c_void_p, C.c_int], C.c_int),
     "p2a_coord_set_tuning": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "p2a_coord_set_skin": ([C.c_void_p, C.c_double], C.c_int),

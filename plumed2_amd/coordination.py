@@ -157,3 +157,44 @@ class Coordination:
         """Without NLIST/NLISTCELLS: keep the pair list built at D_MAX + skin
         while no atom has moved more than skin/2 (0: rebuild every call)."""
         A.check(A.lib().p2a_coord_set_skin(self._ctx, float(skin)))
+
+
+def coordination_numbers_torch(switch, pos, box=None, species=None, species_a=None, species_b=None):
+    """COORDINATIONNUMBER (src/symfunc/CoordinationNumbers.cpp:326-372) values on
+    the GPU: cn[k] = sum over environment atoms j != row k of s(r_kj).
+
+    ``switch`` is a SWITCH definition string or a p2a_switch; ``pos`` a contiguous
+    float64 CUDA tensor [natoms,3]; ``box`` None for NOPBC, else 3x3 rows =
+    lattice vectors.  SPECIES=list gives rows = environment = list; SPECIESA and
+    SPECIESB give rows = A, environment = B.  Indices are 0-based rows of ``pos``
+    (PLUMED's atom numbers minus one).  Returns a float64 CUDA tensor [nrows],
+    enqueued on torch's current stream."""
+    import torch
+
+    if not (pos.is_cuda and pos.dtype == torch.float64 and pos.is_contiguous() and pos.dim() == 2
+            and pos.shape[1] == 3):
+        raise ValueError("positions must be a contiguous float64 CUDA tensor [natoms,3]")
+    if species is not None:
+        if species_a is not None or species_b is not None:
+            raise ValueError("give SPECIES or SPECIESA/SPECIESB, not both")
+        rows = env = list(species)
+    elif species_a is not None and species_b is not None:
+        rows, env = list(species_a), list(species_b)
+    else:
+        raise ValueError("missing atoms input use SPECIES or SPECIESA/SPECIESB")
+    n = int(pos.shape[0])
+    for idx in (rows, env):
+        if any(int(i) < 0 or int(i) >= n for i in idx):
+            raise ValueError("atom index outside the positions array")
+    sw = parse_switch(switch) if isinstance(switch, str) else switch
+    dev = pos.device
+    rows_t = torch.tensor(rows, dtype=torch.int32, device=dev)
+    env_t = torch.tensor(env, dtype=torch.int32, device=dev)
+    cn = torch.empty(len(rows), dtype=torch.float64, device=dev)
+    b9 = None if box is None else _box9(box)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    A.check(A.lib().p2a_coordination_numbers_device(
+        stream, C.byref(sw), None if b9 is None else b9.ctypes.data, 0 if box is None else 1, n, len(rows),
+        rows_t.data_ptr() if rows else None, len(env), env_t.data_ptr() if env else None, pos.data_ptr(),
+        cn.data_ptr()))
+    return cn

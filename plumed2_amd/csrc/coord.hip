@@ -486,6 +486,38 @@ static void collect_profile(p2a_coord* c) {
   c->pending.clear();
 }
 
+// COORDINATIONNUMBER rows: cn[k] = sum over env atoms j != rows[k] of
+// s(|x_j - x_rows[k]|), one thread per row, FP64, environment in list order.
+// An index outside [0, natoms) makes that row NaN instead of being read.
+__global__ __launch_bounds__(256) void k_cn_rows(int natoms, int nrows, const int* __restrict__ rows, int nenv,
+                                                 const int* __restrict__ env, int use_pbc, PbcArg P, p2a_switch sw,
+                                                 const double* __restrict__ pos, double* __restrict__ cn) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nrows) return;
+  const int i = rows[k];
+  if (i < 0 || i >= natoms) {
+    cn[k] = __builtin_nan("");
+    return;
+  }
+  const double* xi = pos + 3 * (size_t)i;
+  double sum = 0.0;
+  for (int q = 0; q < nenv; q++) {
+    const int j = env[q];
+    if (j < 0 || j >= natoms) {
+      sum = __builtin_nan("");
+      break;
+    }
+    if (j == i) continue;
+    const double* xj = pos + 3 * (size_t)j;
+    double d[3];
+    if (use_pbc) pbc_distance(P, xi, xj, d);
+    else { d[0] = xj[0] - xi[0]; d[1] = xj[1] - xi[1]; d[2] = xj[2] - xi[2]; }
+    double df;
+    sum += switch_calc_sqr(sw, d[0] * d[0] + d[1] * d[1] + d[2] * d[2], df);
+  }
+  cn[k] = sum;
+}
+
 }  // namespace p2a
 
 using namespace p2a;
@@ -676,6 +708,28 @@ int p2a_coord_set_profiling(p2a_coord* c, int on) {
     }
   }
   return P2A_OK;
+}
+
+int p2a_coordination_numbers_device(void* stream, const p2a_switch* sw, const double* box9, int use_pbc,
+                                    int64_t natoms, int64_t nrows, const int* rows_dev, int64_t nenv,
+                                    const int* env_dev, const double* pos_dev, double* cn_dev) {
+  if (!sw) return fail(P2A_ERR_ARG, "null argument");
+  if (natoms < 0 || nrows < 0 || nenv < 0 || natoms > INT32_MAX || nrows > INT32_MAX || nenv > INT32_MAX)
+    return fail(P2A_ERR_ARG, "sizes must lie in [0, 2^31)");
+  // no rows: nothing to write (an empty output may have a null address)
+  if (nrows == 0) return P2A_OK;
+  if (!pos_dev || !cn_dev || !rows_dev || (nenv > 0 && !env_dev)) return fail(P2A_ERR_ARG, "null argument");
+  if (use_pbc && !box9) return fail(P2A_ERR_ARG, "box is required with periodic boundaries");
+  PbcArg P;
+  std::memset(&P, 0, sizeof(P));
+  if (use_pbc) {
+    PbcHost h;
+    pbc_setup(box9, h);
+    fill_pbc_arg(h, P);
+  }
+  hipLaunchKernelGGL(k_cn_rows, dim3(grid1((size_t)nrows, 256)), dim3(256), 0, (hipStream_t)stream, (int)natoms,
+                     (int)nrows, rows_dev, (int)nenv, env_dev, use_pbc ? 1 : 0, P, *sw, pos_dev, cn_dev);
+  return launch_check("coordination numbers");
 }
 
 }  // extern "C"

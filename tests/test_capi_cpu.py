@@ -132,3 +132,19 @@ def test_batch_validates_arguments_and_needs_a_device():
         return
     with pytest.raises(P.Plumed2AmdError, match="no HIP device"):
         P.CoordinationBatch([3000, 4000], "RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2", 1.2)
+
+
+def test_coordination_numbers_argument_checks_without_device():
+    """The COORDINATIONNUMBER entry point settles empty and invalid calls on the
+    host, before any HIP call: no rows is a no-op even with null buffers."""
+    import ctypes as C
+
+    import plumed2_amd as P
+    from plumed2_amd import _capi as A
+
+    sw = P.parse_switch("RATIONAL R_0=0.3")
+    f = A.lib().p2a_coordination_numbers_device
+    assert f(None, C.byref(sw), None, 0, 10, 0, None, 5, None, None, None) == 0
+    assert f(None, None, None, 0, 10, 1, None, 5, None, None, None) != 0
+    assert f(None, C.byref(sw), None, 0, -1, 1, None, 5, None, None, None) != 0
+    assert f(None, C.byref(sw), None, 0, 10, 3, None, 5, None, None, None) != 0
