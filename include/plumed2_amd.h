@@ -167,6 +167,22 @@ int p2a_coord_calculate_device(p2a_coord *ctx, int64_t step, const double *box9,
 int p2a_coord_backward_device(p2a_coord *ctx, const double *box9, const double *pos_dev, double *deriv_dev,
                               double *virial_dev);
 
+/* Number of neighbour-structure builds enqueued so far on this context (host
+ * counter, no synchronisation).  A caller that keeps the positions of a
+ * forward call for a later backward records it after the forward; when it
+ * has changed by the time of the backward, another call has replaced the
+ * structure, and p2a_coord_rebuild_device restores one for those positions.
+ * No reference counterpart (the reference evaluates value and derivatives in
+ * one pass, CoordinationBase.cpp:142-232). */
+int p2a_coord_list_generation(p2a_coord *ctx, int64_t *generation);
+
+/* Build the neighbour structure now, at these device positions and box, as
+ * NeighborList::update does on a rebuild step (NeighborList.cpp:168-308).
+ * Asynchronous on the context's stream; no-op on the FP64 paths (NOPBC,
+ * PAIR, narrow cells), which keep no structure.  With NLIST/NLISTCELLS the
+ * list built here serves the following steps until the next NL_STRIDE step. */
+int p2a_coord_rebuild_device(p2a_coord *ctx, const double *box9, const double *pos_dev);
+
 typedef struct p2a_coord_stats {
   int64_t natoms;
   int64_t contributing_pairs; /* pairs with r^2 <= D_MAX^2 at the last step   */

@@ -60,6 +60,9 @@ def lib():
         L.ocoord_calculate.argtypes = [C.c_void_p, C.c_void_p, C.c_int, _dp, _i64p, C.c_int, C.c_int, C.c_int,
                                        C.c_void_p, C.c_int64, C.POINTER(C.c_double), _dp, _dp]
         L.ocoord_calculate.restype = C.c_int64
+        L.ocn_rows.argtypes = [C.c_void_p, C.c_void_p, C.c_int, _dp, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                               C.c_void_p, C.c_void_p, _dp, _dp, _dp]
+        L.ocn_rows.restype = None
         L.ogrid_sizeof.restype = C.c_size_t
         L.ogrid_init.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _u32p, _i32p]
         L.ogrid_free.argtypes = [C.c_void_p]
@@ -248,32 +251,37 @@ def coordination(sw: Switch, box, pos, nA, nB=0, *, absidx=None, use_pbc=True, d
     return val.value, deriv.reshape(nat, 3), vir.reshape(3, 3), int(nc)
 
 
-def coordination_numbers(sw: Switch, box, pos, species=None, species_a=None, species_b=None, *, use_pbc=True):
-    """COORDINATIONNUMBER restated in FP64: the vector cn_i = sum_j s(r_ij).
+def coordination_numbers(sw: Switch, box, pos, species=None, species_a=None, species_b=None, *, use_pbc=True,
+                         weights=None):
+    """COORDINATIONNUMBER restated in FP64: the vector cn_k = sum_j w_kj.
 
     The reference builds it as a shortcut (symfunc/CoordinationNumbers.cpp:326-372):
-    CONTACT_MATRIX rows (adjmat/ContactMatrix.cpp) times a vector of ONES.
-    SPECIES=list gives one row per listed atom over the other listed atoms (the
-    diagonal i == j is not a contact); SPECIESA/SPECIESB give one row per A atom
-    over the B atoms.  Each row equals COORDINATION of GROUPA={i} against that
-    environment, so it is evaluated through `coordination` (same switching
-    function, same minimum-image distance).  Indices are 0-based rows of `pos`.
-    Returns cn[len(rows)]."""
+    CONTACT_MATRIX rows times a vector of ONES.  SPECIES=list gives one row per
+    listed atom over the listed atoms; SPECIESA/SPECIESB give one row per A atom
+    over the B atoms.  The matrix element is ContactMatrix::calculateWeight
+    (adjmat/ContactMatrix.h:95-121): nothing for |r|^2 < DBL_EPSILON (the
+    diagonal, or two atoms at one place), s(r) zeroed below DBL_EPSILON.
+    Indices are 0-based rows of `pos`.  Returns cn[len(rows)]; with `weights`
+    (one per row) also the vector-Jacobian product sum_k weights[k] d cn_k/dx
+    [natoms,3] and the matching box derivative [3,3]."""
     pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
     if species is not None:
-        rows = [int(i) for i in species]
-        envs = [[int(j) for j in species if int(j) != i] for i in rows]
+        rows = env = [int(i) for i in species]
     else:
-        rows = [int(i) for i in species_a]
-        envs = [[int(j) for j in species_b if int(j) != i] for i in rows]
+        rows, env = [int(i) for i in species_a], [int(j) for j in species_b]
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    env = np.ascontiguousarray(env, dtype=np.int32)
     pbc = Pbc(box) if not isinstance(box, Pbc) else box
+    n = pos.shape[0]
     cn = np.zeros(len(rows))
-    for k, (i, env) in enumerate(zip(rows, envs)):
-        if not env:
-            continue
-        sub = pos[[i] + env]
-        cn[k] = coordination(sw, pbc, sub, 1, len(env), use_pbc=use_pbc)[0]
-    return cn
+    der = np.zeros(3 * n)
+    vir = np.zeros(9)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    lib().ocn_rows(sw.ptr, pbc.ptr, int(use_pbc), pos.reshape(-1), n, len(rows), rows.ctypes.data, len(env),
+                   env.ctypes.data, None if w is None else w.ctypes.data, cn, der, vir)
+    if weights is None:
+        return cn
+    return cn, der.reshape(n, 3), vir.reshape(3, 3)
 
 
 # --------------------------------------------------------------------------

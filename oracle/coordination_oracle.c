@@ -690,6 +690,58 @@ OR_EXPORT int64_t ocoord_calculate(const osw_t *s, const opbc_t *p, int pbc, con
 }
 
 /* ------------------------------------------------------------------------ */
+/* COORDINATIONNUMBER: src/symfunc/CoordinationNumbers.cpp:326-372, a        */
+/* CONTACT_MATRIX (src/adjmat/ContactMatrix.h:95-121) times a vector of ONES  */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * cn[k] = sum over env atoms j of the matrix element of (rows[k], j):
+ * ContactMatrix::calculateWeight on pos = Pbc::distance(x_i, x_j)
+ * (AdjacencyMatrixBase): skipped when |pos|^2 < DBL_EPSILON (an atom is not
+ * bonded to itself, nor to another atom at the same place), s = calculateSqr,
+ * zeroed when s < DBL_EPSILON.  With w != NULL the vector-Jacobian product
+ * deriv[3*natoms] = sum_k w[k] d cn[k] / d x and the box derivative
+ * virial[9] = -sum_k w[k] sum_j dfunc pos (x) pos (the Colvar layout of
+ * CoordinationBase.cpp:200-206) are accumulated too.  Indices outside
+ * [0, natoms) make the row NaN.
+ */
+OR_EXPORT void ocn_rows(const osw_t *s, const opbc_t *p, int pbc, const double *pos, int natoms, int nrows,
+                        const int32_t *rows, int nenv, const int32_t *env, const double *w, double *cn,
+                        double *deriv, double *virial) {
+  if (w) {
+    memset(deriv, 0, sizeof(double) * 3 * (size_t)natoms);
+    memset(virial, 0, sizeof(double) * 9);
+  }
+  for (int k = 0; k < nrows; k++) {
+    const int i = rows[k];
+    if (i < 0 || i >= natoms) { cn[k] = NAN; continue; }
+    double sum = 0.0;
+    for (int q = 0; q < nenv; q++) {
+      const int j = env[q];
+      if (j < 0 || j >= natoms) { sum = NAN; break; }
+      double d[3];
+      const double *a = pos + 3 * (size_t)i, *b = pos + 3 * (size_t)j;
+      if (pbc) opbc_distance(p, a, b, d); else { d[0] = b[0] - a[0]; d[1] = b[1] - a[1]; d[2] = b[2] - a[2]; }
+      const double mod2 = o_mod2(d);
+      if (mod2 < DBL_EPSILON) continue;
+      double dfunc = 0.0;
+      const double v = osw_calculate_sqr(s, mod2, &dfunc);
+      if (v < DBL_EPSILON) continue;
+      sum += v;
+      if (w) {
+        const double f = w[k] * dfunc;
+        for (int c = 0; c < 3; c++) {
+          deriv[3 * (size_t)i + c] -= f * d[c];
+          deriv[3 * (size_t)j + c] += f * d[c];
+        }
+        for (int x = 0; x < 3; x++) for (int y = 0; y < 3; y++) virial[3 * x + y] -= f * d[x] * d[y];
+      }
+    }
+    cn[k] = sum;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* METAD grid: src/tools/Grid.cpp, src/bias/MetaD.cpp                        */
 /* ------------------------------------------------------------------------ */
 

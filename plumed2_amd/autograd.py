@@ -3,7 +3,10 @@
 The forward runs the value-only pair kernel; the backward is the library's
 backward entry point, which recomputes the pair terms with derivatives on the
 neighbour structure the forward built, so no per-pair data is kept between
-the two.  Positions may be float32 or float64 CUDA tensors; the gradient comes
+the two.  The forward records which build of the structure it used; when
+another forward on the same action has replaced it before this backward (two
+forwards, then the backward of both), the backward first rebuilds the
+structure at its own positions.  Positions may be float32 or float64 CUDA tensors; the gradient comes
 back in the input dtype.  Both passes run on torch's current stream.
 """
 from __future__ import annotations
@@ -27,6 +30,7 @@ class CoordinationFunction(torch.autograd.Function):
         val = torch.empty(1, dtype=torch.float64, device=positions.device)
         action.set_stream(torch.cuda.current_stream(positions.device).cuda_stream)
         action.calculate_device(pos64.data_ptr(), box, step, val.data_ptr(), 0, 0)
+        ctx.generation = action.list_generation()
         ctx.action = action
         ctx.box = _box9(box)
         ctx.in_dtype = positions.dtype
@@ -39,6 +43,8 @@ class CoordinationFunction(torch.autograd.Function):
         action = ctx.action
         der = torch.empty_like(pos64)
         action.set_stream(torch.cuda.current_stream(pos64.device).cuda_stream)
+        if action.list_generation() != ctx.generation:
+            A.check(A.lib().p2a_coord_rebuild_device(action._ctx, ctx.box.ctypes.data, C.c_void_p(pos64.data_ptr())))
         A.check(A.lib().p2a_coord_backward_device(action._ctx, ctx.box.ctypes.data, C.c_void_p(pos64.data_ptr()),
                                                   C.c_void_p(der.data_ptr()), None))
         g = grad.to(torch.float64) * der

@@ -140,6 +140,17 @@ class Coordination:
         self.calculate_device(pos.data_ptr(), box, step, val.data_ptr(), der.data_ptr(), vir.data_ptr())
         return val, der, vir
 
+    def list_generation(self) -> int:
+        """Builds of the neighbour structure enqueued so far (no synchronisation)."""
+        g = C.c_int64()
+        A.check(A.lib().p2a_coord_list_generation(self._ctx, C.byref(g)))
+        return g.value
+
+    def rebuild_device(self, pos_ptr: int, box):
+        """Build the neighbour structure now at these device positions."""
+        b = _box9(box)
+        A.check(A.lib().p2a_coord_rebuild_device(self._ctx, b.ctypes.data, C.c_void_p(pos_ptr)))
+
     def stats(self):
         st = A.p2a_coord_stats()
         A.check(A.lib().p2a_coord_get_stats(self._ctx, C.byref(st)))

@@ -486,9 +486,12 @@ static void collect_profile(p2a_coord* c) {
   c->pending.clear();
 }
 
-// COORDINATIONNUMBER rows: cn[k] = sum over env atoms j != rows[k] of
-// s(|x_j - x_rows[k]|), one thread per row, FP64, environment in list order.
-// An index outside [0, natoms) makes that row NaN instead of being read.
+// COORDINATIONNUMBER rows: cn[k] = sum over env atoms j of the contact
+// matrix element of (rows[k], j), ContactMatrix::calculateWeight
+// (src/adjmat/ContactMatrix.h:95-121): nothing when |r|^2 < DBL_EPSILON (the
+// diagonal, or two atoms at one place), s(r) zeroed below DBL_EPSILON.  One
+// thread per row, FP64, environment in list order.  An index outside
+// [0, natoms) makes that row NaN instead of being read.
 __global__ __launch_bounds__(256) void k_cn_rows(int natoms, int nrows, const int* __restrict__ rows, int nenv,
                                                  const int* __restrict__ env, int use_pbc, PbcArg P, p2a_switch sw,
                                                  const double* __restrict__ pos, double* __restrict__ cn) {
@@ -507,13 +510,16 @@ __global__ __launch_bounds__(256) void k_cn_rows(int natoms, int nrows, const in
       sum = __builtin_nan("");
       break;
     }
-    if (j == i) continue;
     const double* xj = pos + 3 * (size_t)j;
     double d[3];
     if (use_pbc) pbc_distance(P, xi, xj, d);
     else { d[0] = xj[0] - xi[0]; d[1] = xj[1] - xi[1]; d[2] = xj[2] - xi[2]; }
+    const double mod2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (mod2 < DBL_EPSILON) continue;
     double df;
-    sum += switch_calc_sqr(sw, d[0] * d[0] + d[1] * d[1] + d[2] * d[2], df);
+    const double v = switch_calc_sqr(sw, mod2, df);
+    if (v < DBL_EPSILON) continue;
+    sum += v;
   }
   cn[k] = sum;
 }
@@ -635,6 +641,29 @@ int p2a_coord_backward_device(p2a_coord* c, const double* box9, const double* po
   if (!c->built || std::memcmp(c->box_at_build, box9, sizeof(c->box_at_build)) != 0)
     return fail(P2A_ERR_ARG, "backward needs the neighbour structure of a forward call on the same box");
   return run_pairs(c, box9, pos_dev, c->out_buf.p, deriv_dev, wdev, false);
+}
+
+int p2a_coord_list_generation(p2a_coord* c, int64_t* gen) {
+  if (!c || !gen) return fail(P2A_ERR_ARG, "null argument");
+  // the host counts the rebuilds it enqueues: no synchronisation
+  *gen = c->rebuilds;
+  return P2A_OK;
+}
+
+int p2a_coord_rebuild_device(p2a_coord* c, const double* box9, const double* pos_dev) {
+  if (!c || !box9 || !pos_dev) return fail(P2A_ERR_ARG, "null argument");
+  P2A_HIP(hipSetDevice(c->device));
+  int rc;
+  if (!c->stats.p) {
+    if ((rc = c->stats.reserve(8))) return rc;
+    hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(64), 0, c->stream, c->stats.p, (size_t)8);
+  }
+  const int path = choose_path(c, box9);
+  if (path != c->path) c->built = false;
+  c->path = path;
+  if (path != 0) return P2A_OK;  // the FP64 kernels keep no pair structure
+  c->firsttime = false;
+  return rebuild(c, box9, pos_dev);
 }
 
 int p2a_coord_calculate(p2a_coord* c, int64_t step, const double* box9, const double* pos, double* value,

@@ -35,7 +35,7 @@ class MetaDHIP : public bias::Bias {
   p2a_metad* ctx_ = nullptr;
   unsigned ncv_ = 0;
   std::vector<double> sigma_;
-  double biasf_ = 1.0;
+  double biasf_ = -1.0;  // BIASFACTOR absent: plain METAD (MetaD.cpp:735)
   bool welltemp_ = false;
   int pace_ = 1;
   bool first_step_ = true;
@@ -83,7 +83,9 @@ MetaDHIP::MetaDHIP(const ActionOptions& ao) : PLUMED_BIAS_INIT(ao) {
   if (pace_ <= 0) error("frequency for hill addition is nonsensical");
   parse("BIASFACTOR", biasf_);
   if (biasf_ < 1.0 && biasf_ != -1.0) error("well tempered bias factor is nonsensical");
-  welltemp_ = biasf_ > 1.0;
+  // well-tempered from BIASFACTOR=1 up (MetaD.cpp:860-865); BIASFACTOR=1 then
+  // applies no bias and no force, as in calculate() below
+  welltemp_ = biasf_ >= 1.0;
   const double kbt = getkBT();
   if (welltemp_ && kbt <= 0.0) error("unless the MD engine passes the temperature to plumed, with well-tempered metad you must specify it using TEMP");
   std::vector<std::string> gmin(ncv_), gmax(ncv_);
@@ -222,7 +224,7 @@ void MetaDHIP::update() {
   for (unsigned i = 0; i < ncv_; i++) hills_.printField(getPntrToArgument(i), cv[i]);
   hills_.printField("multivariate", "false");
   for (unsigned i = 0; i < ncv_; i++) hills_.printField("sigma_" + getPntrToArgument(i)->getName(), sigma_[i]);
-  hills_.printField("height", welltemp_ ? h * biasf_ / (biasf_ - 1.0) : h).printField("biasf", biasf_);
+  hills_.printField("height", (welltemp_ && biasf_ > 1.0) ? h * biasf_ / (biasf_ - 1.0) : h).printField("biasf", biasf_);
   hills_.printField();
   hills_.flush();
   if (wgridstride_ > 0 && getStep() % wgridstride_ == 0) dumpGrid();

@@ -105,17 +105,55 @@ def test_autograd_full_size_float32():
 
 
 def test_module_training_step():
+    """One SGD step on the positions: the change of the CV equals the line
+    integral of the gradient along the step (trapezoid rule, exact to third
+    order in the step), which ties the backward to the forward values."""
     import torch
 
-    import plumed2_amd as P
     from plumed2_amd.autograd import CoordinationModule
     x, L = system(400, 3000, 100.0, 5)
     mod = CoordinationModule(f"c: COORDINATION GROUPA=1-400 GROUPB=401-3400 SWITCH={{{SWITCH}}}", np.diag([L] * 3))
-    xt = torch.tensor(x, dtype=torch.float32, device="cuda", requires_grad=True)
-    opt = torch.optim.SGD([xt], lr=1e-4)
-    v0 = mod(xt)
+    x0 = torch.tensor(x, dtype=torch.float64, device="cuda", requires_grad=True)
+    v0 = mod(x0)
     v0.backward()
-    opt.step()
-    v1 = mod(xt.detach().requires_grad_(True))
-    # a gradient step lowers nothing in particular, but the value moves by ~lr*|g|^2
-    assert abs(float(v1) - float(v0)) > 0.0
+    g0 = x0.grad.detach().clone()
+    lr = 1e-5
+    x1 = (x0.detach() - lr * g0).requires_grad_(True)
+    v1 = mod(x1)
+    v1.backward()
+    g1 = x1.grad.detach()
+    dx = (x1.detach() - x0.detach())
+    dv = float(v1.detach()) - float(v0.detach())
+    trap = float((0.5 * (g0 + g1) * dx).sum())
+    assert dv < 0.0
+    assert abs(dv - trap) <= 1e-3 * abs(dv), (dv, trap)
+
+
+@pytest.mark.parametrize("skin,sigma", [(0.0, 0.05), (0.1, 0.05), (0.1, 0.005)])
+def test_two_forwards_before_one_backward(skin, sigma):
+    """loss = m(x1) + m(x2), then one backward.  When the second forward
+    replaced the pair structure, the backward of the first rebuilds it at x1;
+    each gradient is checked against the oracle derivatives at its own
+    positions (sigma 0.05 nm moves pairs across D_MAX between x1 and x2)."""
+    import torch
+
+    import plumed2_amd as P
+    from plumed2_amd.autograd import coordination
+    na, nb = 600, 5000
+    x1, L = system(na, nb, 100.0, 7)
+    x2 = x1 + np.random.default_rng(8).normal(0.0, sigma, x1.shape)
+    box = np.diag([L] * 3)
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{na} GROUPB={na + 1}-{na + nb} SWITCH={{{SWITCH}}}")
+    act.set_skin(skin)
+    t1 = torch.tensor(x1, dtype=torch.float64, device="cuda", requires_grad=True)
+    t2 = torch.tensor(x2, dtype=torch.float64, device="cuda", requires_grad=True)
+    g_before = act.list_generation()
+    loss = coordination(t1, act, box, 0) + coordination(t2, act, box, 1)
+    builds = act.list_generation() - g_before
+    loss.backward()
+    assert builds == (1 if (skin > 0 and sigma < 0.01) else 2), builds
+    sw = O.Switch(SWITCH)
+    for t, xx in ((t1, x1), (t2, x2)):
+        _, od, _, _ = O.coordination(sw, box, xx, na, nb)
+        g = t.grad.cpu().numpy()
+        assert np.max(np.abs(g - od)) <= 1e-5 * np.max(np.abs(od))

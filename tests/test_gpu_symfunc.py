@@ -55,6 +55,20 @@ def test_coordination_numbers_match_oracle(defn, box):
     assert np.allclose(got, want, rtol=1e-12, atol=1e-13)
 
 
+def test_coincident_atoms_and_tiny_values():
+    """Two distinct atoms at one place are not in contact, and contacts whose
+    switch value lies below DBL_EPSILON count nothing (ContactMatrix.h:95-121)."""
+    box = np.diag([30.0, 30.0, 30.0])
+    pos = np.array([[1.0, 1.0, 1.0], [1.0, 1.0, 1.0], [1.2, 1.0, 1.0], [1.0 + 0.2 * 46.0, 1.0, 1.0]])
+    for defn in ("RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=0.9", "EXP R_0=0.2"):
+        got = _run(defn, pos, box, species=range(4))
+        want = O.coordination_numbers(O.Switch(defn), box, pos, species=range(4))
+        assert np.allclose(got, want, rtol=1e-12, atol=0.0), (defn, got, want)
+    s02 = O.Switch("RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=0.9").dist(0.2)[0][0]
+    got = _run("RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=0.9", pos, box, species=range(4))
+    np.testing.assert_allclose(got, [s02, s02, 2 * s02, 0.0], rtol=1e-12)
+
+
 def test_coordination_numbers_input_errors():
     import torch
 
