@@ -183,6 +183,31 @@ int p2a_coord_list_generation(p2a_coord *ctx, int64_t *generation);
  * list built here serves the following steps until the next NL_STRIDE step. */
 int p2a_coord_rebuild_device(p2a_coord *ctx, const double *box9, const double *pos_dev);
 
+/* Allocation-free COORDINATION: value, derivatives and virial as
+ * p2a_coord_calculate_device returns them, without a context, computed by the
+ * FP64 all-pairs kernels of that entry point (every pair of GROUPA x GROUPB,
+ * or every pair i<j of GROUPA when nb = 0, evaluated at every call).  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  It calls no hipMalloc/hipFree: the per-workgroup
+ * partials live in a static device array of 64 KiB per device (8 doubles per
+ * 256 atoms), so na + nb is at most 262144 (P2A_ERR_ARG beyond).  Every call
+ * on one device shares that array: calls that overlap on one device (other
+ * streams, other threads) race and must be ordered by the caller.
+ *   sw          resolved switch (p2a_switch_parse / p2a_switch_rational)
+ *   box9        host double[9], rows = lattice vectors (ignored if !use_pbc)
+ *   na, nb      GROUPA and GROUPB sizes; positions are GROUPA then GROUPB
+ *   absidx_dev  device int64[na+nb] absolute atom indices (pairs of one atom
+ *               with itself are skipped, CoordinationBase.cpp:183) or NULL
+ *               when all requested atoms are distinct
+ *   pos_dev     device double[na+nb][3]; value_dev [1], deriv_dev
+ *               [na+nb][3], virial_dev [9]: device outputs
+ * Runs on the current device, asynchronously on `stream` (NULL: default
+ * stream), which must belong to the current device. */
+int p2a_coord_calculate_noalloc(void *stream, const p2a_switch *sw, const double *box9, int use_pbc, int64_t na,
+                                int64_t nb, const int64_t *absidx_dev, const double *pos_dev, double *value_dev,
+                                double *deriv_dev, double *virial_dev);
+
 typedef struct p2a_coord_stats {
   int64_t natoms;
   int64_t contributing_pairs; /* pairs with r^2 <= D_MAX^2 at the last step   */

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -524,6 +525,34 @@ __global__ __launch_bounds__(256) void k_cn_rows(int natoms, int nrows, const in
   cn[k] = sum;
 }
 
+// Scratch of the allocation-free entry point: the per-workgroup partials of
+// k_allpairs that k_finalize sums (8 doubles per 256-atom workgroup).  Each
+// device holds its own copy for the life of the process; its address is
+// looked up once per device and kept in g_scratch_dev.
+constexpr int kNoAllocMaxAtoms = 1 << 18;
+constexpr int kNoAllocBlocks = kNoAllocMaxAtoms / 256;
+static __device__ __align__(256) double scratch[8 * kNoAllocBlocks];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[kMaxDevices] = {};
+
+// device address of `scratch` on the current device (hipGetSymbolAddress,
+// once per device); the host-side name is not a device address
+static int noalloc_scratch(int device, double** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(P2A_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(P2A_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[device] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_dev[device];
+  return P2A_OK;
+}
+
 }  // namespace p2a
 
 using namespace p2a;
@@ -641,6 +670,45 @@ int p2a_coord_backward_device(p2a_coord* c, const double* box9, const double* po
   if (!c->built || std::memcmp(c->box_at_build, box9, sizeof(c->box_at_build)) != 0)
     return fail(P2A_ERR_ARG, "backward needs the neighbour structure of a forward call on the same box");
   return run_pairs(c, box9, pos_dev, c->out_buf.p, deriv_dev, wdev, false);
+}
+
+int p2a_coord_calculate_noalloc(void* stream, const p2a_switch* sw, const double* box9, int use_pbc, int64_t na,
+                                int64_t nb, const int64_t* absidx_dev, const double* pos_dev, double* value_dev,
+                                double* deriv_dev, double* virial_dev) {
+  if (!sw || !pos_dev || !value_dev || !deriv_dev || !virial_dev) return fail(P2A_ERR_ARG, "null argument");
+  if (na <= 0) return fail(P2A_ERR_ARG, "GROUPA should contain at least one atom");
+  if (nb < 0) return fail(P2A_ERR_ARG, "GROUPB cannot have a negative size");
+  if (na + nb > kNoAllocMaxAtoms)
+    return fail(P2A_ERR_ARG, "p2a_coord_calculate_noalloc handles at most " + std::to_string(kNoAllocMaxAtoms) +
+                                 " atoms (its scratch holds 8 doubles per 256 atoms)");
+  if (use_pbc && !box9) return fail(P2A_ERR_ARG, "box is required with periodic boundaries");
+  int dev = 0;
+  P2A_HIP(hipGetDevice(&dev));
+  if (stream) {
+    int sdev = -1;
+    P2A_HIP(hipStreamGetDevice((hipStream_t)stream, &sdev));
+    if (sdev != dev) return fail(P2A_ERR_ARG, "the stream belongs to another device than the current one");
+  }
+  double* part = nullptr;
+  int rc;
+  if ((rc = noalloc_scratch(dev, &part))) return rc;
+  PbcArg P;
+  std::memset(&P, 0, sizeof(P));
+  if (use_pbc) {
+    PbcHost h;
+    pbc_setup(box9, h);
+    if (h.type == 0) return fail(P2A_ERR_ARG, "the box is singular");
+    fill_pbc_arg(h, P);
+  }
+  const int natoms = (int)(na + nb);
+  const int nblocks = grid1(natoms, 256);
+  hipStream_t s = (hipStream_t)stream;
+  // every workgroup writes its 8 partials, so nothing in scratch needs clearing
+  hipLaunchKernelGGL(k_allpairs, dim3(nblocks), dim3(256), 0, s, (int)na, (int)nb, use_pbc ? 1 : 0, P, *sw, pos_dev,
+                     (const long long*)absidx_dev, deriv_dev, part);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(128), 0, s, nblocks, (const double*)part, value_dev, virial_dev,
+                     (double*)nullptr);
+  return launch_check("p2a_coord_calculate_noalloc");
 }
 
 int p2a_coord_list_generation(p2a_coord* c, int64_t* gen) {

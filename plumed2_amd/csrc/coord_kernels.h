@@ -1323,8 +1323,10 @@ __global__ void k_finalize(int nitems, const double* __restrict__ partial, doubl
       double v6[6] = {s[1][0], s[2][0], s[3][0], s[4][0], s[5][0], s[6][0]};
       write_virial(virial, v6);
     }
-    stats_out[0] = s[7][0];
-    stats_out[1] += s[7][0];  // running total since the last reset
+    if (stats_out) {
+      stats_out[0] = s[7][0];
+      stats_out[1] += s[7][0];  // running total since the last reset
+    }
   }
 }
 
@@ -1382,7 +1384,8 @@ __global__ void k_allpairs(int nA, int nB, int use_pbc, PbcArg P, p2a_switch sw,
     }
     const double* xi = pos + 3 * (size_t)i;
     for (int j = j0; j < j1; j++) {
-      if (j == i || absidx[i] == absidx[j]) continue;
+      // absidx null: the requested atoms are distinct atoms
+      if (j == i || (absidx && absidx[i] == absidx[j])) continue;
       const double* xj = pos + 3 * (size_t)j;
       // pair (p,q) is oriented p -> q with p in GROUPA (two groups) or p<q
       bool own = two ? (i < nA) : (i < j);
