@@ -250,6 +250,63 @@ def bench_coord_100k(P, torch, steps=200, warm=20):
             "ms_per_step": ms, "pair_evals_per_s": st["total_contributing_pairs"] / (ms * steps / 1e3)}
 
 
+def bench_metad_chain(P, torch, steps=200, warm=20):
+    """The METAD workload as one coupled step on one stream: COORDINATION of
+    100,000 atoms (as the 50k case, NLISTCELLS NL_CUTOFF=1.5 NL_STRIDE=10),
+    DISTANCE between the centres of mass of two 500-atom groups, METAD on both
+    (512x512 bins, SIGMA=20,0.05, HEIGHT=1.2, PACE=500, BIASFACTOR=10,
+    TEMP=300), then the METAD forces on every atom and on the cell.  The grid
+    spans [288000, 293000] x [0, 6]: the coordination of this system is
+    ~2.9e5, outside GRID_MAX=5000.  Frames follow a random walk (sigma 0.005
+    nm) generated on the GPU before timing."""
+    n = 100_000
+    L = (n / DENSITY) ** (1.0 / 3.0)
+    box = np.diag([L] * 3)
+    x0 = np.random.default_rng(5).uniform(0.0, L, (n, 3))
+
+    def nearest(point, k, exclude=()):
+        d = x0 - point
+        d -= L * np.round(d / L)
+        r = (d * d).sum(1)
+        r[list(exclude)] = np.inf
+        return np.argsort(r, kind="stable")[:k]
+
+    g1 = nearest(np.array([3.0, 3.0, 3.0]), 500)
+    g2 = nearest(np.array([5.5, 4.5, 4.0]), 500, exclude=g1)
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    frames = torch.from_numpy(x0).to(dev) + torch.cumsum(
+        torch.randn((warm + steps, n, 3), dtype=torch.float64, device=dev, generator=g) * SIGMA, dim=0)
+    coord = P.Coordination(f"c: COORDINATION GROUPA=1-{n} SWITCH={{{SWITCH}}} NLISTCELLS NL_CUTOFF={NL_CUTOFF} "
+                           f"NL_STRIDE={NL_STRIDE}")
+    cd = P.CenterDistance(n, g1, g2)
+    m = P.MetaD("m: METAD ARG=c,d SIGMA=20,0.05 HEIGHT=1.2 PACE=500 BIASFACTOR=10 TEMP=300 "
+                "GRID_MIN=288000,0 GRID_MAX=293000,6 GRID_BIN=512,512")
+    m.hills_path = None
+    chain = P.MetadStep(coord, m, cd)
+    for s in range(warm):
+        chain.step(s, frames[s], box)
+    torch.cuda.synchronize()
+    coord.set_profiling(True, every=PROFILE_EVERY)
+    a, b = _events(torch)
+    a.record()
+    for s in range(warm, warm + steps):
+        chain.step(s, frames[s], box)
+    b.record()
+    torch.cuda.synchronize()
+    m.check()
+    st = coord.stats()
+    ms = a.elapsed_time(b) / steps
+    coord.close()
+    cd.close()
+    m.close()
+    return {"workload": f"METAD on COORDINATION ({n} atoms, rho={DENSITY}, L={L:.4f} nm, NLISTCELLS 1.5/10) and "
+                        "DISTANCE between two 500-atom centres of mass; 512x512 grid, PACE=500, forces on atoms",
+            "ms_per_step": ms, "steps": steps,
+            "pair_evals_per_s": st["total_contributing_pairs"] / (ms * steps / 1e3)}
+
+
 def bench_batch(P, torch, steps=10):
     """256 independent systems, 2,000-64,000 atoms each (log-uniform, seed 4),
     triclinic cells at 33.4 atoms/nm^3, SWITCH={RATIONAL R_0=0.3 NN=6 MM=12
@@ -295,7 +352,8 @@ def bench_batch(P, torch, steps=10):
 
 def run_extras(P, torch, which: str):
     runs = {"batch": ("batch_256_systems", bench_batch), "autograd": ("autograd_two_group_500k", bench_autograd),
-            "coord100k": ("coordination_100k", bench_coord_100k), "metad": ("metad_grid", bench_metad)}
+            "coord100k": ("coordination_100k", bench_coord_100k), "metad": ("metad_grid", bench_metad),
+            "chain": ("metad_coupled_step_100k", bench_metad_chain)}
     out = {}
     for key in [w.strip() for w in which.split(",") if w.strip()]:
         name, fn = runs[key]
@@ -311,7 +369,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--no-extras", action="store_true", help="skip the autograd / METAD / 100k measurements")
-    ap.add_argument("--extras", default="batch,autograd,coord100k,metad",
+    ap.add_argument("--extras", default="batch,autograd,coord100k,chain,metad",
                     help="comma list of extra measurements to run")
     ap.add_argument("--only-extras", action="store_true", help="run the extras only (no 50k main line)")
     ap.add_argument("--copy-calibration", action="store_true",

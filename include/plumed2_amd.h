@@ -368,6 +368,38 @@ int p2a_metad_set_grid(p2a_metad *ctx, const double *values, const double *deriv
 int p2a_apply_force_device(void *stream, int64_t natoms, const double *f_dev, const double *deriv_dev,
                            const double *virial_cv_dev, double *forces_dev, double *virial_dev);
 
+/* ------------------------------------------------------------------------
+ * DISTANCE between the centres of mass of two groups
+ *   c1: COM ATOMS=<group1>   c2: COM ATOMS=<group2>   d: DISTANCE ATOMS=c1,c2
+ * the second CV of a METAD on COORDINATION and a distance between centres.
+ * ------------------------------------------------------------------------ */
+typedef struct p2a_cdist p2a_cdist;
+
+/* Construction of the three actions: Center::Center (src/vatom/Center.cpp:
+ * 236-317, COM: mass weights) and Distance::Distance (src/colvar/Distance.h:
+ * 75-117).  group1/group2: host arrays of 0-based indices into the positions
+ * array of natoms atoms (distinct within each group); masses: host [n1+n2]
+ * (group 1 then group 2) or NULL for equal masses; use_pbc = !NOPBC (both the
+ * groups and the two centres are made whole, ActionAtomistic::makeWhole). */
+int p2a_cdist_create(p2a_cdist **ctx, int device, int64_t natoms, int64_t n1, const int64_t *group1, int64_t n2,
+                     const int64_t *group2, const double *masses, int use_pbc);
+int p2a_cdist_destroy(p2a_cdist *ctx);
+/* Value, derivatives and box derivative: Center::calculate (Center.cpp:
+ * 319-445) of both groups, then Distance::calculate (Distance.h:141-250).
+ * deriv_dev [n1+n2][3]: derivative with respect to each listed atom (group 1
+ * then group 2); virial_dev [9]: the box derivative of DISTANCE (the centres
+ * are linear in the atoms and add none).  Device arrays, host box; one
+ * workgroup, asynchronous on `stream` (NULL: default stream). */
+int p2a_cdist_calculate_device(p2a_cdist *ctx, void *stream, const double *box9, const double *pos_dev,
+                               double *value_dev, double *deriv_dev, double *virial_dev);
+/* Chain rule of a force f on the distance onto the listed atoms
+ * (ActionWithVirtualAtom::apply, src/core/ActionWithVirtualAtom.cpp:68-124,
+ * through ActionWithValue::checkForForces): forces[3*i+q] += f * deriv, and
+ * virial[9] += f * virial_cv.  f_dev is a device scalar (e.g. the METAD
+ * force on this CV); forces_dev is the [natoms][3] array of all atoms. */
+int p2a_cdist_apply_force_device(p2a_cdist *ctx, void *stream, const double *f_dev, const double *deriv_dev,
+                                 const double *virial_cv_dev, double *forces_dev, double *virial_dev);
+
 /* COORDINATIONNUMBER (src/symfunc/CoordinationNumbers.cpp:326-372, a shortcut
  * for CONTACT_MATRIX (src/adjmat/ContactMatrix.cpp) times ONES): per-atom
  * coordination numbers cn[k] = sum over env atoms j != rows[k] of

@@ -63,6 +63,9 @@ def lib():
         L.ocn_rows.argtypes = [C.c_void_p, C.c_void_p, C.c_int, _dp, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                C.c_void_p, C.c_void_p, _dp, _dp, _dp]
         L.ocn_rows.restype = None
+        L.ocenter_distance.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                       C.c_void_p, _dp, _dp, _dp, _dp]
+        L.ocenter_distance.restype = C.c_double
         L.ogrid_sizeof.restype = C.c_size_t
         L.ogrid_init.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _u32p, _i32p]
         L.ogrid_free.argtypes = [C.c_void_p]
@@ -282,6 +285,24 @@ def coordination_numbers(sw: Switch, box, pos, species=None, species_a=None, spe
     if weights is None:
         return cn
     return cn, der.reshape(n, 3), vir.reshape(3, 3)
+
+
+def center_distance(box, pos, group1, group2, masses=None, *, use_pbc=True):
+    """DISTANCE between the centres of mass of two groups (c1: COM ATOMS=...,
+    c2: COM ATOMS=..., d: DISTANCE ATOMS=c1,c2), restated in FP64 with the
+    reference's order of operations.  Returns (value, deriv[n1+n2,3] for the
+    listed atoms, group 1 then group 2, virial[3,3], c1, c2)."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    g1 = np.ascontiguousarray(group1, dtype=np.int32)
+    g2 = np.ascontiguousarray(group2, dtype=np.int32)
+    pbc = Pbc(box) if not isinstance(box, Pbc) else box
+    m = None if masses is None else np.ascontiguousarray(masses, dtype=np.float64)
+    der = np.zeros(3 * (len(g1) + len(g2)))
+    vir = np.zeros(9)
+    c1, c2 = np.zeros(3), np.zeros(3)
+    v = lib().ocenter_distance(pbc.ptr, int(use_pbc), pos.reshape(-1), len(g1), g1.ctypes.data, len(g2),
+                               g2.ctypes.data, None if m is None else m.ctypes.data, der, vir, c1, c2)
+    return v, der.reshape(-1, 3), vir.reshape(3, 3), c1, c2
 
 
 # --------------------------------------------------------------------------

@@ -742,6 +742,71 @@ OR_EXPORT void ocn_rows(const osw_t *s, const opbc_t *p, int pbc, const double *
 }
 
 /* ------------------------------------------------------------------------ */
+/* DISTANCE between two centres: COM (src/vatom/Center.cpp:236-445) of each  */
+/* group, then DISTANCE (src/colvar/Distance.h:141-250) between them          */
+/* ------------------------------------------------------------------------ */
+
+/* one centre: ActionAtomistic::makeWhole (src/core/ActionAtomistic.cpp:
+ * 587-607, no MOLINFO) chains each atom to the previous one through
+ * Pbc::distance, then pos += w_i x_i in atom order with w_i = m_i / M (COM,
+ * Center.cpp:333-367) */
+static void o_center(const opbc_t *p, int pbc, const double *pos, int n, const int32_t *idx, const double *m,
+                     double *w, double c[3]) {
+  double *x = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+  for (int k = 0; k < n; k++) for (int q = 0; q < 3; q++) x[3 * k + q] = pos[3 * (size_t)idx[k] + q];
+  if (pbc)
+    for (int j = 0; j + 1 < n; j++) {
+      double d[3];
+      opbc_distance(p, x + 3 * j, x + 3 * (j + 1), d);
+      for (int q = 0; q < 3; q++) x[3 * (j + 1) + q] = x[3 * j + q] + d[q];
+    }
+  double mass = 0.0;
+  for (int k = 0; k < n; k++) mass += m ? m[k] : 1.0;
+  for (int k = 0; k < n; k++) w[k] = (m ? m[k] : 1.0) / mass;
+  c[0] = c[1] = c[2] = 0.0;
+  for (int k = 0; k < n; k++) for (int q = 0; q < 3; q++) c[q] += w[k] * x[3 * k + q];
+  free(x);
+}
+
+/*
+ * c1: COM ATOMS=idx1, c2: COM ATOMS=idx2, d: DISTANCE ATOMS=c1,c2.
+ * masses: [n1+n2] (group 1 then group 2) or NULL for equal masses.  With pbc
+ * both centres come from whole groups and the distance is made whole as
+ * well (Distance.h:150-152).  Outputs: the value; deriv[(n1+n2)*3], the
+ * derivative with respect to each listed atom (the chain rule through the
+ * centres' derivatives w_i * identity); virial[9], the box derivative
+ * -sum_v c_v (x) dd/dc_v of DISTANCE (Colvar::setBoxDerivativesNoPbc; the
+ * centres, linear in the atoms, add none, ActionWithVirtualAtom.h:54-63).
+ * An atom listed in both groups collects both terms in its two entries.
+ */
+OR_EXPORT double ocenter_distance(const opbc_t *p, int pbc, const double *pos, int n1, const int32_t *idx1, int n2,
+                                  const int32_t *idx2, const double *masses, double *deriv, double *virial,
+                                  double *c1out, double *c2out) {
+  double *w = (double *)malloc(sizeof(double) * (size_t)(n1 + n2 > 0 ? n1 + n2 : 1));
+  double c1[3], c2[3];
+  o_center(p, pbc, pos, n1, idx1, masses, w, c1);
+  o_center(p, pbc, pos, n2, idx2, masses ? masses + n1 : NULL, w + n1, c2);
+  if (pbc) {
+    double d[3];
+    opbc_distance(p, c1, c2, d);
+    for (int q = 0; q < 3; q++) c2[q] = c1[q] + d[q];
+  }
+  double d[3] = {c2[0] - c1[0], c2[1] - c1[1], c2[2] - c1[2]};
+  const double value = sqrt(o_mod2(d));
+  const double inv = 1.0 / value;
+  double g0[3], g1[3];
+  for (int q = 0; q < 3; q++) { g0[q] = -inv * d[q]; g1[q] = inv * d[q]; }
+  for (int k = 0; k < n1; k++) for (int q = 0; q < 3; q++) deriv[3 * k + q] = w[k] * g0[q];
+  for (int k = 0; k < n2; k++) for (int q = 0; q < 3; q++) deriv[3 * (n1 + k) + q] = w[n1 + k] * g1[q];
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) virial[3 * a + b] = -(c1[a] * g0[b]) - c2[a] * g1[b];
+  if (c1out) memcpy(c1out, c1, sizeof c1);
+  if (c2out) memcpy(c2out, c2, sizeof c2);
+  free(w);
+  return value;
+}
+
+/* ------------------------------------------------------------------------ */
 /* METAD grid: src/tools/Grid.cpp, src/bias/MetaD.cpp                        */
 /* ------------------------------------------------------------------------ */
 

@@ -9,5 +9,6 @@ from .coordination import (Coordination, CoordinationSpec, coordination_numbers_
                            describe_switch, parse_switch, rational_switch)
 from .batch import CoordinationBatch  # noqa: F401
 from .metad import MetaD  # noqa: F401
+from .chain import CenterDistance, MetadStep  # noqa: F401
 
 __version__ = "0.1.0"

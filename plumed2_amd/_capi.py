@@ -85,6 +85,13 @@ _SIGS = {
     "p2a_coord_set_profiling": ([C.c_void_p, C.c_int], C.c_int),
     "p2a_coord_set_tuning": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "p2a_coord_set_skin": ([C.c_void_p, C.c_double], C.c_int),
+    "p2a_cdist_create": ([C.POINTER(C.c_void_p), C.c_int, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
+                          C.c_void_p, C.c_int], C.c_int),
+    "p2a_cdist_destroy": ([C.c_void_p], C.c_int),
+    "p2a_cdist_calculate_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p], C.c_int),
+    "p2a_cdist_apply_force_device": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p], C.c_int),
     "p2a_batch_create": ([C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_void_p, C.POINTER(p2a_switch), C.c_double],
                          C.c_int),
     "p2a_batch_destroy": ([C.c_void_p], C.c_int),

@@ -28,6 +28,10 @@ FILES["rt67"] = ["basic/rt67/COLVAR.reference", "basic/rt67/traj.gro"]
 FILES["rt-backup-grid"] = ["basic/rt-backup-grid/trajectory.xyz", "basic/rt-backup-grid/grid.reference",
                             "basic/rt-backup-grid/bck.last.grid.reference"]
 
+# DISTANCE between a centre of mass (made whole) and an atom, forces of a RESTRAINT on it
+FILES["rt18"] = ["basic/rt18/COLVAR.reference", "basic/rt18/forces.reference", "basic/rt18/mc",
+                 "trajectories/trajectory.xyz"]
+
 # per-atom coordination numbers (COORDINATIONNUMBER): SPECIESA/SPECIESB and SPECIES + MORE_THAN
 FILES["symfunc-rt-single-groupb"] = ["trajectories/trajectory.xyz", "symfunc/rt-single-groupb/colvar.reference"]
 FILES["symfunc-rt20"] = ["symfunc/rt20/colv2.reference"]
