@@ -54,7 +54,8 @@ struct p2a_batch {
   p2a::DBuf<double4> origin;
   p2a::DBuf<float4> bb0, bb, p;
   p2a::DBuf<float2> kr;
-  p2a::DBuf<int> ecount, estart, istart, totals, redo, scan_bsum;
+  p2a::DBuf<int> ecount, estart, istart, totals, redo, recs, scan_bsum;
+  p2a::DBuf<int2> fix;
   p2a::DBuf<unsigned> ell;  // candidates the list count pass keeps for the fill pass
   p2a::DBuf<p2a::Entry> ents;
   p2a::DBuf<int4> items;
@@ -79,7 +80,8 @@ static int batch_reserve_lists(p2a_batch* b, int ent_cap, int items_cap) {
   if ((rc = b->ents.reserve(std::max(ent_cap, 1))) || (rc = b->items.reserve(std::max(items_cap, 1))) ||
       (rc = b->fpart.reserve((size_t)CL * 3 * std::max(items_cap, 1))) ||
       (rc = b->partial.reserve(8 * (size_t)std::max(items_cap, 1))) ||
-      (rc = b->redo.reserve(std::max(items_cap, 1))))
+      (rc = b->redo.reserve(std::max(items_cap, 1))) || (rc = b->fix.reserve(std::max(items_cap, 1))) ||
+      (rc = b->recs.reserve((size_t)kBandSlots * std::max(items_cap, 1))))
     return rc;
   b->ent_cap = ent_cap;
   b->items_cap = items_cap;
@@ -127,7 +129,7 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
                      b->col_cl.p, b->origin.p, b->origin.p, b->ecount.p, (const int*)b->estart.p, b->ents.p, b->ell.p);
   hipLaunchKernelGGL(k_items, dim3(grid1(clc, 256)), dim3(256), 0, s, clc, ncl_dev, b->ecount.p, b->estart.p,
                      b->istart.p, b->items.p, chunk, b->items_cap);
-  // totals[4..5] serve as the pair kernels' redo counters
+  // totals[4..7] serve as the pair kernels' band counters
   hipLaunchKernelGGL(k_pack_bbox, dim3(grid1(nslot, 256)), dim3(256), 0, s, (int)nslot, ncl_dev, b->slot_loc.p,
                      b->info.p, pos, 0, (const SysArg*)b->sys.p, (const int*)b->atom_sys.p, b->origin.p, b->p.p,
                      b->bb.p, b->totals.p + 4);
@@ -143,8 +145,10 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
   A.twogroup = 0;
   A.pbc = b->pbc.p;
   A.cl_sys = b->cl_sys.p;
+  A.counts = b->totals.p + 4;
   A.redo = b->redo.p;
-  A.redo_count = b->totals.p + 4;
+  A.fix = b->fix.p;
+  A.recs = b->recs.p;
   int grid = grid1(std::max(b->items_cap, 1), PAIR_BLOCK / WAVE);
   if (b->tune.blocks_per_cu > 0) grid = std::min(grid, b->tune.blocks_per_cu * b->cu_count);
   launch_pairs(A, grid, std::min(grid, b->cu_count * kExactBlocksPerCu), s, false, chunk, b->items.p, b->ents.p, b->p.p, b->p.p, b->bb.p, b->bb.p,

@@ -24,7 +24,9 @@ struct Dir {  // one cluster-pair list: i clusters of group I against group J
   DBuf<Entry> ents;
   DBuf<int4> items;
   DBuf<double> fpart;  // per item: derivative partials of the 8 i atoms
-  DBuf<int> redo;      // items redone with FP64 cut decisions
+  DBuf<int> redo;      // items redone whole with FP64 cut decisions
+  DBuf<int2> fix;      // items with a few band pairs, fixed up
+  DBuf<int> recs;      // band records, kBandSlots per item
   DBuf<unsigned> ell;  // candidates the list count pass keeps for the fill pass
   int nitems = 0, item_offset = 0;
   int64_t nentries = 0;
@@ -70,7 +72,7 @@ struct p2a_coord {
   p2a::PairTuning tune;
   int cu_count = 256;
   p2a::DBuf<double> partial, red, stats, pos_buf, deriv_buf, out_buf;
-  p2a::DBuf<int> redo_count;  // one counter per direction
+  p2a::DBuf<int> counts;      // band counters, two per direction (items redone, items fixed up)
   p2a::DBuf<int> scan_bsum;   // block sums of the list scans
   p2a::DBuf<long long> abs_dev;
   int64_t rebuilds = 0;
@@ -237,7 +239,8 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     c->nitems += D.nitems;
     tot_entries += D.nentries;
     if ((rc = D.ents.reserve(std::max<int64_t>(D.nentries, 1))) || (rc = D.items.reserve(std::max(D.nitems, 1))) ||
-        (rc = D.fpart.reserve((size_t)CL * 3 * std::max(D.nitems, 1))) || (rc = D.redo.reserve(std::max(D.nitems, 1))))
+        (rc = D.fpart.reserve((size_t)CL * 3 * std::max(D.nitems, 1))) || (rc = D.redo.reserve(std::max(D.nitems, 1))) ||
+        (rc = D.fix.reserve(std::max(D.nitems, 1))) || (rc = D.recs.reserve((size_t)kBandSlots * std::max(D.nitems, 1))))
       return rc;
     Ls[d].ent_cap = (int)D.ents.n;
     if (GI.ncl > 0) {
@@ -250,7 +253,7 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     }
   }
   c->nentries = tot_entries;
-  if ((rc = c->partial.reserve(8 * (size_t)std::max(c->nitems, 1))) || (rc = c->redo_count.reserve(2))) return rc;
+  if ((rc = c->partial.reserve(8 * (size_t)std::max(c->nitems, 1))) || (rc = c->counts.reserve(4))) return rc;
   // absolute index per slot (two groups: pairs of one atom with itself are skipped)
   if (ngroups == 2) {
     for (int gi = 0; gi < 2; gi++) {
@@ -330,7 +333,7 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     if (G.ncl == 0) continue;
     hipLaunchKernelGGL(k_pack_bbox, dim3(grid1((size_t)G.ncl * CL, 256)), dim3(256), 0, s, G.ncl * CL,
                        (const int*)nullptr, G.slot_loc.p, G.info.p, pos_dev, G.off, (const SysArg*)G.sys.p,
-                       (const int*)nullptr, G.origin.p, G.p.p, G.bb.p, c->redo_count.p);
+                       (const int*)nullptr, G.origin.p, G.p.p, G.bb.p, c->counts.p);
   }
   PairArgs A;
   std::memset(&A, 0, sizeof(A));
@@ -368,8 +371,10 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     A.ent_cap = (int)D.ents.n;
     int grid = grid1(D.nitems, PAIR_BLOCK / WAVE);
     if (c->tune.blocks_per_cu > 0) grid = std::min(grid, c->tune.blocks_per_cu * c->cu_count);
+    A.counts = c->counts.p + 2 * d;
     A.redo = D.redo.p;
-    A.redo_count = c->redo_count.p + d;
+    A.fix = D.fix.p;
+    A.recs = D.recs.p;
     launch_pairs(A, grid, std::min(grid, c->cu_count * kExactBlocksPerCu), s, value_only, c->tune.chunk, D.items.p, D.ents.p, GI.p.p, GJ.p.p, GI.bb.p,
                  GJ.bb.p, GI.abs_slot.p, GJ.abs_slot.p, pos_dev, D.fpart.p, c->partial.p + 8 * (size_t)D.item_offset);
   }

@@ -103,10 +103,6 @@ struct SysArg {
   int pad_;
 };
 
-struct SwF {  // FP32 constants of fixedRational<N>
-  float dmax2, invr0_2, stretch, shift, cder;  // cder = -N*invr0_2*stretch
-};
-
 struct PbcArg {
   int type;
   double box[9], inv[9], red[9], ired[9];
@@ -736,15 +732,16 @@ __global__ void k_items(int ncl, const int* __restrict__ ncl_dev, const int* __r
 
 // k_pack and k_bbox in one pass: one thread per slot, the 8 slots of a
 // cluster are 8 adjacent lanes, which reduce the cluster's box by shuffles.
-// Also clears the pair kernels' redo counters (zero2[0..1]).
+// Also clears the pair kernels' band counters (zero4[0..3]: items redone and
+// items fixed up, for up to two list directions).
 __global__ __launch_bounds__(256) void k_pack_bbox(int nslot, const int* __restrict__ ncl_dev,
                                                    const int* __restrict__ slot_loc, const int4* __restrict__ info,
                                                    const double* __restrict__ pos, int off,
                                                    const SysArg* __restrict__ sys, const int* __restrict__ atom_sys,
                                                    const double4* __restrict__ origin, float4* __restrict__ out,
-                                                   float4* __restrict__ bb, int* __restrict__ zero2) {
+                                                   float4* __restrict__ bb, int* __restrict__ zero4) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (zero2 && s < 2) zero2[s] = 0;
+  if (zero4 && s < 4) zero4[s] = 0;
   if (ncl_dev) nslot = *ncl_dev * CL;
   // whole groups of 8 lanes stay together (nslot is a multiple of 8)
   if (s >= nslot) return;
@@ -821,39 +818,111 @@ __device__ void pbc_distance(const PbcArg& P, const double* a, const double* b, 
   }
 }
 
-// s and dfunc of fixedRational<2P> on r^2 in FP32
+// ------------------------------------------------------------------------
+// cross-lane sums on the VALU (no LDS): DPP rotations inside 16-lane rows,
+// v_permlane16_swap / v_permlane32_swap across rows
+// ------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, false);
+}
+constexpr int kRor1 = 0x121, kRor2 = 0x122, kRor4 = 0x124, kRor8 = 0x128;
+constexpr int kShr1 = 0x111, kShr2 = 0x112, kShr4 = 0x114, kShr8 = 0x118;
+constexpr int kBcast15 = 0x142, kBcast31 = 0x143;
+
+// v[l] + v[l ^ 16]: odd 16-lane rows of one copy swap with even rows of the other
+__device__ __forceinline__ float xor16_sum(float v) {
+  const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
+// v[l] + v[l ^ 32]
+__device__ __forceinline__ float xor32_sum(float v) {
+  const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
+// sum over the 8 lanes l, l^8, l^16, ..., l^56 (lane = ii + 8 jj: the sum over jj)
+__device__ __forceinline__ float sum_over_jj(float v) {
+  v += dpp_f<kRor8>(v);
+  return xor32_sum(xor16_sum(v));
+}
+// sum over the 64 lanes, in every lane
+__device__ __forceinline__ float wave_sum_f(float v) {
+  v += dpp_f<kRor8>(v);
+  v += dpp_f<kRor4>(v);
+  v += dpp_f<kRor2>(v);
+  v += dpp_f<kRor1>(v);
+  return xor32_sum(xor16_sum(v));
+}
+// exclusive prefix sum over the 64 lanes; *total = sum of all lanes (uniform)
+__device__ __forceinline__ int wave_excl_scan_dpp(int x, int& total) {
+  int v = x;
+  v += dpp_i<kShr1, 0xF>(v);
+  v += dpp_i<kShr2, 0xF>(v);
+  v += dpp_i<kShr4, 0xF>(v);
+  v += dpp_i<kShr8, 0xF>(v);
+  v += dpp_i<kBcast15, 0xA>(v);
+  v += dpp_i<kBcast31, 0xC>(v);
+  total = __builtin_amdgcn_readlane(v, 63);
+  return v - x;
+}
+
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+  const int lane = threadIdx.x & 63;
+  return (1ull << lane) - 1ull;
+}
+
+// ------------------------------------------------------------------------
+// the pair kernel
+// ------------------------------------------------------------------------
+
+// FP32 switch value and derivative factor of fixedRational<2P> on r^2
+// without its stretch: sv = 1/(1+x^P) and dfu = x^(P-1) sv^2 with x = r^2/r0^2.
+// The stretched value is stretch*sv + shift and dfunc = -2P/r0^2 stretch dfu
+// (SwitchingFunction.cpp:203-215); both factors are applied once per work
+// item, in FP64 (PairArgs::sv_scale, sv_shift, df_scale).
 template <int P>
-__device__ __forceinline__ void sw_fixed(const SwF& f, float r2, float& s, float& df) {
-  float x = r2 * f.invr0_2;
+__device__ __forceinline__ void sw_fixed(float invr0_2, float r2, float& sv, float& dfu) {
+  const float x = r2 * invr0_2;
   float xp = 1.0f;
 #pragma unroll
   for (int k = 1; k < P; k++) xp *= x;  // x^(P-1)
-  float res = __builtin_amdgcn_rcpf(fmaf(xp, x, 1.0f));
-  s = fmaf(res, f.stretch, f.shift);
-  df = f.cder * xp * res * res;
+  const float res = __builtin_amdgcn_rcpf(fmaf(xp, x, 1.0f));
+  sv = res;
+  dfu = xp * res * res;
 }
 
 template <int P>
-__device__ __forceinline__ float sw_fixed_value(const SwF& f, float r2) {
-  float x = r2 * f.invr0_2;
+__device__ __forceinline__ float sw_fixed_value(float invr0_2, float r2) {
+  const float x = r2 * invr0_2;
   float xp = 1.0f;
 #pragma unroll
   for (int k = 1; k < P; k++) xp *= x;
-  return fmaf(__builtin_amdgcn_rcpf(fmaf(xp, x, 1.0f)), f.stretch, f.shift);
+  return __builtin_amdgcn_rcpf(fmaf(xp, x, 1.0f));
 }
+
+// band records kept per work item; an item with more band pairs is redone
+// whole with FP64 cut decisions
+constexpr int kBandSlots = 8;
 
 struct PairArgs {
   int nitems;
   const int* nitems_dev;  // device item count (null: nitems)
   int items_cap, ent_cap; // capacities of the item and entry arrays
   int twogroup;
-  float dmax2, prune2;
-  SwF f;
+  float dmax2, prune2, band;
+  float invr0_2;          // FP32 constant of fixedRational
+  double sv_scale, sv_shift, df_scale;  // per-item FP32 sums -> outputs
   p2a_switch sw;          // FP64 parameters (generic switch and the cut band)
   const PbcArg* pbc;      // device copies of the prepared cells (FP64 cut decisions)
   const int* cl_sys;      // system of each i cluster (null: one system)
-  int* redo;              // items that need the FP64 cut decision (capacity: items)
-  int* redo_count;        // their number; zeroed before the FP32 pass
+  int* counts;            // [0] items redone whole, [1] items fixed up; zeroed before the FP32 pass
+  int* redo;              // items redone with FP64 cut decisions (capacity: items)
+  int2* fix;              // {item, band records} of the items fixed up (capacity: items)
+  int* recs;              // kBandSlots band records per item
 };
 
 // FP64 re-evaluation of the squared distance of one pair, used only when the
@@ -876,88 +945,81 @@ struct P3 {
   float x, y, z;
 };
 
+// per-lane FP32 sums of one work item: sv (counted pairs), dfu * d (i
+// derivative), dfu * d (x) d (virial, counted pairs)
 struct PairAcc {
-  float cv, fx, fy, fz, vxx, vxy, vxz, vyy, vyz, vzz;
-  int npair;
+  float sv, fx, fy, fz, vxx, vxy, vxz, vyy, vyz, vzz;
 };
 
-// Evaluate one group of 8 staged j atoms against the 8 i atoms of the item:
-// lane (ii, jj) takes i = ii, j = staged atom g + jj.  KIND is uniform over
-// the group: 0 = the i cluster with itself (slot order), 1 = entries that
-// count the pair (value, virial, derivative), 2 = derivative only.
-// A pair whose FP32 r^2 lies within `band` of D_MAX^2 needs the FP64 cut
-// decision: EXACT=false only raises `band_hit` (the item is then redone by
-// the EXACT=true kernel), EXACT=true decides it on the spot.
-template <int P, bool GENERIC, bool TWO, bool VO, int KIND, bool EXACT>
-__device__ __forceinline__ void eval_group(const PairArgs& A, const P3& pj, const short* __restrict__ smw,
-                                           const int* __restrict__ scw, const int* __restrict__ saw, int g,
-                                           int ii, int jj, const float4& pi, int reqi, int ai,
-                                           const float4* __restrict__ posJ, const PbcArg* __restrict__ pbc,
-                                           const double* __restrict__ pos64, float band, bool& band_hit,
-                                           PairAcc& acc) {
+// One group of 8 staged j atoms against the 8 i atoms of the item: lane
+// (ii, jj) takes i = ii, j = staged atom g + jj.  KIND is uniform over the
+// group: 0 = the i cluster with itself (slot order), 1 = entries that count
+// the pair (value, virial, derivative), 2 = derivative only.
+// A pair whose FP32 r^2 lies within `band` of D_MAX^2 is left out and
+// recorded (its item is fixed up, or redone whole, with the FP64 cut
+// decision); nrec (uniform) counts the item's records.
+template <int P, bool GENERIC, bool TWO, bool VO, int KIND>
+__device__ __forceinline__ void pair_term(const PairArgs& A, const P3& pj, const float4& pi, int ii, int jj, int g,
+                                          int ai, const int* __restrict__ saw, const short* __restrict__ smw,
+                                          int item, int& nrec, int& npair, PairAcc& acc) {
   const float dx = pj.x - pi.x;
   const float dy = pj.y - pi.y;
   const float dz = pj.z - pi.z;
   const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-  bool cut = r2 <= A.dmax2;
-  if (__builtin_expect(__ballot(fabsf(r2 - A.dmax2) < band) != 0ull, 0)) {
-    if constexpr (EXACT) {
-      const int meta = smw[g + jj];  // (entry << 3 | slot), -1 for padding
-      if (fabsf(r2 - A.dmax2) < band && reqi >= 0 && meta >= 0) {
-        const int cj = scw[meta >> 3] & kCjMask;
-        const int reqj = __float_as_int(posJ[cj * CL + (meta & 7)].w);
-        if (reqj >= 0) cut = switch_inside(A.sw, exact_r2(pbc, pos64, reqi, reqj));
-      }
-    } else {
-      band_hit = true;
+  const float t = r2 - A.dmax2;  // exact sign: t <= 0 iff r2 <= dmax2
+  bool keep = true;
+  if constexpr (KIND == 0) keep = jj != ii;  // an atom is not paired with itself
+  if constexpr (TWO) keep = keep && (saw[g + jj] != ai);
+  if constexpr (VO && KIND == 0) keep = keep && jj > ii;  // value only: each pair once
+  bool cut = keep && t <= 0.f;
+  const bool inb = keep && fabsf(t) < A.band;
+  const unsigned long long bm = __ballot(inb);
+  if (__builtin_expect(bm != 0ull, 0)) {
+    if (inb) {
+      const int k = nrec + __popcll(bm & lanemask_lt());
+      if (k < kBandSlots) A.recs[(size_t)item * kBandSlots + k] = (int)smw[g + jj] | (ii << 9) | (KIND << 12);
     }
+    nrec += __popcll(bm);
+    cut = cut && !inb;
   }
-  if (KIND == 0) cut = cut && (jj != ii);  // an atom is not paired with itself
-  if constexpr (TWO) cut = cut && (saw[g + jj] != ai);
-  if constexpr (VO) {
-    float s;
-    if constexpr (GENERIC) {
-      double dfd;
-      s = (float)switch_calc_sqr(A.sw, (double)r2, dfd);
-    } else {
-      s = sw_fixed_value<P>(A.f, r2);
-    }
-    const bool c2 = KIND == 0 ? (cut && (jj > ii)) : cut;
-    acc.npair += __popcll(__ballot(c2));
-    acc.cv += c2 ? s : 0.f;
-    return;
-  }
-  float s, df;
+  float sv, dfu;
   if constexpr (GENERIC) {
     double dfd;
-    double sd = switch_calc_sqr(A.sw, (double)r2, dfd);
-    s = (float)sd;
-    df = (float)dfd;
+    sv = (float)switch_calc_sqr(A.sw, (double)r2, dfd);
+    dfu = (float)dfd;
+  } else if constexpr (VO) {
+    sv = sw_fixed_value<P>(A.invr0_2, r2);
+    dfu = 0.f;
   } else {
-    sw_fixed<P>(A.f, r2, s, df);
+    sw_fixed<P>(A.invr0_2, r2, sv, dfu);
   }
-  df = cut ? df : 0.f;
-  const float gx = df * dx, gy = df * dy, gz = df * dz;
+  if constexpr (VO) {
+    npair += __popcll(__ballot(cut));
+    acc.sv += cut ? sv : 0.f;
+    return;
+  }
+  dfu = cut ? dfu : 0.f;
+  const float gx = dfu * dx, gy = dfu * dy, gz = dfu * dz;
   // derivative of atom i over every partner (the full list holds both
   // orientations of each cluster pair)
   acc.fx += gx;
   acc.fy += gy;
   acc.fz += gz;
-  if (KIND == 1) {
+  if constexpr (KIND == 1) {
     // value and virial once per unordered pair
-    acc.npair += __popcll(__ballot(cut));
-    acc.cv += cut ? s : 0.f;
+    npair += __popcll(__ballot(cut));
+    acc.sv += cut ? sv : 0.f;
     acc.vxx = fmaf(gx, dx, acc.vxx);
     acc.vxy = fmaf(gx, dy, acc.vxy);
     acc.vxz = fmaf(gx, dz, acc.vxz);
     acc.vyy = fmaf(gy, dy, acc.vyy);
     acc.vyz = fmaf(gy, dz, acc.vyz);
     acc.vzz = fmaf(gz, dz, acc.vzz);
-  } else if (KIND == 0) {
+  } else if constexpr (KIND == 0) {
     const bool c2 = cut && (jj > ii);
     const float wgt = c2 ? 1.f : 0.f;
-    acc.npair += __popcll(__ballot(c2));
-    acc.cv = fmaf(wgt, s, acc.cv);
+    npair += __popcll(__ballot(c2));
+    acc.sv = fmaf(wgt, sv, acc.sv);
     const float hx = wgt * gx, hy = wgt * gy, hz = wgt * gz;
     acc.vxx = fmaf(hx, dx, acc.vxx);
     acc.vxy = fmaf(hx, dy, acc.vxy);
@@ -968,77 +1030,419 @@ __device__ __forceinline__ void eval_group(const PairArgs& A, const P3& pj, cons
   }
 }
 
-// One wave per work item (i cluster, chunk of <= CH list entries); a grid
-// smaller than the item count makes the waves loop over items.
+// The same evaluation with the FP64 cut decision taken on the spot for the
+// pairs in the band (the whole-item redo of the band kernel).
+template <int P, bool GENERIC, bool TWO, bool VO, int KIND>
+__device__ __forceinline__ void pair_term_exact(const PairArgs& A, const P3& pj, const float4& pi, int ii, int jj,
+                                                int g, int reqi, int ai, const int* __restrict__ saw,
+                                                const short* __restrict__ smw, const int* __restrict__ scw,
+                                                const float4* __restrict__ posJ, const PbcArg* __restrict__ pbc,
+                                                const double* __restrict__ pos64, int& npair, PairAcc& acc) {
+  const float dx = pj.x - pi.x;
+  const float dy = pj.y - pi.y;
+  const float dz = pj.z - pi.z;
+  const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+  const float t = r2 - A.dmax2;
+  bool keep = true;
+  if constexpr (KIND == 0) keep = jj != ii;
+  if constexpr (TWO) keep = keep && (saw[g + jj] != ai);
+  if constexpr (VO && KIND == 0) keep = keep && jj > ii;
+  bool cut = keep && t <= 0.f;
+  if (keep && fabsf(t) < A.band) {
+    const int meta = smw[g + jj];  // (entry << 3 | slot), -1 for padding
+    if (reqi >= 0 && meta >= 0) {
+      const int cj = scw[meta >> 3] & kCjMask;
+      const int reqj = __float_as_int(posJ[cj * CL + (meta & 7)].w);
+      if (reqj >= 0) cut = switch_inside(A.sw, exact_r2(pbc, pos64, reqi, reqj));
+    }
+  }
+  float sv, dfu;
+  if constexpr (GENERIC) {
+    double dfd;
+    sv = (float)switch_calc_sqr(A.sw, (double)r2, dfd);
+    dfu = (float)dfd;
+  } else {
+    sw_fixed<P>(A.invr0_2, r2, sv, dfu);
+  }
+  if constexpr (VO) {
+    npair += __popcll(__ballot(cut));
+    acc.sv += cut ? sv : 0.f;
+    return;
+  }
+  dfu = cut ? dfu : 0.f;
+  const float gx = dfu * dx, gy = dfu * dy, gz = dfu * dz;
+  acc.fx += gx;
+  acc.fy += gy;
+  acc.fz += gz;
+  const bool c2 = KIND == 1 ? cut : (KIND == 0 ? (cut && jj > ii) : false);
+  if constexpr (KIND != 2) {
+    const float wgt = c2 ? 1.f : 0.f;
+    npair += __popcll(__ballot(c2));
+    acc.sv = fmaf(wgt, sv, acc.sv);
+    const float hx = wgt * gx, hy = wgt * gy, hz = wgt * gz;
+    acc.vxx = fmaf(hx, dx, acc.vxx);
+    acc.vxy = fmaf(hx, dy, acc.vxy);
+    acc.vxz = fmaf(hx, dz, acc.vxz);
+    acc.vyy = fmaf(hy, dy, acc.vyy);
+    acc.vyz = fmaf(hy, dz, acc.vyz);
+    acc.vzz = fmaf(hz, dz, acc.vzz);
+  }
+}
+
+// Groups [gb, ge) of one kind (uniform bounds, multiples of 8), two groups per
+// trip; each lane's next staged atom is read one group ahead (the staging
+// area has spare slots past the last group for that read).
+template <int P, bool GENERIC, bool TWO, bool VO, int KIND>
+__device__ __forceinline__ void pair_groups(const PairArgs& A, const P3* __restrict__ sjw, int gb, int ge,
+                                            const float4& pi, int ii, int jj, int ai, const int* __restrict__ saw,
+                                            const short* __restrict__ smw, int item, int& nrec, int& npair,
+                                            PairAcc& acc) {
+  int g = gb;
+  P3 pa = sjw[g + jj];
+  for (; g + CL < ge; g += 2 * CL) {
+    const P3 pb = sjw[g + CL + jj];
+    pair_term<P, GENERIC, TWO, VO, KIND>(A, pa, pi, ii, jj, g, ai, saw, smw, item, nrec, npair, acc);
+    pa = sjw[g + 2 * CL + jj];
+    pair_term<P, GENERIC, TWO, VO, KIND>(A, pb, pi, ii, jj, g + CL, ai, saw, smw, item, nrec, npair, acc);
+  }
+  if (g < ge) pair_term<P, GENERIC, TWO, VO, KIND>(A, pa, pi, ii, jj, g, ai, saw, smw, item, nrec, npair, acc);
+}
+
+// item outputs from the lane sums: fpart[item][ii][3] = -df_scale sum dfu d
+// over the item's partners of i atom ii; partial[item][8] = value, the six
+// products sum dfd d (x) d (write_virial negates them), pair count
+template <bool VO>
+__device__ __forceinline__ void pair_item_out(const PairArgs& A, const PairAcc& acc, int npair, int item, int ii,
+                                              int jj, double* __restrict__ fpart, double* __restrict__ partial) {
+  if constexpr (!VO) {
+    const float fx = sum_over_jj(acc.fx), fy = sum_over_jj(acc.fy), fz = sum_over_jj(acc.fz);
+    if (jj == 0) {
+      double* fp = fpart + (size_t)item * (CL * 3) + ii * 3;
+      fp[0] = -A.df_scale * (double)fx;
+      fp[1] = -A.df_scale * (double)fy;
+      fp[2] = -A.df_scale * (double)fz;
+    }
+  }
+  const float sv = wave_sum_f(acc.sv);
+  float v6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if constexpr (!VO) {
+    v6[0] = wave_sum_f(acc.vxx);
+    v6[1] = wave_sum_f(acc.vxy);
+    v6[2] = wave_sum_f(acc.vxz);
+    v6[3] = wave_sum_f(acc.vyy);
+    v6[4] = wave_sum_f(acc.vyz);
+    v6[5] = wave_sum_f(acc.vzz);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    double* out = partial + (size_t)item * 8;
+    out[0] = A.sv_scale * (double)sv + A.sv_shift * (double)npair;
+    for (int q = 0; q < 6; q++) out[1 + q] = A.df_scale * (double)v6[q];
+    out[7] = (double)npair;
+  }
+}
+
+// Staging of one work item by one wave (i cluster ci, entries [it.y, e1)):
 //  1. lane-parallel pruning of the chunk's entries against the current
 //     bounding boxes (D_MAX);
 //  2. each passing lane loads the 8 atoms of its j cluster, adds the image
 //     offset and keeps the atoms within D_MAX of the i cluster's box; the
 //     kept atoms are staged in LDS, compacted and grouped by entry kind
 //     (self / counting / derivative-only), each kind padded to a multiple
-//     of 8 with far-away atoms, so that every group of 8 has one kind;
-//  3. dense loop over the groups: lane l evaluates the pair
-//     (i = l&7 of cluster ci, j = staged atom 8g + (l>>3)).
-// VO (value only): no derivatives and no virial; derivative-only entries
-// are skipped, so the loop visits each unordered pair once.
-// EXACT=false is the FP32 pass over every item; an item with a pair in the
-// band around D_MAX^2 writes nothing and is appended to A.redo.  EXACT=true
-// redoes those items with the FP64 cut decision inline (it needs many more
-// registers, which is why it is a separate kernel); there the four waves of
-// a workgroup share one item, each evaluating every fourth group, and their
-// partials are summed in wave order.
-template <int P, bool GENERIC, bool TWO, bool VO, int CH, bool EXACT>
-__global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(EXACT ? 1 : 5))) void k_pairs(PairArgs A, const int4* __restrict__ items,
-                                                       const Entry* __restrict__ ents,
-                                                       const float4* __restrict__ posI,
-                                                       const float4* __restrict__ posJ,
-                                                       const float4* __restrict__ bbI, const float4* __restrict__ bbJ,
-                                                       const int* __restrict__ absI, const int* __restrict__ absJ,
-                                                       const double* __restrict__ pos64,
-                                                       double* __restrict__ fpart, double* __restrict__ partial) {
+//     of 8 with far-away atoms, so that every group of 8 has one kind.
+// Returns the group bounds (uniform): n0 (0 or 8), [b1, b1+n1), [b2, b2+n2).
+template <bool TWO, bool VO, int CH>
+__device__ __forceinline__ void stage_item(const PairArgs& A, const Entry& my_e, int lane, int e0, int e1,
+                                           const float4& ic, const float4& ih, const float4* __restrict__ posJ,
+                                           const float4* __restrict__ bbJ, const int* __restrict__ absJ,
+                                           P3* __restrict__ sjw, short* __restrict__ smw, int* __restrict__ scw,
+                                           int* __restrict__ saw, int& n0, int& b1, int& n1, int& b2, int& n2) {
+  Entry my = {0, 0.f, 0.f, 0.f};
+  bool pass = false;
+  {
+    const int e = e0 + lane;
+    if (lane < CH && e < e1) {
+      my = my_e;
+      if (!VO || (my.code & kCountBit) || my.code < 0) {
+        const int cj = my.code & kCjMask;
+        const float4 jc = bbJ[2 * cj], jh = bbJ[2 * cj + 1];
+        const float gx = fmaxf(fabsf(jc.x + my.dx - ic.x) - ih.x - jh.x, 0.f);
+        const float gy = fmaxf(fabsf(jc.y + my.dy - ic.y) - ih.y - jh.y, 0.f);
+        const float gz = fmaxf(fabsf(jc.z + my.dz - ic.z) - ih.z - jh.z, 0.f);
+        pass = gx * gx + gy * gy + gz * gz <= A.prune2;
+      }
+    }
+  }
+  float4 q[CL];
+  int m8 = 0, kind = 3;
+  if (pass) {
+    const int cj = my.code & kCjMask;
+#pragma unroll
+    for (int t = 0; t < CL; t++) {
+      const float4 v = posJ[cj * CL + t];
+      q[t] = make_float4(v.x + my.dx, v.y + my.dy, v.z + my.dz, 0.f);
+    }
+    if (my.code < 0) {
+      kind = 0;
+      m8 = 0xFF;  // the i cluster with itself keeps slot order
+    } else {
+      kind = (my.code & kCountBit) ? 1 : 2;
+#pragma unroll
+      for (int t = 0; t < CL; t++) {
+        const float gx = fmaxf(fabsf(q[t].x - ic.x) - ih.x, 0.f);
+        const float gy = fmaxf(fabsf(q[t].y - ic.y) - ih.y, 0.f);
+        const float gz = fmaxf(fabsf(q[t].z - ic.z) - ih.z, 0.f);
+        if (gx * gx + gy * gy + gz * gz <= A.prune2) m8 |= 1 << t;
+      }
+    }
+  }
+  const int cnt = __popc(m8);
+  n0 = __ballot(kind == 0) ? CL : 0;  // at most one self entry per item
+  int tot1, tot2 = 0;
+  const int o1 = wave_excl_scan_dpp(kind == 1 ? cnt : 0, tot1);
+  int o2 = 0;
+  if constexpr (!VO) o2 = wave_excl_scan_dpp(kind == 2 ? cnt : 0, tot2);
+  n1 = (tot1 + CL - 1) & ~(CL - 1);
+  n2 = (tot2 + CL - 1) & ~(CL - 1);
+  b1 = n0;
+  b2 = n0 + n1;
+  if (pass) {
+    scw[lane] = my.code;
+    const int cj = my.code & kCjMask;
+    int p = kind == 0 ? 0 : (kind == 1 ? b1 + o1 : b2 + o2);
+#pragma unroll
+    for (int t = 0; t < CL; t++) {
+      if ((m8 >> t) & 1) {
+        sjw[p] = P3{q[t].x, q[t].y, q[t].z};
+        smw[p] = (short)((lane << 3) | t);
+        if constexpr (TWO) saw[p] = absJ[cj * CL + t];
+        p++;
+      }
+    }
+  }
+  // padding of each kind up to a multiple of 8: far away, never within D_MAX
+  if (lane < CL) {
+    const P3 far = {1.0e5f, 1.0e5f, 1.0e5f};
+    if (tot1 + lane < n1) {
+      sjw[b1 + tot1 + lane] = far;
+      smw[b1 + tot1 + lane] = -1;
+      if constexpr (TWO) saw[b1 + tot1 + lane] = -1;
+    }
+    if (tot2 + lane < n2) {
+      sjw[b2 + tot2 + lane] = far;
+      smw[b2 + tot2 + lane] = -1;
+      if constexpr (TWO) saw[b2 + tot2 + lane] = -1;
+    }
+  }
+  // the staging above is read by other lanes of this wave only
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// FP32 pass: one wave per work item (i cluster, chunk of <= CH list
+// entries); a grid smaller than the item count makes the waves loop over
+// items.  Staging (stage_item), then a dense loop over the groups of 8
+// staged atoms: lane l evaluates the pair (i = l&7 of cluster ci, j = staged
+// atom 8g + (l>>3)).  VO (value only): no derivatives and no virial;
+// derivative-only entries are skipped, so each unordered pair is visited
+// once.  Pairs within the band around D_MAX^2 are recorded instead of
+// evaluated: an item with up to kBandSlots of them is queued in A.fix (its
+// outputs are written here and completed by the band kernel), an item with
+// more in A.redo (its outputs come from the band kernel alone).
+template <int P, bool GENERIC, bool TWO, bool VO, int CH>
+__global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_pairs(
+    PairArgs A, const int4* __restrict__ items, const Entry* __restrict__ ents, const float4* __restrict__ posI,
+    const float4* __restrict__ posJ, const float4* __restrict__ bbI, const float4* __restrict__ bbJ,
+    const int* __restrict__ absI, const int* __restrict__ absJ, double* __restrict__ fpart,
+    double* __restrict__ partial) {
   constexpr int NW = PAIR_BLOCK / WAVE;
   constexpr int CAP = CH * CL + 4 * CL;  // + padding of 3 kinds + the read-ahead slots
   __shared__ P3 sj[NW][CAP];
   __shared__ short sm[NW][CAP];
   __shared__ int scode[NW][CH];
   __shared__ int sabs[TWO ? NW : 1][TWO ? CAP : 1];
-  // EXACT: per-wave partials of the shared item (i derivatives, then value,
-  // virial and pair count)
-  __shared__ double sred[EXACT ? NW : 1][EXACT ? 3 * CL + 8 : 1];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int ii = lane & 7, jj = lane >> 3;
-  const int nitems = EXACT ? *A.redo_count : (A.nitems_dev ? min(*A.nitems_dev, A.items_cap) : A.nitems);
-  const float band = 4.0e-6f * A.dmax2;
+  const int nitems = A.nitems_dev ? min(*A.nitems_dev, A.items_cap) : A.nitems;
   P3* sjw = sj[w];
   short* smw = sm[w];
   int* scw = scode[w];
   int* saw = TWO ? sabs[w] : nullptr;
 
   // the next item's descriptor and entries are loaded while the current item
-  // computes (a wave handles several items when the grid is persistent).
-  // EXACT: the waves of a workgroup share one item (the few items of this
-  // pass are latency-bound chains), so k is uniform over the workgroup.
-  const int kstride = EXACT ? gridDim.x : gridDim.x * NW;
-  int k = EXACT ? blockIdx.x : blockIdx.x * NW + w;
-  int item_n = 0;
+  // computes (a wave handles several items when the grid is persistent)
+  const int kstride = gridDim.x * NW;
+  int k = blockIdx.x * NW + w;
   int4 it_n = make_int4(0, 0, 0, 0);
   Entry my_n = {0, 0.f, 0.f, 0.f};
   if (k < nitems) {
-    item_n = EXACT ? A.redo[k] : k;
-    it_n = items[item_n];
+    it_n = items[k];
     const int e = it_n.y + lane;
     if (lane < CH && e < min(it_n.z, A.ent_cap)) my_n = ents[e];
   }
   for (; k < nitems; k += kstride) {
-    const int item = item_n;
+    const int item = k;
     const int4 it = it_n;
     const Entry my_e = my_n;
     const int kn = k + kstride;
+    if (kn < nitems) it_n = items[kn];
+    const int ci = it.x;
+    const int e1 = min(it.z, A.ent_cap);
+    const float4 pi = posI[ci * CL + ii];
+    const int ai = TWO ? absI[ci * CL + ii] : 0;
+    const float4 ic = bbI[2 * ci], ih = bbI[2 * ci + 1];
+    int n0, b1, n1, b2, n2;
+    stage_item<TWO, VO, CH>(A, my_e, lane, it.y, e1, ic, ih, posJ, bbJ, absJ, sjw, smw, scw, saw, n0, b1, n1, b2, n2);
     if (kn < nitems) {
-      item_n = EXACT ? A.redo[kn] : kn;
-      it_n = items[item_n];
+      const int e = it_n.y + lane;
+      my_n = (lane < CH && e < min(it_n.z, A.ent_cap)) ? ents[e] : Entry{0, 0.f, 0.f, 0.f};
     }
+    // group bounds are wave-uniform: scalar loops
+    n1 = __builtin_amdgcn_readfirstlane(n1);
+    n2 = __builtin_amdgcn_readfirstlane(n2);
+    b1 = __builtin_amdgcn_readfirstlane(b1);
+    b2 = __builtin_amdgcn_readfirstlane(b2);
+    PairAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int nrec = 0, npair = 0;
+    if (n0) pair_term<P, GENERIC, TWO, VO, 0>(A, sjw[jj], pi, ii, jj, 0, ai, saw, smw, item, nrec, npair, acc);
+    pair_groups<P, GENERIC, TWO, VO, 1>(A, sjw, b1, b1 + n1, pi, ii, jj, ai, saw, smw, item, nrec, npair, acc);
+    if constexpr (!VO)
+      pair_groups<P, GENERIC, TWO, VO, 2>(A, sjw, b2, b2 + n2, pi, ii, jj, ai, saw, smw, item, nrec, npair, acc);
+    if (nrec > kBandSlots) {
+      // redone whole by the band kernel, which writes all of the item's outputs
+      if (lane == 0) A.redo[atomicAdd(A.counts, 1)] = item;
+    } else {
+      if (nrec > 0 && lane == 0) A.fix[atomicAdd(A.counts + 1, 1)] = make_int2(item, nrec);
+      pair_item_out<VO>(A, acc, npair, item, ii, jj, fpart, partial);
+    }
+    // the next item overwrites the staging read above
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Band kernel, after the FP32 pass.
+// (1) Fix-up: one wave per item of A.fix.  Lane r < nrec takes band record r
+//     (program order of the FP32 pass), rebuilds the pair's FP32 r^2 exactly
+//     as the staging did, decides the cut on the reference's FP64
+//     Pbc::distance (exact_r2) and forms the pair's FP32 terms; lane 0 adds
+//     them to the item's outputs in record order.
+// (2) Redo: the items of A.redo (more than kBandSlots band pairs), each
+//     shared by the four waves of a workgroup: the FP32 pass's staging and
+//     evaluation with the FP64 cut decision inline; the waves' partials are
+//     summed in wave order.  It needs ~50 more VGPRs than the FP32 pass,
+//     which is why it is a separate kernel.
+template <int P, bool GENERIC, bool TWO, bool VO, int CH>
+__global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(1))) void k_pairs_band(
+    PairArgs A, const int4* __restrict__ items, const Entry* __restrict__ ents, const float4* __restrict__ posI,
+    const float4* __restrict__ posJ, const float4* __restrict__ bbI, const float4* __restrict__ bbJ,
+    const int* __restrict__ absI, const int* __restrict__ absJ, const double* __restrict__ pos64,
+    double* __restrict__ fpart, double* __restrict__ partial) {
+  constexpr int NW = PAIR_BLOCK / WAVE;
+  constexpr int CAP = CH * CL + 4 * CL;
+  __shared__ P3 sj[NW][CAP];
+  __shared__ short sm[NW][CAP];
+  __shared__ int scode[NW][CH];
+  __shared__ int sabs[TWO ? NW : 1][TWO ? CAP : 1];
+  __shared__ double sred[NW][3 * CL + 8];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int ii = lane & 7, jj = lane >> 3;
+
+  // (1) fix-up
+  const int nfix = A.counts[1];
+  for (int q = blockIdx.x * NW + w; q < nfix; q += gridDim.x * NW) {
+    const int2 fx = A.fix[q];
+    const int item = fx.x, nr = min(fx.y, kBandSlots);
+    const int4 it = items[item];
+    const int ci = it.x;
+    const PbcArg* pbc = A.pbc + nsys_of(A.cl_sys, ci);
+    int rii = 0, cnt = 0;
+    float sv = 0.f, g[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f};
+    if (lane < nr) {
+      const int rec = A.recs[(size_t)item * kBandSlots + lane];
+      const int meta = rec & 511, kind = rec >> 12;
+      rii = (rec >> 9) & 7;
+      const int e = it.y + (meta >> 3), t = meta & 7;
+      const Entry en = ents[e];
+      const int cj = en.code & kCjMask;
+      const float4 pi = posI[ci * CL + rii];
+      const float4 v = posJ[cj * CL + t];
+      // the same FP32 operations as stage_item and pair_term
+      const float qx = v.x + en.dx, qy = v.y + en.dy, qz = v.z + en.dz;
+      d[0] = qx - pi.x;
+      d[1] = qy - pi.y;
+      d[2] = qz - pi.z;
+      const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
+      const bool cut = switch_inside(A.sw, exact_r2(pbc, pos64, __float_as_int(pi.w), __float_as_int(v.w)));
+      if (cut) {
+        float dfu;
+        if constexpr (GENERIC) {
+          double dfd;
+          sv = (float)switch_calc_sqr(A.sw, (double)r2, dfd);
+          dfu = (float)dfd;
+        } else {
+          sw_fixed<P>(A.invr0_2, r2, sv, dfu);
+        }
+        if (!VO) {
+          g[0] = dfu * d[0];
+          g[1] = dfu * d[1];
+          g[2] = dfu * d[2];
+        }
+        cnt = (kind == 1 || (kind == 0 && t > rii)) ? 1 : 0;
+      }
+    }
+    // lane 0 adds the records in order
+    double add[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double fadd[CL][3];
+#pragma unroll
+    for (int a = 0; a < CL; a++) fadd[a][0] = fadd[a][1] = fadd[a][2] = 0.0;
+    unsigned fmask = 0;
+    for (int r = 0; r < nr; r++) {
+      const int ri = __shfl(rii, r, 64);
+      const int rc = __shfl(cnt, r, 64);
+      const float rs = __shfl(sv, r, 64);
+      float rg[3], rd[3];
+      for (int c = 0; c < 3; c++) {
+        rg[c] = __shfl(g[c], r, 64);
+        rd[c] = __shfl(d[c], r, 64);
+      }
+      if (!VO) {
+        for (int a = 0; a < CL; a++)
+          if (a == ri)
+            for (int c = 0; c < 3; c++) fadd[a][c] += -A.df_scale * (double)rg[c];
+        fmask |= 1u << ri;
+      }
+      if (rc) {
+        add[0] += A.sv_scale * (double)rs + A.sv_shift;
+        add[1] += A.df_scale * (double)(rg[0] * rd[0]);
+        add[2] += A.df_scale * (double)(rg[0] * rd[1]);
+        add[3] += A.df_scale * (double)(rg[0] * rd[2]);
+        add[4] += A.df_scale * (double)(rg[1] * rd[1]);
+        add[5] += A.df_scale * (double)(rg[1] * rd[2]);
+        add[6] += A.df_scale * (double)(rg[2] * rd[2]);
+        add[7] += 1.0;
+      }
+    }
+    if (lane == 0) {
+      double* out = partial + (size_t)item * 8;
+      for (int c = 0; c < 8; c++) out[c] += add[c];
+      if (!VO)
+        for (int a = 0; a < CL; a++)
+          if ((fmask >> a) & 1)
+            for (int c = 0; c < 3; c++) fpart[(size_t)item * (CL * 3) + a * 3 + c] += fadd[a][c];
+    }
+  }
+
+  // (2) redo: the waves of a workgroup share one item (uniform over the workgroup)
+  const int nredo = A.counts[0];
+  P3* sjw = sj[w];
+  short* smw = sm[w];
+  int* scw = scode[w];
+  int* saw = TWO ? sabs[w] : nullptr;
+  for (int k = blockIdx.x; k < nredo; k += gridDim.x) {
+    const int item = A.redo[k];
+    const int4 it = items[item];
     const int ci = it.x;
     const int e1 = min(it.z, A.ent_cap);
     const float4 pi = posI[ci * CL + ii];
@@ -1046,203 +1450,74 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(EXAC
     const int ai = TWO ? absI[ci * CL + ii] : 0;
     const float4 ic = bbI[2 * ci], ih = bbI[2 * ci + 1];
     const PbcArg* pbc = A.pbc + nsys_of(A.cl_sys, ci);
-
-    // 1. prune the chunk's entries
-    Entry my = {0, 0.f, 0.f, 0.f};
-    bool pass = false;
-    {
-      const int e = it.y + lane;
-      if (lane < CH && e < e1) {
-        my = my_e;
-        if (!VO || (my.code & kCountBit) || my.code < 0) {
-          const int cj = my.code & kCjMask;
-          const float4 jc = bbJ[2 * cj], jh = bbJ[2 * cj + 1];
-          const float gx = fmaxf(fabsf(jc.x + my.dx - ic.x) - ih.x - jh.x, 0.f);
-          const float gy = fmaxf(fabsf(jc.y + my.dy - ic.y) - ih.y - jh.y, 0.f);
-          const float gz = fmaxf(fabsf(jc.z + my.dz - ic.z) - ih.z - jh.z, 0.f);
-          pass = gx * gx + gy * gy + gz * gz <= A.prune2;
-        }
-      }
-    }
-    // 2. keep the j atoms within reach of the i box, stage them by kind
-    float4 q[CL];
-    int m8 = 0, kind = 3;
-    if (pass) {
-      const int cj = my.code & kCjMask;
-#pragma unroll
-      for (int t = 0; t < CL; t++) {
-        const float4 v = posJ[cj * CL + t];
-        q[t] = make_float4(v.x + my.dx, v.y + my.dy, v.z + my.dz, 0.f);
-      }
-      if (my.code < 0) {
-        kind = 0;
-        m8 = 0xFF;  // the i cluster with itself keeps slot order
-      } else {
-        kind = (my.code & kCountBit) ? 1 : 2;
-#pragma unroll
-        for (int t = 0; t < CL; t++) {
-          const float gx = fmaxf(fabsf(q[t].x - ic.x) - ih.x, 0.f);
-          const float gy = fmaxf(fabsf(q[t].y - ic.y) - ih.y, 0.f);
-          const float gz = fmaxf(fabsf(q[t].z - ic.z) - ih.z, 0.f);
-          if (gx * gx + gy * gy + gz * gz <= A.prune2) m8 |= 1 << t;
-        }
-      }
-    }
-    const int cnt = __popc(m8);
-    const int n0 = __ballot(kind == 0) ? CL : 0;  // at most one self entry per item
-    int tot1, tot2 = 0;
-    const int o1 = wave_excl_scan(kind == 1 ? cnt : 0, lane, tot1);
-    int o2 = 0;
-    if constexpr (!VO) o2 = wave_excl_scan(kind == 2 ? cnt : 0, lane, tot2);
-    const int n1 = (tot1 + CL - 1) & ~(CL - 1), n2 = (tot2 + CL - 1) & ~(CL - 1);
-    const int b1 = n0, b2 = n0 + n1;
-    if (pass) {
-      scw[lane] = my.code;
-      const int cj = my.code & kCjMask;
-      int p = kind == 0 ? 0 : (kind == 1 ? b1 + o1 : b2 + o2);
-#pragma unroll
-      for (int t = 0; t < CL; t++) {
-        if ((m8 >> t) & 1) {
-          sjw[p] = P3{q[t].x, q[t].y, q[t].z};
-          smw[p] = (short)((lane << 3) | t);
-          if constexpr (TWO) saw[p] = absJ[cj * CL + t];
-          p++;
-        }
-      }
-    }
-    // padding of each kind up to a multiple of 8: far away, never within D_MAX
-    if (lane < CL) {
-      const P3 far = {1.0e5f, 1.0e5f, 1.0e5f};
-      if (tot1 + lane < n1) {
-        sjw[b1 + tot1 + lane] = far;
-        smw[b1 + tot1 + lane] = -1;
-        if constexpr (TWO) saw[b1 + tot1 + lane] = -1;
-      }
-      if (tot2 + lane < n2) {
-        sjw[b2 + tot2 + lane] = far;
-        smw[b2 + tot2 + lane] = -1;
-        if constexpr (TWO) saw[b2 + tot2 + lane] = -1;
-      }
-    }
-    // the staging above is read by other lanes of this wave only
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-
-    if (kn < nitems) {
-      const int e = it_n.y + lane;
-      my_n = (lane < CH && e < min(it_n.z, A.ent_cap)) ? ents[e] : Entry{0, 0.f, 0.f, 0.f};
-    }
-    PairAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0};
-    bool band_hit = false;  // uniform
-    if constexpr (EXACT) {
-      // every wave staged the whole item; wave w evaluates groups w, w + NW, ...
-      if (n0 && w == 0)
-        eval_group<P, GENERIC, TWO, VO, 0, true>(A, sjw[jj], smw, scw, saw, 0, ii, jj, pi, reqi, ai, posJ, pbc, pos64,
-                                                 band, band_hit, acc);
-      int gi = n0 ? 1 : 0;
-      for (int g = b1; g < b1 + n1; g += CL, gi++)
+    Entry my_e = {0, 0.f, 0.f, 0.f};
+    if (lane < CH && it.y + lane < e1) my_e = ents[it.y + lane];
+    int n0, b1, n1, b2, n2;
+    stage_item<TWO, VO, CH>(A, my_e, lane, it.y, e1, ic, ih, posJ, bbJ, absJ, sjw, smw, scw, saw, n0, b1, n1, b2, n2);
+    PairAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int npair = 0;
+    // every wave staged the whole item; wave w evaluates groups w, w + NW, ...
+    if (n0 && w == 0)
+      pair_term_exact<P, GENERIC, TWO, VO, 0>(A, sjw[jj], pi, ii, jj, 0, reqi, ai, saw, smw, scw, posJ, pbc, pos64,
+                                              npair, acc);
+    int gi = n0 ? 1 : 0;
+    for (int g = b1; g < b1 + n1; g += CL, gi++)
+      if (gi % NW == w)
+        pair_term_exact<P, GENERIC, TWO, VO, 1>(A, sjw[g + jj], pi, ii, jj, g, reqi, ai, saw, smw, scw, posJ, pbc,
+                                                pos64, npair, acc);
+    if constexpr (!VO)
+      for (int g = b2; g < b2 + n2; g += CL, gi++)
         if (gi % NW == w)
-          eval_group<P, GENERIC, TWO, VO, 1, true>(A, sjw[g + jj], smw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc,
-                                                   pos64, band, band_hit, acc);
-      if constexpr (!VO)
-        for (int g = b2; g < b2 + n2; g += CL, gi++)
-          if (gi % NW == w)
-            eval_group<P, GENERIC, TWO, VO, 2, true>(A, sjw[g + jj], smw, scw, saw, g, ii, jj, pi, reqi, ai, posJ,
-                                                     pbc, pos64, band, band_hit, acc);
-      double* red = sred[w];
-      if constexpr (!VO) {
-        double dfx = acc.fx, dfy = acc.fy, dfz = acc.fz;
-        for (int o = 8; o < 64; o <<= 1) {
-          dfx += __shfl_xor(dfx, o, 64);
-          dfy += __shfl_xor(dfy, o, 64);
-          dfz += __shfl_xor(dfz, o, 64);
-        }
-        if (jj == 0) {
-          red[ii * 3] = dfx;
-          red[ii * 3 + 1] = dfy;
-          red[ii * 3 + 2] = dfz;
-        }
-      }
-      double r[7] = {acc.cv, acc.vxx, acc.vxy, acc.vxz, acc.vyy, acc.vyz, acc.vzz};
-      for (int q2 = 0; q2 < 7; q2++) r[q2] = wave_sum(r[q2]);
-      if (lane == 0) {
-        for (int q2 = 0; q2 < 7; q2++) red[3 * CL + q2] = r[q2];
-        red[3 * CL + 7] = (double)acc.npair;
-      }
-      __syncthreads();
-      // the waves' partials summed in wave order
-      if (w == 0) {
-        if (!VO && lane < 3 * CL) {
-          double s = 0.0;
-          for (int v = 0; v < NW; v++) s += sred[v][lane];
-          fpart[(size_t)item * (CL * 3) + lane] = -s;
-        }
-        if (lane < 8) {
-          double s = 0.0;
-          for (int v = 0; v < NW; v++) s += sred[v][3 * CL + lane];
-          partial[(size_t)item * 8 + lane] = s;
-        }
-      }
-      // the next item overwrites the staging and sred
-      __syncthreads();
-      continue;
-    }
-    // each group's staged atom is read from LDS one group ahead (the staging
-    // area has 8 spare slots for the read past the last group)
-    if (n0)
-      eval_group<P, GENERIC, TWO, VO, 0, EXACT>(A, sjw[jj], smw, scw, saw, 0, ii, jj, pi, reqi, ai, posJ, pbc, pos64,
-                                                band, band_hit, acc);
-    {
-      P3 pj = sjw[b1 + jj];
-      for (int g = b1; g < b1 + n1; g += CL) {
-        const P3 pn = sjw[g + CL + jj];
-        eval_group<P, GENERIC, TWO, VO, 1, EXACT>(A, pj, smw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc, pos64,
-                                                  band, band_hit, acc);
-        pj = pn;
-      }
-    }
+          pair_term_exact<P, GENERIC, TWO, VO, 2>(A, sjw[g + jj], pi, ii, jj, g, reqi, ai, saw, smw, scw, posJ,
+                                                  pbc, pos64, npair, acc);
+    double* red = sred[w];
     if constexpr (!VO) {
-      P3 pj = sjw[b2 + jj];
-      for (int g = b2; g < b2 + n2; g += CL) {
-        const P3 pn = sjw[g + CL + jj];
-        eval_group<P, GENERIC, TWO, VO, 2, EXACT>(A, pj, smw, scw, saw, g, ii, jj, pi, reqi, ai, posJ, pbc, pos64,
-                                                  band, band_hit, acc);
-        pj = pn;
-      }
-    }
-    if (!EXACT && band_hit) {
-      // the item is redone with FP64 cut decisions; its outputs come from there
-      if (lane == 0) A.redo[atomicAdd(A.redo_count, 1)] = item;
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      continue;
-    }
-    // derivative of atom i: -sum over the 8 lanes sharing i (lanes ii + 8k);
-    // chunks of one cluster are combined by k_gather_reduce in a fixed order
-    if constexpr (!VO) {
-      double dfx = acc.fx, dfy = acc.fy, dfz = acc.fz;
-      for (int o = 8; o < 64; o <<= 1) {
-        dfx += __shfl_xor(dfx, o, 64);
-        dfy += __shfl_xor(dfy, o, 64);
-        dfz += __shfl_xor(dfz, o, 64);
-      }
+      const float fx = sum_over_jj(acc.fx), fy = sum_over_jj(acc.fy), fz = sum_over_jj(acc.fz);
       if (jj == 0) {
-        double* fp = fpart + (size_t)item * (CL * 3) + ii * 3;
-        fp[0] = -dfx;
-        fp[1] = -dfy;
-        fp[2] = -dfz;
+        red[ii * 3] = (double)fx;
+        red[ii * 3 + 1] = (double)fy;
+        red[ii * 3 + 2] = (double)fz;
       }
     }
-    double r[7] = {acc.cv, acc.vxx, acc.vxy, acc.vxz, acc.vyy, acc.vyz, acc.vzz};
-    for (int q2 = 0; q2 < 7; q2++) r[q2] = wave_sum(r[q2]);
-    if (lane == 0) {
-      double* out = partial + (size_t)item * 8;
-      for (int q2 = 0; q2 < 7; q2++) out[q2] = r[q2];
-      out[7] = (double)acc.npair;
+    const float r0 = wave_sum_f(acc.sv);
+    float r6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if constexpr (!VO) {
+      r6[0] = wave_sum_f(acc.vxx);
+      r6[1] = wave_sum_f(acc.vxy);
+      r6[2] = wave_sum_f(acc.vxz);
+      r6[3] = wave_sum_f(acc.vyy);
+      r6[4] = wave_sum_f(acc.vyz);
+      r6[5] = wave_sum_f(acc.vzz);
     }
-    // the next item overwrites the staging read above
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+      red[3 * CL] = (double)r0;
+      for (int q = 0; q < 6; q++) red[3 * CL + 1 + q] = (double)r6[q];
+      red[3 * CL + 7] = (double)npair;
+    }
+    __syncthreads();
+    // the waves' partials summed in wave order
+    if (w == 0) {
+      if (!VO && lane < 3 * CL) {
+        double s = 0.0;
+        for (int v = 0; v < NW; v++) s += sred[v][lane];
+        fpart[(size_t)item * (CL * 3) + lane] = -A.df_scale * s;
+      }
+      if (lane < 8) {
+        double s = 0.0;
+        for (int v = 0; v < NW; v++) s += sred[v][3 * CL + lane];
+        double o = s;
+        if (lane == 0) {
+          double np = 0.0;
+          for (int v = 0; v < NW; v++) np += sred[v][3 * CL + 7];
+          o = A.sv_scale * s + A.sv_shift * np;
+        } else if (lane < 7) {
+          o = A.df_scale * s;
+        }
+        partial[(size_t)item * 8 + lane] = o;
+      }
+    }
+    // the next item overwrites the staging and sred
+    __syncthreads();
   }
 }
 
@@ -1537,12 +1812,13 @@ struct PairTuning {
   int blocks_per_cu = 0; // 0: one wave per item; > 0: persistent grid of that many blocks per CU
 };
 
-// workgroups per CU of the FP64-decision pass (its register use allows 3)
+// workgroups per CU of the band kernel (its register use allows 3)
 constexpr int kExactBlocksPerCu = 3;
 
-// launches of k_pairs for a switch, group mode and tuning: the FP32 pass
-// over all items, then the FP64-decision pass over the items it set aside
-// (A.redo / A.redo_count must be set; exact_grid workgroups loop over them)
+// launches of the pair kernels for a switch, group mode and tuning: the FP32
+// pass over all items, then the band kernel over the items it queued
+// (A.counts / A.redo / A.fix / A.recs must be set; exact_grid workgroups
+// loop over them)
 inline void launch_pairs(const PairArgs& A, int grid, int exact_grid, hipStream_t s, bool value_only, int chunk,
                          const int4* items, const Entry* ents, const float4* posI,
                          const float4* posJ, const float4* bbI, const float4* bbJ, const int* absI, const int* absJ,
@@ -1550,9 +1826,9 @@ inline void launch_pairs(const PairArgs& A, int grid, int exact_grid, hipStream_
   const p2a_switch& sw = A.sw;
 #define P2A_K(P, GEN, TWO, VO, CH)                                                                                \
   do {                                                                                                            \
-    hipLaunchKernelGGL((k_pairs<P, GEN, TWO, VO, CH, false>), dim3(grid), dim3(PAIR_BLOCK), 0, s, A, items, ents,  \
-                       posI, posJ, bbI, bbJ, absI, absJ, pos64, fpart, partial);                                  \
-    hipLaunchKernelGGL((k_pairs<P, GEN, TWO, VO, CH, true>), dim3(exact_grid), dim3(PAIR_BLOCK), 0, s, A, items,   \
+    hipLaunchKernelGGL((k_pairs<P, GEN, TWO, VO, CH>), dim3(grid), dim3(PAIR_BLOCK), 0, s, A, items, ents, posI,    \
+                       posJ, bbI, bbJ, absI, absJ, fpart, partial);                                              \
+    hipLaunchKernelGGL((k_pairs_band<P, GEN, TWO, VO, CH>), dim3(exact_grid), dim3(PAIR_BLOCK), 0, s, A, items,    \
                        ents, posI, posJ, bbI, bbJ, absI, absJ, pos64, fpart, partial);                            \
   } while (0)
 #define P2A_KCH(P, GEN, TWO, VO)          \
@@ -1600,16 +1876,22 @@ inline void launch_list_scans(hipStream_t s, int ncap, const int* n_dev, const i
                      out1);
 }
 
-// FP32 switch constants and cut parameters of the pair kernel
+// FP32 constants of the pair kernels, the cut band, and the FP64 factors
+// that turn the per-item FP32 sums into outputs: for fixedRational<N> the
+// kernels sum sv = 1/(1+x^(N/2)) and dfu = x^(N/2-1) sv^2, so that
+// value = stretch sum(sv) + shift npair and dfunc = -N/r0^2 stretch dfu
+// (SwitchingFunction.cpp:203-215); the generic switches sum s and dfunc.
 inline void pair_args_switch(const p2a_switch& sw, PairArgs& A) {
   A.dmax2 = (float)sw.dmax_2;
   double pr = sw.dmax + 1e-4;
   A.prune2 = (float)(pr * pr);
-  A.f.dmax2 = (float)sw.dmax_2;
-  A.f.invr0_2 = (float)sw.invr0_2;
-  A.f.stretch = (float)sw.stretch;
-  A.f.shift = (float)sw.shift;
-  A.f.cder = (float)(-(double)sw.nn * sw.invr0_2 * sw.stretch);
+  // 1.8x the worst-case FP32 error of r^2 from cluster-local coordinates
+  A.band = 4.0e-6f * A.dmax2;
+  A.invr0_2 = (float)sw.invr0_2;
+  const bool fixed = sw.type == P2A_SW_RATIONAL_FIXED && sw.nn / 2 >= 1 && sw.nn / 2 <= 6;
+  A.sv_scale = fixed ? sw.stretch : 1.0;
+  A.sv_shift = fixed ? sw.shift : 0.0;
+  A.df_scale = fixed ? -(double)sw.nn * sw.invr0_2 * sw.stretch : 1.0;
   A.sw = sw;
 }
 
