@@ -129,10 +129,9 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
                      b->col_cl.p, b->origin.p, b->origin.p, b->ecount.p, (const int*)b->estart.p, b->ents.p, b->ell.p);
   hipLaunchKernelGGL(k_items, dim3(grid1(clc, 256)), dim3(256), 0, s, clc, ncl_dev, b->ecount.p, b->estart.p,
                      b->istart.p, b->items.p, chunk, b->items_cap);
-  // totals[4..7] serve as the pair kernels' band counters
   hipLaunchKernelGGL(k_pack_bbox, dim3(grid1(nslot, 256)), dim3(256), 0, s, (int)nslot, ncl_dev, b->slot_loc.p,
                      b->info.p, pos, 0, (const SysArg*)b->sys.p, (const int*)b->atom_sys.p, b->origin.p, b->p.p,
-                     b->bb.p, b->totals.p + 4);
+                     b->bb.p, b->totals.p + 4, 2);
   int rc;
   if ((rc = launch_check("batch: list build"))) return rc;
   PairArgs A;
@@ -145,18 +144,20 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
   A.twogroup = 0;
   A.pbc = b->pbc.p;
   A.cl_sys = b->cl_sys.p;
-  A.counts = b->totals.p + 4;
+  A.pos64 = pos;
+  A.item_partials = 1;  // per-system sums need the items' own partials
+  A.ipart = b->partial.p;
+  A.counts = b->totals.p + 4;  // cleared by k_pack_bbox
   A.redo = b->redo.p;
   A.fix = b->fix.p;
   A.recs = b->recs.p;
   int grid = grid1(std::max(b->items_cap, 1), PAIR_BLOCK / WAVE);
   if (b->tune.blocks_per_cu > 0) grid = std::min(grid, b->tune.blocks_per_cu * b->cu_count);
-  launch_pairs(A, grid, std::min(grid, b->cu_count * kExactBlocksPerCu), s, false, chunk, b->items.p, b->ents.p, b->p.p, b->p.p, b->bb.p, b->bb.p,
-               (const int*)nullptr, (const int*)nullptr, pos, b->fpart.p, b->partial.p);
+  launch_pairs(A, grid, s, false, chunk, b->items.p, b->ents.p, b->p.p, b->p.p, b->bb.p, b->bb.p, (const int*)nullptr,
+               (const int*)nullptr, b->fpart.p, b->partial.p);
   if ((rc = launch_check("batch: pair kernel"))) return rc;
   hipLaunchKernelGGL(k_gather_reduce, dim3(grid1(nslot * 3, 256)), dim3(256), 0, s, grid1(nslot * 3, 256), clc,
-                     ncl_dev, b->p.p, b->istart.p, b->items_cap, b->fpart.p, deriv, 0, (const double*)nullptr,
-                     (double*)nullptr);
+                     ncl_dev, b->p.p, b->istart.p, b->items_cap, b->fpart.p, deriv, 0, RedArgs{});
   hipLaunchKernelGGL(k_sys_reduce, dim3(b->nsys), dim3(256), 0, s, b->nsys, b->sys_colbase.p, b->col_cl.p,
                      b->istart.p, b->items_cap, b->partial.p, values, virials, b->npairs.p);
   return launch_check("batch: reductions");

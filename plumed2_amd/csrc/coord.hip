@@ -24,6 +24,7 @@ struct Dir {  // one cluster-pair list: i clusters of group I against group J
   DBuf<Entry> ents;
   DBuf<int4> items;
   DBuf<double> fpart;  // per item: derivative partials of the 8 i atoms
+  DBuf<double> ipart;  // per item with band pairs: value, virial, pair count
   DBuf<int> redo;      // items redone whole with FP64 cut decisions
   DBuf<int2> fix;      // items with a few band pairs, fixed up
   DBuf<int> recs;      // band records, kBandSlots per item
@@ -61,7 +62,10 @@ struct p2a_coord {
   bool firsttime = true;
   int path = 0;
   double rlist = 0.0;
-  double box_at_build[9] = {0};
+  double box_at_build[9] = {0};    // cell the list entries are imaged for
+  double box_at_rebuild[9] = {0};  // cell of the last rebuild (wrapping integers, images)
+  int max_wrap = 0;                // largest |lattice translation| of a wrapped atom at that rebuild
+  bool totals_clear = false;
   bool built = false;
   p2a::Group g[2];
   p2a::DBuf<int> totals;       // [4..7] atoms/clusters per group, [8..11] entries/items per direction
@@ -72,8 +76,9 @@ struct p2a_coord {
   p2a::PairTuning tune;
   int cu_count = 256;
   p2a::DBuf<double> partial, red, stats, pos_buf, deriv_buf, out_buf;
-  p2a::DBuf<int> counts;      // band counters, two per direction (items redone, items fixed up)
   p2a::DBuf<int> scan_bsum;   // block sums of the list scans
+  p2a::DBuf<int> bandz;       // band counters (two per direction), then one bitmap of items per direction
+  int band_words[2] = {0, 0};
   p2a::DBuf<long long> abs_dev;
   int64_t rebuilds = 0;
   bool profiling = false;
@@ -97,6 +102,12 @@ struct p2a_coord {
   double skin = 0.0, skin_active = 0.0;
   p2a::DBuf<double> pos_ref;  // positions at the last build
   p2a::DBuf<int> moved;
+  int* host_ro = nullptr;  // pinned host words for the few device-to-host reads
+  // neighbour list of the FP64 path: partners of every atom (symmetric CSR),
+  // or one flag per pair with PAIR; cell of every atom for NLISTCELLS
+  p2a::DBuf<int> nl_count, nl_start, nl_nbr, nl_cell;
+  p2a::DBuf<long long> nl_total;
+  bool nl_valid = false;
 };
 
 namespace p2a {
@@ -147,7 +158,9 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     sys_columns(box9, G.n, G.S);
     G.S.colbase = 0;
     G.ncol = G.S.ncx * G.S.ncy;
-    int maxcl = G.n / CL + G.ncol + 1;
+    int64_t maxcl64 = (int64_t)G.n / CL + G.ncol + 1;
+    if (maxcl64 > kMaxClusters) return fail(P2A_ERR_UNSUPPORTED, "too many atom clusters for the list encoding");
+    const int maxcl = (int)maxcl64;
     G.maxcl = maxcl;
     if ((rc = G.sys.reserve(1)) || (rc = G.info.reserve(G.n)) || (rc = G.key.reserve(G.n)) ||
         (rc = G.col_count.reserve(G.ncol + 1)) || (rc = G.col_start.reserve(G.ncol + 1)) ||
@@ -157,15 +170,20 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
         (rc = G.bb.reserve(2 * (size_t)maxcl)) || (rc = G.p.reserve((size_t)maxcl * CL)) ||
         (rc = G.kr.reserve(maxcl)) || (rc = G.abs_slot.reserve((size_t)maxcl * CL)))
       return rc;
-    // upload the descriptor when it changed (the previous step may still read the old one)
+    // the descriptor goes to the device in stream order when it changed
     if (!G.S_valid || std::memcmp(&G.S, &G.S_dev, sizeof(SysArg)) != 0) {
-      P2A_HIP(hipStreamSynchronize(s));
-      P2A_HIP(hipMemcpy(G.sys.p, &G.S, sizeof(SysArg), hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(k_store<SysArg>, dim3(1), dim3(64), 0, s, G.S, G.sys.p);
       G.S_dev = G.S;
       G.S_valid = true;
     }
   }
   if ((rc = c->totals.reserve(16))) return rc;
+  if (!c->totals_clear) {
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, c->totals.p, 16, 0);
+    c->totals_clear = true;
+  }
+  // two alternating slots for the largest wrap of this rebuild
+  const int wslot = 12 + (int)(c->rebuilds & 1);
   for (int gi = 0; gi < ngroups; gi++) {
     Group& G = c->g[gi];
     hipLaunchKernelGGL(k_fill_int, dim3(grid1(G.ncol + 1, 256)), dim3(256), 0, s, G.col_count.p, G.ncol + 1, 0);
@@ -173,7 +191,8 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     hipLaunchKernelGGL(k_fill_int, dim3(grid1(G.slot_loc.n, 256)), dim3(256), 0, s, G.slot_loc.p, (int)G.slot_loc.n, -1);
     if (G.n > 0)
       hipLaunchKernelGGL(k_bin, dim3(grid1(G.n, 256)), dim3(256), 0, s, G.n, pos_dev, G.off, (const SysArg*)G.sys.p,
-                         (const int*)nullptr, G.info.p, G.key.p, G.col_count.p);
+                         (const int*)nullptr, G.info.p, G.key.p, G.col_count.p, c->totals.p + wslot,
+                         c->totals.p + (25 - wslot));
     hipLaunchKernelGGL(k_scan_cols, dim3(1), dim3(1024), 0, s, G.ncol, G.col_count.p, G.col_start.p, G.col_cl.p,
                        c->totals.p + 4 + 2 * gi);
     if (G.n > 0) {
@@ -224,10 +243,11 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
                       8 + 2 * d, 9 + 2 * d);
   }
   if ((rc = launch_check("rebuild: list count"))) return rc;
-  int hostT[16];
-  P2A_HIP(hipMemcpyAsync(hostT, c->totals.p, sizeof(hostT), hipMemcpyDeviceToHost, s));
+  int* hostT = c->host_ro;
+  P2A_HIP(hipMemcpyAsync(hostT, c->totals.p, 16 * sizeof(int), hipMemcpyDeviceToHost, s));
   P2A_HIP(hipStreamSynchronize(s));
   for (int gi = 0; gi < ngroups; gi++) c->g[gi].ncl = hostT[5 + 2 * gi];
+  c->max_wrap = hostT[wslot];
   c->nitems = 0;
   for (int d = 0; d < c->ndir; d++) {
     Dir& D = c->dir[d];
@@ -239,8 +259,10 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     c->nitems += D.nitems;
     tot_entries += D.nentries;
     if ((rc = D.ents.reserve(std::max<int64_t>(D.nentries, 1))) || (rc = D.items.reserve(std::max(D.nitems, 1))) ||
-        (rc = D.fpart.reserve((size_t)CL * 3 * std::max(D.nitems, 1))) || (rc = D.redo.reserve(std::max(D.nitems, 1))) ||
-        (rc = D.fix.reserve(std::max(D.nitems, 1))) || (rc = D.recs.reserve((size_t)kBandSlots * std::max(D.nitems, 1))))
+        (rc = D.fpart.reserve((size_t)CL * 3 * std::max(D.nitems, 1))) ||
+        (rc = D.ipart.reserve(8 * (size_t)std::max(D.nitems, 1))) || (rc = D.redo.reserve(std::max(D.nitems, 1))) ||
+        (rc = D.fix.reserve(std::max(D.nitems, 1))) ||
+        (rc = D.recs.reserve((size_t)kBandSlots * std::max(D.nitems, 1))))
       return rc;
     Ls[d].ent_cap = (int)D.ents.n;
     if (GI.ncl > 0) {
@@ -253,7 +275,8 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     }
   }
   c->nentries = tot_entries;
-  if ((rc = c->partial.reserve(8 * (size_t)std::max(c->nitems, 1))) || (rc = c->counts.reserve(4))) return rc;
+  for (int d = 0; d < 2; d++) c->band_words[d] = d < c->ndir ? (c->dir[d].nitems + 31) / 32 : 0;
+  if ((rc = c->bandz.reserve(4 + (size_t)c->band_words[0] + c->band_words[1]))) return rc;
   // absolute index per slot (two groups: pairs of one atom with itself are skipped)
   if (ngroups == 2) {
     for (int gi = 0; gi < 2; gi++) {
@@ -270,32 +293,90 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     P2A_HIP(hipMemcpyAsync(c->pos_ref.p, pos_dev, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
   std::memcpy(c->box_at_build, box9, sizeof(c->box_at_build));
+  std::memcpy(c->box_at_rebuild, box9, sizeof(c->box_at_rebuild));
   c->built = true;
   c->rebuilds++;
   return P2A_OK;
 }
 
 // prepared cell for this box, on the host (fallback kernels take it by value)
-// and on the device (the pair kernel reads it for its FP64 cut decisions)
+// and on the device (the pair kernel reads it for its FP64 cut decisions);
+// the device copy is written by a kernel that takes it as an argument, so
+// a new box costs no host wait
 static int upload_pbc(p2a_coord* c, const double* box9) {
   int rc;
   if (c->pbc_valid && std::memcmp(c->pbc_box, box9, sizeof(c->pbc_box)) == 0 && c->pbc_dev.p) return P2A_OK;
   if ((rc = c->pbc_dev.reserve(1))) return rc;
-  // the previous upload may still read the staging copy
-  P2A_HIP(hipStreamSynchronize(c->stream));
   pbc_setup(box9, c->pbc);
   std::memcpy(c->pbc_box, box9, sizeof(c->pbc_box));
   c->pbc_valid = true;
   fill_pbc_arg(c->pbc, c->pbc_stage);
-  P2A_HIP(hipMemcpyAsync(c->pbc_dev.p, &c->pbc_stage, sizeof(PbcArg), hipMemcpyHostToDevice, c->stream));
-  return P2A_OK;
+  hipLaunchKernelGGL(k_store<PbcArg>, dim3(1), dim3(64), 0, c->stream, c->pbc_stage, c->pbc_dev.p);
+  return launch_check("cell upload");
 }
 
-// deterministic sum of nred 8-double partials into value / virial / stats;
-// with gather_dir >= 0 the first stage also forms that direction's derivatives
-static int reduce_partials(p2a_coord* c, int nred, double* value_dev, double* virial_dev, int gather_dir = -1,
-                           double* deriv_dev = nullptr) {
-  int rc;
+// A kept neighbour list under a new cell (NLIST / NLISTCELLS between two
+// rebuilds, as NeighborList keeps its pair indices across box changes,
+// NeighborList.cpp:433-456): the packed positions and the entry offsets take
+// the lattice vectors of the new cell with the integer images chosen at the
+// rebuild.  For a pair closer than D_MAX that image is the minimum image of
+// the new cell, since every cell width stays above twice the list radius
+// (choose_path), which is what Pbc::distance finds for the reference.
+// Whether the kept list, imaged for box9, still holds every pair of the
+// reference's list that comes within D_MAX.  A listed pair is measured as
+// x_j - x_i + k.a with integers k of at most 2 max_wrap + 1 per component
+// (wrapping of both atoms plus the cluster image); it is the minimum image of
+// the new cell whenever it is shorter than D_MAX (the cell stays wider than
+// twice the list radius), and a pair whose minimum image now lies within
+// D_MAX was within D_MAX + 2 displacement + |k.(a' - a)| at the rebuild.
+// The cell change may use a quarter of NL_CUTOFF - D_MAX for the last term;
+// beyond that the list is rebuilt.
+static bool reimage_exact(const p2a_coord* c, const double* box9) {
+  const double margin = c->spec.nl_cutoff - c->spec.sw.dmax;
+  if (!(margin > 0.0)) return false;
+  double drift = 0.0;
+  for (int r = 0; r < 3; r++) {
+    double q = 0.0;
+    for (int k = 0; k < 3; k++) {
+      const double t = box9[3 * r + k] - c->box_at_rebuild[3 * r + k];
+      q += t * t;
+    }
+    drift += std::sqrt(q);
+  }
+  return (2.0 * c->max_wrap + 1.0) * drift <= 0.25 * margin;
+}
+
+static int reimage(p2a_coord* c, const double* box9) {
+  hipStream_t s = c->stream;
+  const int ngroups = c->spec.nb > 0 ? 2 : 1;
+  BoxArg B;
+  std::memcpy(B.a, box9, sizeof(B.a));
+  for (int gi = 0; gi < ngroups; gi++) {
+    Group& G = c->g[gi];
+    SysArg S = G.S;  // the cell grid of the rebuild stays; only the lattice changes
+    sys_cell(box9, c->rlist, S);
+    hipLaunchKernelGGL(k_store<SysArg>, dim3(1), dim3(64), 0, s, S, G.sys.p);
+    G.S_dev = S;
+  }
+  const int wpb = 256 / WAVE;
+  for (int d = 0; d < c->ndir; d++) {
+    Dir& D = c->dir[d];
+    Group& GI = c->g[D.I];
+    Group& GJ = c->g[D.J];
+    if (GI.ncl == 0) continue;
+    hipLaunchKernelGGL(k_reimage, dim3(grid1(GI.ncl, wpb)), dim3(256), 0, s, GI.ncl, (const int*)D.estart.p,
+                       (const int*)D.ecount.p, (int)D.ents.n, (const double4*)GI.origin.p,
+                       (const double4*)GJ.origin.p, B, D.ents.p);
+  }
+  std::memcpy(c->box_at_build, box9, sizeof(c->box_at_build));
+  return launch_check("reimage");
+}
+
+// value / virial / pair count from nred rows of c->partial (and the band
+// items of ndir lists), in one workgroup and a fixed order; with
+// gather_dir >= 0 the same launch also forms that direction's derivatives
+static int reduce_partials(p2a_coord* c, int nred, int ndir, double* value_dev, double* virial_dev,
+                           int gather_dir = -1, double* deriv_dev = nullptr) {
   hipStream_t s = c->stream;
   int ngather = 0, ncl = 0;
   const float4* gp = nullptr;
@@ -312,14 +393,21 @@ static int reduce_partials(p2a_coord* c, int nred, double* value_dev, double* vi
     items_cap = (int)D.items.n;
     fpart = D.fpart.p;
   }
-  if (nred > 4 * kRedBlocks || ngather > 0) {
-    if ((rc = c->red.reserve(8 * kRedBlocks))) return rc;
-    hipLaunchKernelGGL(k_gather_reduce, dim3(ngather + kRedBlocks), dim3(256), 0, s, ngather, ncl,
-                       (const int*)nullptr, gp, istart, items_cap, fpart, deriv_dev, nred, c->partial.p, c->red.p);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(128), 0, s, kRedBlocks, c->red.p, value_dev, virial_dev, c->stats.p);
-  } else {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(128), 0, s, nred, c->partial.p, value_dev, virial_dev, c->stats.p);
+  RedArgs R;
+  std::memset(&R, 0, sizeof(R));
+  R.nred = nred;
+  R.partial = c->partial.p;
+  R.ndir = ndir;
+  for (int d = 0; d < ndir; d++) {
+    R.bandmap[d] = reinterpret_cast<const unsigned*>(c->bandz.p + 4 + (d == 0 ? 0 : c->band_words[0]));
+    R.nwords[d] = c->band_words[d];
+    R.ipart[d] = c->dir[d].ipart.p;
   }
+  R.value = value_dev;
+  R.virial = virial_dev;
+  R.stats = c->stats.p;
+  hipLaunchKernelGGL(k_gather_reduce, dim3(ngather + 1), dim3(256), 0, s, ngather, ncl, (const int*)nullptr, gp,
+                     istart, items_cap, fpart, deriv_dev, 1, R);
   return P2A_OK;
 }
 
@@ -331,9 +419,12 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
   for (int gi = 0; gi < ngroups; gi++) {
     Group& G = c->g[gi];
     if (G.ncl == 0) continue;
+    // the first launch also clears the band counters and bitmaps
+    const bool first = gi == 0 || c->g[0].ncl == 0;
     hipLaunchKernelGGL(k_pack_bbox, dim3(grid1((size_t)G.ncl * CL, 256)), dim3(256), 0, s, G.ncl * CL,
                        (const int*)nullptr, G.slot_loc.p, G.info.p, pos_dev, G.off, (const SysArg*)G.sys.p,
-                       (const int*)nullptr, G.origin.p, G.p.p, G.bb.p, c->counts.p);
+                       (const int*)nullptr, G.origin.p, G.p.p, G.bb.p, first ? c->bandz.p : (int*)nullptr,
+                       first ? 4 + c->band_words[0] + c->band_words[1] : 0);
   }
   PairArgs A;
   std::memset(&A, 0, sizeof(A));
@@ -343,6 +434,20 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
   A.pbc = c->pbc_dev.p;
   A.cl_sys = nullptr;
   A.nitems_dev = nullptr;
+  A.pos64 = pos_dev;
+  A.item_partials = 0;  // value / virial / pair count per workgroup
+  // value only: the counting direction (the first) holds every counted pair
+  const int ndir_run = value_only ? 1 : c->ndir;
+  int grids[2] = {0, 0}, nred = 0;
+  for (int d = 0; d < ndir_run; d++) {
+    Dir& D = c->dir[d];
+    if (D.nitems == 0) continue;
+    int grid = grid1(D.nitems, PAIR_BLOCK / WAVE);
+    if (c->tune.blocks_per_cu > 0) grid = std::min(grid, c->tune.blocks_per_cu * c->cu_count);
+    grids[d] = grid;
+    nred += grid;
+  }
+  if ((rc = c->partial.reserve(8 * (size_t)std::max(nred, 1)))) return rc;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const bool rec = c->profiling && (c->prof_calls++ % c->prof_every) == 0;
   if (rec) {
@@ -359,42 +464,43 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     c->ev_pool.pop_back();
     P2A_HIP(hipEventRecord(e0, s));
   }
-  // value only: the counting direction (the first) holds every counted pair
-  const int ndir_run = value_only ? 1 : c->ndir;
+  int off = 0;
   for (int d = 0; d < ndir_run; d++) {
     Dir& D = c->dir[d];
-    if (D.nitems == 0) continue;
+    if (grids[d] == 0) continue;
     Group& GI = c->g[D.I];
     Group& GJ = c->g[D.J];
     A.nitems = D.nitems;
     A.items_cap = (int)D.items.n;
     A.ent_cap = (int)D.ents.n;
-    int grid = grid1(D.nitems, PAIR_BLOCK / WAVE);
-    if (c->tune.blocks_per_cu > 0) grid = std::min(grid, c->tune.blocks_per_cu * c->cu_count);
-    A.counts = c->counts.p + 2 * d;
+    A.counts = c->bandz.p + 2 * d;
+    A.bandmap = reinterpret_cast<unsigned*>(c->bandz.p + 4 + (d == 0 ? 0 : c->band_words[0]));
+    A.ipart = D.ipart.p;
     A.redo = D.redo.p;
     A.fix = D.fix.p;
     A.recs = D.recs.p;
-    launch_pairs(A, grid, std::min(grid, c->cu_count * kExactBlocksPerCu), s, value_only, c->tune.chunk, D.items.p, D.ents.p, GI.p.p, GJ.p.p, GI.bb.p,
-                 GJ.bb.p, GI.abs_slot.p, GJ.abs_slot.p, pos_dev, D.fpart.p, c->partial.p + 8 * (size_t)D.item_offset);
+    launch_pairs(A, grids[d], s, value_only, c->tune.chunk, D.items.p, D.ents.p, GI.p.p, GJ.p.p, GI.bb.p, GJ.bb.p,
+                 GI.abs_slot.p, GJ.abs_slot.p, D.fpart.p, c->partial.p + 8 * (size_t)off);
+    off += grids[d];
   }
   if (rec) {
     P2A_HIP(hipEventRecord(e1, s));
     c->pending.emplace_back(e0, e1);
   }
   if ((rc = launch_check("pair kernel"))) return rc;
-  // derivatives of the last direction go with the first reduction stage
+  // derivatives of the last direction go with the reduction
   const int last = value_only ? -1 : c->ndir - 1;
   for (int d = 0; d < last; d++) {
     Dir& D = c->dir[d];
     Group& GI = c->g[D.I];
     if (GI.ncl == 0) continue;
-    hipLaunchKernelGGL(k_gather_reduce, dim3(grid1((size_t)GI.ncl * CL * 3, 256)), dim3(256), 0, s,
-                       grid1((size_t)GI.ncl * CL * 3, 256), GI.ncl, (const int*)nullptr, GI.p.p, D.istart.p,
-                       (int)D.items.n, D.fpart.p, deriv_dev, 0, (const double*)nullptr, (double*)nullptr);
+    const int ng = grid1((size_t)GI.ncl * CL * 3, 256);
+    RedArgs R;
+    std::memset(&R, 0, sizeof(R));
+    hipLaunchKernelGGL(k_gather_reduce, dim3(ng), dim3(256), 0, s, ng, GI.ncl, (const int*)nullptr, GI.p.p,
+                       D.istart.p, (int)D.items.n, D.fpart.p, deriv_dev, 0, R);
   }
-  if ((rc = reduce_partials(c, value_only ? c->dir[0].nitems : c->nitems, value_dev, virial_dev, last, deriv_dev)))
-    return rc;
+  if ((rc = reduce_partials(c, nred, ndir_run, value_dev, virial_dev, last, deriv_dev))) return rc;
   return launch_check("finalize");
 }
 
@@ -419,7 +525,121 @@ static int run_fallback(p2a_coord* c, const double* box9, const double* pos_dev,
                        pos_dev, c->abs_dev.p, deriv_dev, c->partial.p);
   }
   if ((rc = launch_check("fp64 pair kernel"))) return rc;
-  if ((rc = reduce_partials(c, std::max(nblocks, 1), value_dev, virial_dev))) return rc;
+  if ((rc = reduce_partials(c, std::max(nblocks, 1), 0, value_dev, virial_dev))) return rc;
+  return launch_check("finalize");
+}
+
+// LinkCells grid of the reference's cells branch for this cell and these
+// positions (LinkCells::setupCells / createCells, LinkCells.cpp:49-122): the
+// given cell, or, when the box is all zero, an orthorhombic box of whole
+// NL_CUTOFF multiples around the atoms (whose extent needs the positions on
+// the host: this rare case waits for a copy)
+static int cell_grid(p2a_coord* c, const double* box9, const double* pos_dev, CellGrid& G) {
+  std::memset(&G, 0, sizeof(G));
+  const double cut = c->spec.nl_cutoff;
+  double box[9];
+  bool zero = true;
+  for (int k = 0; k < 9; k++) zero = zero && box9[k] == 0.0;
+  if (zero) {
+    const size_t n3 = 3 * (size_t)c->natoms;
+    std::vector<double> x(n3);
+    P2A_HIP(hipMemcpyAsync(x.data(), pos_dev, n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    P2A_HIP(hipStreamSynchronize(c->stream));
+    std::memset(box, 0, sizeof(box));
+    for (int k = 0; k < 3; k++) {
+      double lo = x[k], hi = x[k];
+      for (int i = 1; i < c->natoms; i++) {
+        if (x[3 * (size_t)i + k] > hi) hi = x[3 * (size_t)i + k];
+        if (x[3 * (size_t)i + k] < lo) lo = x[3 * (size_t)i + k];
+      }
+      box[4 * k] = cut < std::sqrt(DBL_MAX) ? cut * (1 + std::ceil((hi - lo) / cut)) : hi - lo + 1;
+      G.origin[k] = (lo + hi) / 2;
+    }
+    G.shift_origin = 1;
+  } else {
+    std::memcpy(box, box9, sizeof(box));
+  }
+  PbcHost h;
+  pbc_setup(box, h);
+  if (h.type == 0 && !zero) return fail(P2A_ERR_ARG, "cell lists cannot be built for a box with null volume");
+  std::memcpy(G.inv, h.inv, sizeof(G.inv));
+  for (int k = 0; k < 3; k++) {
+    // reciprocal row k = column k of the inverse
+    const double len = std::sqrt(h.inv[k] * h.inv[k] + h.inv[3 + k] * h.inv[3 + k] + h.inv[6 + k] * h.inv[6 + k]);
+    const int n = (int)std::floor(1.0 / len / cut);
+    G.nc[k] = n == 0 ? 1 : n;
+  }
+  G.use_pbc = c->spec.pbc ? 1 : 0;
+  return P2A_OK;
+}
+
+// The FP64 path with NLIST / NLISTCELLS: the reference's candidate pairs are
+// listed on the first step and every NL_STRIDE steps (NeighborList::prepare,
+// NeighborList.cpp:433-456) and evaluated, with the current positions and
+// cell, until the next rebuild.  Classic: pairs with |d|^2 <= NL_CUTOFF^2
+// from Pbc::distance (NeighborList.cpp:237-308); cells: the pairs of atoms
+// whose link cells are within one stencil (NeighborList.cpp:177-236).
+// build: list the pairs at these positions first; value_dev null: no evaluation
+static int run_fallback_nl(p2a_coord* c, bool build, const double* box9, const double* pos_dev,
+                           double* value_dev, double* deriv_dev, double* virial_dev) {
+  hipStream_t s = c->stream;
+  int rc;
+  if ((rc = upload_pbc(c, box9))) return rc;
+  const PbcArg& P = c->pbc_stage;
+  const int nA = (int)c->spec.na, nB = (int)c->spec.nb;
+  const int natoms = c->natoms;
+  const double cut2 = c->spec.nl_cutoff * c->spec.nl_cutoff;
+  const int use_pbc = c->spec.pbc ? 1 : 0;
+  if (build || !c->nl_valid) {
+    if (c->spec.pair) {
+      if ((rc = c->nl_count.reserve(std::max(nA, 1)))) return rc;
+      hipLaunchKernelGGL(k_nl_pairs_keep, dim3(grid1(std::max(nA, 1), 256)), dim3(256), 0, s, nA, use_pbc, P, cut2,
+                         pos_dev, c->nl_count.p);
+    } else {
+      CellGrid G;
+      std::memset(&G, 0, sizeof(G));
+      const int mode = c->spec.nl_mode;
+      if ((rc = c->nl_count.reserve(natoms)) || (rc = c->nl_start.reserve((size_t)natoms + 1)) ||
+          (rc = c->nl_total.reserve(1)))
+        return rc;
+      if (mode == P2A_NL_CELLS) {
+        if ((rc = cell_grid(c, box9, pos_dev, G))) return rc;
+        if ((rc = c->nl_cell.reserve(3 * (size_t)natoms))) return rc;
+        hipLaunchKernelGGL(k_nl_cells, dim3(grid1(natoms, 256)), dim3(256), 0, s, natoms, G, pos_dev, c->nl_cell.p);
+      }
+      hipLaunchKernelGGL(k_nl_atoms<0>, dim3(grid1(natoms, 256)), dim3(256), 0, s, nA, nB, mode, use_pbc, P, cut2, G,
+                         pos_dev, (const int*)c->nl_cell.p, c->nl_count.p, (const int*)nullptr, (int*)nullptr);
+      hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, natoms, (const int*)c->nl_count.p, c->nl_start.p,
+                         c->nl_total.p);
+      long long total = 0;
+      P2A_HIP(hipMemcpyAsync(c->host_ro + 40, c->nl_total.p, sizeof(long long), hipMemcpyDeviceToHost, s));
+      P2A_HIP(hipStreamSynchronize(s));
+      std::memcpy(&total, c->host_ro + 40, sizeof(total));
+      if (total > INT32_MAX) return fail(P2A_ERR_UNSUPPORTED, "neighbour list too long for the FP64 path");
+      if ((rc = c->nl_nbr.reserve(std::max<long long>(total, 1)))) return rc;
+      hipLaunchKernelGGL(k_nl_atoms<1>, dim3(grid1(natoms, 256)), dim3(256), 0, s, nA, nB, mode, use_pbc, P, cut2, G,
+                         pos_dev, (const int*)c->nl_cell.p, (int*)nullptr, (const int*)c->nl_start.p, c->nl_nbr.p);
+    }
+    if ((rc = launch_check("FP64 neighbour list"))) return rc;
+    c->nl_valid = true;
+    c->rebuilds++;
+  }
+  if (!value_dev) return P2A_OK;
+  int nblocks;
+  if (c->spec.pair) {
+    nblocks = grid1(nA, 256);
+    if ((rc = c->partial.reserve(8 * (size_t)std::max(nblocks, 1)))) return rc;
+    hipLaunchKernelGGL(k_pairmode, dim3(std::max(nblocks, 1)), dim3(256), 0, s, nA, c->spec.pbc, P, c->spec.sw,
+                       pos_dev, c->abs_dev.p, deriv_dev, c->partial.p, (const int*)c->nl_count.p);
+  } else {
+    nblocks = grid1(natoms, 256);
+    if ((rc = c->partial.reserve(8 * (size_t)std::max(nblocks, 1)))) return rc;
+    hipLaunchKernelGGL(k_allpairs, dim3(std::max(nblocks, 1)), dim3(256), 0, s, nA, nB, c->spec.pbc, P, c->spec.sw,
+                       pos_dev, c->abs_dev.p, deriv_dev, c->partial.p, (const int*)c->nl_start.p,
+                       (const int*)c->nl_nbr.p);
+  }
+  if ((rc = launch_check("fp64 pair kernel"))) return rc;
+  if ((rc = reduce_partials(c, std::max(nblocks, 1), 0, value_dev, virial_dev))) return rc;
   return launch_check("finalize");
 }
 
@@ -446,28 +666,38 @@ static int calc_device(p2a_coord* c, int64_t step, const double* box9, const dou
     ddev = c->deriv_buf.p;
   }
   int path = choose_path(c, box9);
-  if (path != c->path) c->built = false;
+  if (path != c->path) {
+    // the list of the other path is not carried over: rebuild now
+    c->built = false;
+    c->nl_valid = false;
+  }
   c->path = path;
-  if (path != 0) return run_fallback(c, box9, pos_dev, vdev, ddev, wdev);
+  if (path != 0) {
+    if (c->spec.nl_mode != P2A_NL_NONE) {
+      const bool build = c->spec.nl_stride <= 1 || step % c->spec.nl_stride == 0;
+      return run_fallback_nl(c, build, box9, pos_dev, vdev, ddev, wdev);
+    }
+    return run_fallback(c, box9, pos_dev, vdev, ddev, wdev);
+  }
   // NeighborList::prepare (NeighborList.cpp:433-456): rebuild on the first
   // step and every NL_STRIDE steps.  Without a neighbour list every step, or,
   // with a skin, when some atom has moved more than skin/2 since the build:
   // every pair within D_MAX now was within D_MAX + skin then, so the pairs
   // evaluated are those of a list rebuilt now.
   bool need = !c->built || c->firsttime;
-  // the images of the list belong to one cell: a new cell needs a new list
-  if (!need && std::memcmp(c->box_at_build, box9, sizeof(c->box_at_build)) != 0) need = true;
+  const bool new_cell = !need && std::memcmp(c->box_at_build, box9, sizeof(c->box_at_build)) != 0;
   if (c->spec.nl_mode == P2A_NL_NONE) {
+    // the skin test measures motion in one cell: a new cell rebuilds
+    if (new_cell) need = true;
     if (!need && c->skin_active > 0.0) {
       if ((rc = c->moved.reserve(1))) return rc;
       const double half = 0.5 * c->skin_active;
       hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->moved.p, 1, 0);
       hipLaunchKernelGGL(k_moved, dim3(grid1((size_t)c->natoms, 256)), dim3(256), 0, c->stream, c->natoms, pos_dev,
                          (const double*)c->pos_ref.p, half * half, c->moved.p);
-      int m = 1;
-      P2A_HIP(hipMemcpyAsync(&m, c->moved.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      P2A_HIP(hipMemcpyAsync(c->host_ro + 32, c->moved.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
       P2A_HIP(hipStreamSynchronize(c->stream));
-      need = m != 0;
+      need = c->host_ro[32] != 0;
     } else {
       need = true;
     }
@@ -475,7 +705,12 @@ static int calc_device(p2a_coord* c, int64_t step, const double* box9, const dou
     need = true;
   }
   c->firsttime = false;
-  if (need && (rc = rebuild(c, box9, pos_dev))) return rc;
+  if (!need && new_cell && !reimage_exact(c, box9)) need = true;
+  if (need) {
+    if ((rc = rebuild(c, box9, pos_dev))) return rc;
+  } else if (new_cell) {
+    if ((rc = reimage(c, box9))) return rc;
+  }
   return run_pairs(c, box9, pos_dev, vdev, ddev, wdev, value_only);
 }
 
@@ -579,6 +814,8 @@ int p2a_coord_create(p2a_coord** out, int device, const p2a_coord_spec* spec) {
   if (spec->nl_mode != P2A_NL_NONE && (spec->nl_cutoff <= 0.0 || spec->nl_stride <= 0))
     return fail(P2A_ERR_ARG, "NL_CUTOFF and NL_STRIDE should be explicitly specified and positive");
   if (spec->na + spec->nb > (int64_t)1 << 29) return fail(P2A_ERR_UNSUPPORTED, "more than 2^29 atoms");
+  if (spec->pair && spec->nl_mode == P2A_NL_CELLS)
+    return fail(P2A_ERR_ARG, "Cell list should not be active with a Pair NL");
   if (spec->nl_mode != P2A_NL_NONE && spec->sw.dmax > spec->nl_cutoff)
     return fail(P2A_ERR_UNSUPPORTED,
                 "D_MAX larger than NL_CUTOFF: the stale neighbour list of the reference would drop pairs; "
@@ -610,6 +847,11 @@ int p2a_coord_create(p2a_coord** out, int device, const p2a_coord_spec* spec) {
     return fail(P2A_ERR_HIP, "hipStreamCreate failed");
   }
   c->stream = c->own_stream;
+  if (hipHostMalloc((void**)&c->host_ro, 64 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    c->host_ro = nullptr;
+    p2a_coord_destroy(c);
+    return fail(P2A_ERR_HIP, "hipHostMalloc failed");
+  }
   int rc;
   if ((rc = c->abs_dev.reserve(c->natoms))) {
     p2a_coord_destroy(c);
@@ -635,6 +877,7 @@ int p2a_coord_destroy(p2a_coord* c) {
   }
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  if (c->host_ro) (void)hipHostFree(c->host_ro);
   delete c;
   return P2A_OK;
 }
@@ -671,7 +914,11 @@ int p2a_coord_backward_device(p2a_coord* c, const double* box9, const double* po
   int rc;
   if ((rc = c->out_buf.reserve(16))) return rc;
   double* wdev = virial_dev ? virial_dev : c->out_buf.p + 1;
-  if (c->path != 0 || choose_path(c, box9) != 0) return run_fallback(c, box9, pos_dev, c->out_buf.p, deriv_dev, wdev);
+  if (c->path != 0 || choose_path(c, box9) != 0) {
+    // the pairs of the last listing (a neighbour list is kept as the forward left it)
+    if (c->spec.nl_mode != P2A_NL_NONE) return run_fallback_nl(c, false, box9, pos_dev, c->out_buf.p, deriv_dev, wdev);
+    return run_fallback(c, box9, pos_dev, c->out_buf.p, deriv_dev, wdev);
+  }
   if (!c->built || std::memcmp(c->box_at_build, box9, sizeof(c->box_at_build)) != 0)
     return fail(P2A_ERR_ARG, "backward needs the neighbour structure of a forward call on the same box");
   return run_pairs(c, box9, pos_dev, c->out_buf.p, deriv_dev, wdev, false);
@@ -732,9 +979,16 @@ int p2a_coord_rebuild_device(p2a_coord* c, const double* box9, const double* pos
     hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(64), 0, c->stream, c->stats.p, (size_t)8);
   }
   const int path = choose_path(c, box9);
-  if (path != c->path) c->built = false;
+  if (path != c->path) {
+    c->built = false;
+    c->nl_valid = false;
+  }
   c->path = path;
-  if (path != 0) return P2A_OK;  // the FP64 kernels keep no pair structure
+  if (path != 0) {
+    // without a neighbour list the FP64 kernels keep no pair structure
+    if (c->spec.nl_mode == P2A_NL_NONE) return P2A_OK;
+    return run_fallback_nl(c, true, box9, pos_dev, nullptr, nullptr, nullptr);
+  }
   c->firsttime = false;
   return rebuild(c, box9, pos_dev);
 }

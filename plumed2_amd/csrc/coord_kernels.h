@@ -34,7 +34,8 @@
 //   cl_bb      2 x float4 per cluster (centre, half extents), every step
 //   cl_kr      float2 per cluster: range of fractional c of its atoms
 //   list       CSR over i clusters of Entry{code, delta}: code = j cluster |
-//              count << 30 | diagonal << 31; delta = o_j + T - o_i (FP32)
+//              image << 25 | count << 30 | diagonal << 31; delta =
+//              o_j + T - o_i (FP32), T the image's lattice translation
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -92,6 +93,11 @@ struct Entry {
   float dx, dy, dz;
 };
 
+// lattice vectors of a cell (rows), by value
+struct BoxArg {
+  double a[9];
+};
+
 // cell and column grid of one system
 struct SysArg {
   double a[9];    // lattice vectors as rows
@@ -110,12 +116,16 @@ struct PbcArg {
   double shifts[8][kMaxShift][3];
 };
 
-// Entry code bits: j cluster (bits 0-29), "count" (bit 30: this entry adds
-// to the value and the virial; each unordered pair of atoms is counted by
-// exactly one entry), "diagonal" (bit 31: i cluster paired with itself
-// without image, pairs with j == i are skipped and counted when j > i).
-constexpr int kCjMask = 0x3FFFFFFF;
+// Entry code bits: j cluster (bits 0-24), image code (bits 25-29:
+// (mx+1) + 3(my+1) + 9(mz+1), the lattice translation of the j cluster),
+// "count" (bit 30: this entry adds to the value and the virial; each
+// unordered pair of atoms is counted by exactly one entry), "diagonal" (bit
+// 31: i cluster paired with itself without image, pairs with j == i are
+// skipped and counted when j > i).
+constexpr int kCjMask = 0x1FFFFFF;
+constexpr int kImgShift = 25;
 constexpr int kCountBit = 0x40000000;
+constexpr int kMaxClusters = kCjMask;
 
 namespace {
 
@@ -173,14 +183,18 @@ __device__ __forceinline__ void block_col_range(int col, int* sh, int& lo, int& 
 
 // fractional coordinates, image and column of every atom (atom i of the
 // arrays is requested atom off + i); column counts
+// mmax (optional): largest |lattice translation| of a wrapped atom, as an
+// atomic maximum; mclear is zeroed for the next rebuild (two alternating slots)
 __global__ __launch_bounds__(256) void k_bin(int n, const double* __restrict__ pos, int off,
                                              const SysArg* __restrict__ sys, const int* __restrict__ atom_sys,
                                              int4* __restrict__ info, double* __restrict__ key,
-                                             int* __restrict__ col_count) {
+                                             int* __restrict__ col_count, int* __restrict__ mmax = nullptr,
+                                             int* __restrict__ mclear = nullptr) {
   __shared__ int sh[512];
   __shared__ int hist[kBinLds];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int col = -1;
+  int am = 0;
   if (i < n) {
     const SysArg& S = sys[nsys_of(atom_sys, i)];
     const double* x = pos + 3 * (size_t)(off + i);
@@ -199,6 +213,12 @@ __global__ __launch_bounds__(256) void k_bin(int n, const double* __restrict__ p
     col = S.colbase + cx + S.ncx * cy;
     info[i] = make_int4(m[0], m[1], m[2], col);
     key[i] = sf[2];
+    am = max(abs(m[0]), max(abs(m[1]), abs(m[2])));
+  }
+  if (mmax) {
+    for (int o = 32; o > 0; o >>= 1) am = max(am, __shfl_xor(am, o, 64));
+    if ((threadIdx.x & 63) == 0 && am > 0) atomicMax(mmax, am);
+    if (i == 0) *mclear = 0;
   }
   int lo, hi;
   block_col_range(col, sh, lo, hi);
@@ -542,11 +562,46 @@ __device__ __forceinline__ Entry make_entry(int mode, int ci, int cj, int mx, in
   const bool positive = (mx > 0) || (mx == 0 && (my > 0 || (my == 0 && mz > 0)));
   const bool counted = mode == 1 || (mode == 0 && (cj > ci || (cj == ci && positive)));
   Entry en;
-  en.code = cj | (counted ? kCountBit : 0) | (diag ? (int)0x80000000u : 0);
+  en.code = cj | (((mx + 1) + 3 * (my + 1) + 9 * (mz + 1)) << kImgShift) | (counted ? kCountBit : 0) |
+            (diag ? (int)0x80000000u : 0);
   en.dx = (float)(oj.x + Tx - oi.x);
   en.dy = (float)(oj.y + Ty - oi.y);
   en.dz = (float)(oj.z + Tz - oi.z);
   return en;
+}
+
+// entry offsets of a kept list for a new cell: the j cluster's lattice
+// translation (mx, my, mz) now uses the lattice vectors a (rows), as the
+// packed positions do; one wave per i cluster, lanes over its entries
+__global__ __launch_bounds__(256) void k_reimage(int ncl, const int* __restrict__ estart,
+                                                 const int* __restrict__ ecount, int ent_cap,
+                                                 const double4* __restrict__ orgI, const double4* __restrict__ orgJ,
+                                                 BoxArg box, Entry* __restrict__ ents) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int ci = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+  if (ci >= ncl) return;
+  const int e0 = estart[ci], n = ecount[ci];
+  const double4 oi = orgI[ci];
+  for (int k = lane; k < n; k += WAVE) {
+    const int e = e0 + k;
+    if (e >= ent_cap) break;
+    Entry en = ents[e];
+    const int cj = en.code & kCjMask;
+    const int im = (en.code >> kImgShift) & 31;
+    const int mx = im % 3 - 1, my = (im / 3) % 3 - 1, mz = im / 9 - 1;
+    const double4 oj = orgJ[cj];
+    en.dx = (float)(oj.x + (mx * box.a[0] + my * box.a[3] + mz * box.a[6]) - oi.x);
+    en.dy = (float)(oj.y + (mx * box.a[1] + my * box.a[4] + mz * box.a[7]) - oi.y);
+    en.dz = (float)(oj.z + (mx * box.a[2] + my * box.a[5] + mz * box.a[8]) - oi.z);
+    ents[e] = en;
+  }
+}
+
+// one value written to device memory in stream order: small descriptors go
+// to the device as kernel arguments, so the host never waits for a staging copy
+template <class T>
+__global__ void k_store(T v, T* __restrict__ dst) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *dst = v;
 }
 
 struct ListArgs {
@@ -732,16 +787,16 @@ __global__ void k_items(int ncl, const int* __restrict__ ncl_dev, const int* __r
 
 // k_pack and k_bbox in one pass: one thread per slot, the 8 slots of a
 // cluster are 8 adjacent lanes, which reduce the cluster's box by shuffles.
-// Also clears the pair kernels' band counters (zero4[0..3]: items redone and
-// items fixed up, for up to two list directions).
+// Also clears nzero ints at `zero` (the pair kernels' band counters and
+// bitmaps).
 __global__ __launch_bounds__(256) void k_pack_bbox(int nslot, const int* __restrict__ ncl_dev,
                                                    const int* __restrict__ slot_loc, const int4* __restrict__ info,
                                                    const double* __restrict__ pos, int off,
                                                    const SysArg* __restrict__ sys, const int* __restrict__ atom_sys,
                                                    const double4* __restrict__ origin, float4* __restrict__ out,
-                                                   float4* __restrict__ bb, int* __restrict__ zero4) {
+                                                   float4* __restrict__ bb, int* __restrict__ zero, int nzero) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (zero4 && s < 4) zero4[s] = 0;
+  for (int z = s; z < nzero; z += gridDim.x * blockDim.x) zero[z] = 0;
   if (ncl_dev) nslot = *ncl_dev * CL;
   // whole groups of 8 lanes stay together (nslot is a multiple of 8)
   if (s >= nslot) return;
@@ -778,6 +833,10 @@ __global__ __launch_bounds__(256) void k_pack_bbox(int nslot, const int* __restr
     bb[2 * c + 1] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]), 0.5f * (hi[2] - lo[2]), 0.f);
   }
 }
+
+// The FP64 helpers below take the reference's rounding: no fused
+// multiply-add contraction (the reference's host build has none).
+#pragma clang fp contract(off)
 
 // Tools::pbc (Tools.h:547-570)
 __device__ __forceinline__ double pbc1(double x) {
@@ -817,6 +876,8 @@ __device__ void pbc_distance(const PbcArg& P, const double* a, const double* b, 
     }
   }
 }
+
+#pragma clang fp contract(fast)
 
 // ------------------------------------------------------------------------
 // cross-lane sums on the VALU (no LDS): DPP rotations inside 16-lane rows,
@@ -870,9 +931,9 @@ __device__ __forceinline__ int wave_excl_scan_dpp(int x, int& total) {
   return v - x;
 }
 
-__device__ __forceinline__ unsigned long long lanemask_lt() {
-  const int lane = threadIdx.x & 63;
-  return (1ull << lane) - 1ull;
+// number of set bits of m in the lanes below this one (v_mbcnt)
+__device__ __forceinline__ int bits_below(unsigned long long m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
 // ------------------------------------------------------------------------
@@ -904,6 +965,8 @@ __device__ __forceinline__ float sw_fixed_value(float invr0_2, float r2) {
   return __builtin_amdgcn_rcpf(fmaf(xp, x, 1.0f));
 }
 
+// band records a wave holds in LDS before it evaluates them (an item with
+// more is evaluated again whole with FP64 cut decisions)
 // band records kept per work item; an item with more band pairs is redone
 // whole with FP64 cut decisions
 constexpr int kBandSlots = 8;
@@ -919,7 +982,15 @@ struct PairArgs {
   p2a_switch sw;          // FP64 parameters (generic switch and the cut band)
   const PbcArg* pbc;      // device copies of the prepared cells (FP64 cut decisions)
   const int* cl_sys;      // system of each i cluster (null: one system)
-  int* counts;            // [0] items redone whole, [1] items fixed up; zeroed before the FP32 pass
+  const double* pos64;    // FP64 positions of the requested atoms (band pairs)
+  // Value, six virial products and pair count: with item_partials every item
+  // writes its own row of ipart; otherwise the items without band pairs add
+  // into their workgroup's row of `partial`, and the items with band pairs
+  // write their row of ipart and set their bit in bandmap.
+  int item_partials;
+  double* ipart;
+  unsigned* bandmap;      // cleared before the pass
+  int* counts;            // [0] items redone whole, [1] items fixed up; cleared before the pass
   int* redo;              // items redone with FP64 cut decisions (capacity: items)
   int2* fix;              // {item, band records} of the items fixed up (capacity: items)
   int* recs;              // kBandSlots band records per item
@@ -933,6 +1004,7 @@ struct PairArgs {
 // index first inside one group) because the rounding of Tools::pbc is not
 // symmetric under d -> -d.
 __device__ double exact_r2(const PbcArg* __restrict__ P, const double* __restrict__ pos, int reqi, int reqj) {
+#pragma clang fp contract(off)
   double d[3];
   const int a = min(reqi, reqj), b = max(reqi, reqj);
   pbc_distance(*P, pos + 3 * (size_t)a, pos + 3 * (size_t)b, d);
@@ -951,17 +1023,37 @@ struct PairAcc {
   float sv, fx, fy, fz, vxx, vxy, vxz, vyy, vyz, vzz;
 };
 
+// the wave's LDS areas for one work item
+struct WaveLds {
+  P3* sj;       // staged j atoms
+  short* sm;    // their origin: entry << 3 | slot, -1 for padding
+  int* sc;      // entry codes of the item
+  int* sa;      // absolute index of each staged atom (two groups)
+};
+
+// FP32 switch terms of one pair on its FP32 r^2
+template <int P, bool GENERIC>
+__device__ __forceinline__ void pair_switch(const PairArgs& A, float r2, float& sv, float& dfu) {
+  if constexpr (GENERIC) {
+    double dfd;
+    sv = (float)switch_calc_sqr(A.sw, (double)r2, dfd);
+    dfu = (float)dfd;
+  } else {
+    sw_fixed<P>(A.invr0_2, r2, sv, dfu);
+  }
+}
+
 // One group of 8 staged j atoms against the 8 i atoms of the item: lane
 // (ii, jj) takes i = ii, j = staged atom g + jj.  KIND is uniform over the
 // group: 0 = the i cluster with itself (slot order), 1 = entries that count
 // the pair (value, virial, derivative), 2 = derivative only.
-// A pair whose FP32 r^2 lies within `band` of D_MAX^2 is left out and
-// recorded (its item is fixed up, or redone whole, with the FP64 cut
-// decision); nrec (uniform) counts the item's records.
+// A pair whose FP32 r^2 lies within `band` of D_MAX^2 is left out here and
+// recorded in the item's band slots; the band kernel completes the item with
+// the FP64 cut decision.  nrec (uniform) counts the item's band pairs.
 template <int P, bool GENERIC, bool TWO, bool VO, int KIND>
 __device__ __forceinline__ void pair_term(const PairArgs& A, const P3& pj, const float4& pi, int ii, int jj, int g,
-                                          int ai, const int* __restrict__ saw, const short* __restrict__ smw,
-                                          int item, int& nrec, int& npair, PairAcc& acc) {
+                                          int ai, const WaveLds& L, int item, int& nrec, int& npair,
+                                          PairAcc& acc) {
   const float dx = pj.x - pi.x;
   const float dy = pj.y - pi.y;
   const float dz = pj.z - pi.z;
@@ -969,29 +1061,25 @@ __device__ __forceinline__ void pair_term(const PairArgs& A, const P3& pj, const
   const float t = r2 - A.dmax2;  // exact sign: t <= 0 iff r2 <= dmax2
   bool keep = true;
   if constexpr (KIND == 0) keep = jj != ii;  // an atom is not paired with itself
-  if constexpr (TWO) keep = keep && (saw[g + jj] != ai);
+  if constexpr (TWO) keep = keep && (L.sa[g + jj] != ai);
   if constexpr (VO && KIND == 0) keep = keep && jj > ii;  // value only: each pair once
-  bool cut = keep && t <= 0.f;
+  // inside D_MAX and outside the band (t <= 0 and not |t| < band)
+  const bool cut = keep && t <= -A.band;
   const bool inb = keep && fabsf(t) < A.band;
   const unsigned long long bm = __ballot(inb);
   if (__builtin_expect(bm != 0ull, 0)) {
     if (inb) {
-      const int k = nrec + __popcll(bm & lanemask_lt());
-      if (k < kBandSlots) A.recs[(size_t)item * kBandSlots + k] = (int)smw[g + jj] | (ii << 9) | (KIND << 12);
+      const int k = nrec + bits_below(bm);
+      if (k < kBandSlots) A.recs[(size_t)item * kBandSlots + k] = (int)L.sm[g + jj] | (ii << 9) | (KIND << 12);
     }
     nrec += __popcll(bm);
-    cut = cut && !inb;
   }
   float sv, dfu;
-  if constexpr (GENERIC) {
-    double dfd;
-    sv = (float)switch_calc_sqr(A.sw, (double)r2, dfd);
-    dfu = (float)dfd;
-  } else if constexpr (VO) {
+  if constexpr (VO && !GENERIC) {
     sv = sw_fixed_value<P>(A.invr0_2, r2);
     dfu = 0.f;
   } else {
-    sw_fixed<P>(A.invr0_2, r2, sv, dfu);
+    pair_switch<P, GENERIC>(A, r2, sv, dfu);
   }
   if constexpr (VO) {
     npair += __popcll(__ballot(cut));
@@ -1030,14 +1118,31 @@ __device__ __forceinline__ void pair_term(const PairArgs& A, const P3& pj, const
   }
 }
 
+// Groups [gb, ge) of one kind (uniform bounds, multiples of 8), two groups per
+// trip; each lane's next staged atom is read one group ahead (the staging
+// area has spare slots past the last group for that read).
+template <int P, bool GENERIC, bool TWO, bool VO, int KIND>
+__device__ __forceinline__ void pair_groups(const PairArgs& A, int gb, int ge, const float4& pi, int ii, int jj,
+                                            int ai, const WaveLds& L, int item, int& nrec, int& npair,
+                                            PairAcc& acc) {
+  int g = gb;
+  P3 pa = L.sj[g + jj];
+  for (; g + CL < ge; g += 2 * CL) {
+    const P3 pb = L.sj[g + CL + jj];
+    pair_term<P, GENERIC, TWO, VO, KIND>(A, pa, pi, ii, jj, g, ai, L, item, nrec, npair, acc);
+    pa = L.sj[g + 2 * CL + jj];
+    pair_term<P, GENERIC, TWO, VO, KIND>(A, pb, pi, ii, jj, g + CL, ai, L, item, nrec, npair, acc);
+  }
+  if (g < ge) pair_term<P, GENERIC, TWO, VO, KIND>(A, pa, pi, ii, jj, g, ai, L, item, nrec, npair, acc);
+}
+
 // The same evaluation with the FP64 cut decision taken on the spot for the
 // pairs in the band (the whole-item redo of the band kernel).
 template <int P, bool GENERIC, bool TWO, bool VO, int KIND>
 __device__ __forceinline__ void pair_term_exact(const PairArgs& A, const P3& pj, const float4& pi, int ii, int jj,
-                                                int g, int reqi, int ai, const int* __restrict__ saw,
-                                                const short* __restrict__ smw, const int* __restrict__ scw,
+                                                int g, int reqi, int ai, const WaveLds& L,
                                                 const float4* __restrict__ posJ, const PbcArg* __restrict__ pbc,
-                                                const double* __restrict__ pos64, int& npair, PairAcc& acc) {
+                                                int& npair, PairAcc& acc) {
   const float dx = pj.x - pi.x;
   const float dy = pj.y - pi.y;
   const float dz = pj.z - pi.z;
@@ -1045,25 +1150,19 @@ __device__ __forceinline__ void pair_term_exact(const PairArgs& A, const P3& pj,
   const float t = r2 - A.dmax2;
   bool keep = true;
   if constexpr (KIND == 0) keep = jj != ii;
-  if constexpr (TWO) keep = keep && (saw[g + jj] != ai);
+  if constexpr (TWO) keep = keep && (L.sa[g + jj] != ai);
   if constexpr (VO && KIND == 0) keep = keep && jj > ii;
   bool cut = keep && t <= 0.f;
   if (keep && fabsf(t) < A.band) {
-    const int meta = smw[g + jj];  // (entry << 3 | slot), -1 for padding
+    const int meta = L.sm[g + jj];  // (entry << 3 | slot), -1 for padding
     if (reqi >= 0 && meta >= 0) {
-      const int cj = scw[meta >> 3] & kCjMask;
+      const int cj = L.sc[meta >> 3] & kCjMask;
       const int reqj = __float_as_int(posJ[cj * CL + (meta & 7)].w);
-      if (reqj >= 0) cut = switch_inside(A.sw, exact_r2(pbc, pos64, reqi, reqj));
+      if (reqj >= 0) cut = switch_inside(A.sw, exact_r2(pbc, A.pos64, reqi, reqj));
     }
   }
   float sv, dfu;
-  if constexpr (GENERIC) {
-    double dfd;
-    sv = (float)switch_calc_sqr(A.sw, (double)r2, dfd);
-    dfu = (float)dfd;
-  } else {
-    sw_fixed<P>(A.invr0_2, r2, sv, dfu);
-  }
+  pair_switch<P, GENERIC>(A, r2, sv, dfu);
   if constexpr (VO) {
     npair += __popcll(__ballot(cut));
     acc.sv += cut ? sv : 0.f;
@@ -1089,59 +1188,7 @@ __device__ __forceinline__ void pair_term_exact(const PairArgs& A, const P3& pj,
   }
 }
 
-// Groups [gb, ge) of one kind (uniform bounds, multiples of 8), two groups per
-// trip; each lane's next staged atom is read one group ahead (the staging
-// area has spare slots past the last group for that read).
-template <int P, bool GENERIC, bool TWO, bool VO, int KIND>
-__device__ __forceinline__ void pair_groups(const PairArgs& A, const P3* __restrict__ sjw, int gb, int ge,
-                                            const float4& pi, int ii, int jj, int ai, const int* __restrict__ saw,
-                                            const short* __restrict__ smw, int item, int& nrec, int& npair,
-                                            PairAcc& acc) {
-  int g = gb;
-  P3 pa = sjw[g + jj];
-  for (; g + CL < ge; g += 2 * CL) {
-    const P3 pb = sjw[g + CL + jj];
-    pair_term<P, GENERIC, TWO, VO, KIND>(A, pa, pi, ii, jj, g, ai, saw, smw, item, nrec, npair, acc);
-    pa = sjw[g + 2 * CL + jj];
-    pair_term<P, GENERIC, TWO, VO, KIND>(A, pb, pi, ii, jj, g + CL, ai, saw, smw, item, nrec, npair, acc);
-  }
-  if (g < ge) pair_term<P, GENERIC, TWO, VO, KIND>(A, pa, pi, ii, jj, g, ai, saw, smw, item, nrec, npair, acc);
-}
-
-// item outputs from the lane sums: fpart[item][ii][3] = -df_scale sum dfu d
-// over the item's partners of i atom ii; partial[item][8] = value, the six
-// products sum dfd d (x) d (write_virial negates them), pair count
-template <bool VO>
-__device__ __forceinline__ void pair_item_out(const PairArgs& A, const PairAcc& acc, int npair, int item, int ii,
-                                              int jj, double* __restrict__ fpart, double* __restrict__ partial) {
-  if constexpr (!VO) {
-    const float fx = sum_over_jj(acc.fx), fy = sum_over_jj(acc.fy), fz = sum_over_jj(acc.fz);
-    if (jj == 0) {
-      double* fp = fpart + (size_t)item * (CL * 3) + ii * 3;
-      fp[0] = -A.df_scale * (double)fx;
-      fp[1] = -A.df_scale * (double)fy;
-      fp[2] = -A.df_scale * (double)fz;
-    }
-  }
-  const float sv = wave_sum_f(acc.sv);
-  float v6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  if constexpr (!VO) {
-    v6[0] = wave_sum_f(acc.vxx);
-    v6[1] = wave_sum_f(acc.vxy);
-    v6[2] = wave_sum_f(acc.vxz);
-    v6[3] = wave_sum_f(acc.vyy);
-    v6[4] = wave_sum_f(acc.vyz);
-    v6[5] = wave_sum_f(acc.vzz);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    double* out = partial + (size_t)item * 8;
-    out[0] = A.sv_scale * (double)sv + A.sv_shift * (double)npair;
-    for (int q = 0; q < 6; q++) out[1 + q] = A.df_scale * (double)v6[q];
-    out[7] = (double)npair;
-  }
-}
-
-// Staging of one work item by one wave (i cluster ci, entries [it.y, e1)):
+// Staging of one work item by one wave (i cluster ci, entries [e0, e1)):
 //  1. lane-parallel pruning of the chunk's entries against the current
 //     bounding boxes (D_MAX);
 //  2. each passing lane loads the 8 atoms of its j cluster, adds the image
@@ -1154,8 +1201,7 @@ template <bool TWO, bool VO, int CH>
 __device__ __forceinline__ void stage_item(const PairArgs& A, const Entry& my_e, int lane, int e0, int e1,
                                            const float4& ic, const float4& ih, const float4* __restrict__ posJ,
                                            const float4* __restrict__ bbJ, const int* __restrict__ absJ,
-                                           P3* __restrict__ sjw, short* __restrict__ smw, int* __restrict__ scw,
-                                           int* __restrict__ saw, int& n0, int& b1, int& n1, int& b2, int& n2) {
+                                           const WaveLds& L, int& n0, int& b1, int& n1, int& b2, int& n2) {
   Entry my = {0, 0.f, 0.f, 0.f};
   bool pass = false;
   {
@@ -1206,15 +1252,15 @@ __device__ __forceinline__ void stage_item(const PairArgs& A, const Entry& my_e,
   b1 = n0;
   b2 = n0 + n1;
   if (pass) {
-    scw[lane] = my.code;
+    L.sc[lane] = my.code;
     const int cj = my.code & kCjMask;
     int p = kind == 0 ? 0 : (kind == 1 ? b1 + o1 : b2 + o2);
 #pragma unroll
     for (int t = 0; t < CL; t++) {
       if ((m8 >> t) & 1) {
-        sjw[p] = P3{q[t].x, q[t].y, q[t].z};
-        smw[p] = (short)((lane << 3) | t);
-        if constexpr (TWO) saw[p] = absJ[cj * CL + t];
+        L.sj[p] = P3{q[t].x, q[t].y, q[t].z};
+        L.sm[p] = (short)((lane << 3) | t);
+        if constexpr (TWO) L.sa[p] = absJ[cj * CL + t];
         p++;
       }
     }
@@ -1223,19 +1269,26 @@ __device__ __forceinline__ void stage_item(const PairArgs& A, const Entry& my_e,
   if (lane < CL) {
     const P3 far = {1.0e5f, 1.0e5f, 1.0e5f};
     if (tot1 + lane < n1) {
-      sjw[b1 + tot1 + lane] = far;
-      smw[b1 + tot1 + lane] = -1;
-      if constexpr (TWO) saw[b1 + tot1 + lane] = -1;
+      L.sj[b1 + tot1 + lane] = far;
+      L.sm[b1 + tot1 + lane] = -1;
+      if constexpr (TWO) L.sa[b1 + tot1 + lane] = -1;
     }
     if (tot2 + lane < n2) {
-      sjw[b2 + tot2 + lane] = far;
-      smw[b2 + tot2 + lane] = -1;
-      if constexpr (TWO) saw[b2 + tot2 + lane] = -1;
+      L.sj[b2 + tot2 + lane] = far;
+      L.sm[b2 + tot2 + lane] = -1;
+      if constexpr (TWO) L.sa[b2 + tot2 + lane] = -1;
     }
   }
   // the staging above is read by other lanes of this wave only
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
+}
+
+// staged atoms per wave: 64 entries x 8, padding of two kinds, and the
+// one-group read-ahead (the largest group end is 520)
+template <int CH>
+constexpr int stage_cap() {
+  return CH * CL + 2 * CL;
 }
 
 // FP32 pass: one wave per work item (i cluster, chunk of <= CH list
@@ -1244,10 +1297,13 @@ __device__ __forceinline__ void stage_item(const PairArgs& A, const Entry& my_e,
 // staged atoms: lane l evaluates the pair (i = l&7 of cluster ci, j = staged
 // atom 8g + (l>>3)).  VO (value only): no derivatives and no virial;
 // derivative-only entries are skipped, so each unordered pair is visited
-// once.  Pairs within the band around D_MAX^2 are recorded instead of
-// evaluated: an item with up to kBandSlots of them is queued in A.fix (its
-// outputs are written here and completed by the band kernel), an item with
-// more in A.redo (its outputs come from the band kernel alone).
+// once.  Pairs within the band around D_MAX^2 are recorded: an item with up
+// to kBandSlots of them is queued in A.fix (its outputs are written here and
+// completed by the band kernel), an item with more in A.redo (its outputs
+// come from the band kernel alone).
+// Outputs: fpart[item][8][3], the derivative of each i atom over the item,
+// and the value / virial / pair count rows described at PairArgs; the
+// workgroup rows are the FP64 sums of each wave's items, added in wave order.
 template <int P, bool GENERIC, bool TWO, bool VO, int CH>
 __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_pairs(
     PairArgs A, const int4* __restrict__ items, const Entry* __restrict__ ents, const float4* __restrict__ posI,
@@ -1255,19 +1311,18 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
     const int* __restrict__ absI, const int* __restrict__ absJ, double* __restrict__ fpart,
     double* __restrict__ partial) {
   constexpr int NW = PAIR_BLOCK / WAVE;
-  constexpr int CAP = CH * CL + 4 * CL;  // + padding of 3 kinds + the read-ahead slots
+  constexpr int CAP = stage_cap<CH>();
   __shared__ P3 sj[NW][CAP];
   __shared__ short sm[NW][CAP];
   __shared__ int scode[NW][CH];
   __shared__ int sabs[TWO ? NW : 1][TWO ? CAP : 1];
+  __shared__ double stot[NW][8];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int ii = lane & 7, jj = lane >> 3;
   const int nitems = A.nitems_dev ? min(*A.nitems_dev, A.items_cap) : A.nitems;
-  P3* sjw = sj[w];
-  short* smw = sm[w];
-  int* scw = scode[w];
-  int* saw = TWO ? sabs[w] : nullptr;
+  const WaveLds L = {sj[w], sm[w], scode[w], TWO ? sabs[w] : nullptr};
+  if (lane < 8) stot[w][lane] = 0.0;
 
   // the next item's descriptor and entries are loaded while the current item
   // computes (a wave handles several items when the grid is persistent)
@@ -1292,7 +1347,7 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
     const int ai = TWO ? absI[ci * CL + ii] : 0;
     const float4 ic = bbI[2 * ci], ih = bbI[2 * ci + 1];
     int n0, b1, n1, b2, n2;
-    stage_item<TWO, VO, CH>(A, my_e, lane, it.y, e1, ic, ih, posJ, bbJ, absJ, sjw, smw, scw, saw, n0, b1, n1, b2, n2);
+    stage_item<TWO, VO, CH>(A, my_e, lane, it.y, e1, ic, ih, posJ, bbJ, absJ, L, n0, b1, n1, b2, n2);
     if (kn < nitems) {
       const int e = it_n.y + lane;
       my_n = (lane < CH && e < min(it_n.z, A.ent_cap)) ? ents[e] : Entry{0, 0.f, 0.f, 0.f};
@@ -1304,20 +1359,60 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
     b2 = __builtin_amdgcn_readfirstlane(b2);
     PairAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     int nrec = 0, npair = 0;
-    if (n0) pair_term<P, GENERIC, TWO, VO, 0>(A, sjw[jj], pi, ii, jj, 0, ai, saw, smw, item, nrec, npair, acc);
-    pair_groups<P, GENERIC, TWO, VO, 1>(A, sjw, b1, b1 + n1, pi, ii, jj, ai, saw, smw, item, nrec, npair, acc);
+    if (n0) pair_term<P, GENERIC, TWO, VO, 0>(A, L.sj[jj], pi, ii, jj, 0, ai, L, item, nrec, npair, acc);
+    pair_groups<P, GENERIC, TWO, VO, 1>(A, b1, b1 + n1, pi, ii, jj, ai, L, item, nrec, npair, acc);
     if constexpr (!VO)
-      pair_groups<P, GENERIC, TWO, VO, 2>(A, sjw, b2, b2 + n2, pi, ii, jj, ai, saw, smw, item, nrec, npair, acc);
-    if (nrec > kBandSlots) {
-      // redone whole by the band kernel, which writes all of the item's outputs
-      if (lane == 0) A.redo[atomicAdd(A.counts, 1)] = item;
-    } else {
-      if (nrec > 0 && lane == 0) A.fix[atomicAdd(A.counts + 1, 1)] = make_int2(item, nrec);
-      pair_item_out<VO>(A, acc, npair, item, ii, jj, fpart, partial);
+      pair_groups<P, GENERIC, TWO, VO, 2>(A, b2, b2 + n2, pi, ii, jj, ai, L, item, nrec, npair, acc);
+    if (nrec > 0 && lane == 0) {
+      if (nrec > kBandSlots) A.redo[atomicAdd(A.counts, 1)] = item;
+      else A.fix[atomicAdd(A.counts + 1, 1)] = make_int2(item, nrec);
+      if (!A.item_partials) atomicOr(&A.bandmap[item >> 5], 1u << (item & 31));
+    }
+    // an item redone whole gets all of its outputs from the band kernel
+    if (nrec <= kBandSlots) {
+      if constexpr (!VO) {
+        const float fx = sum_over_jj(acc.fx), fy = sum_over_jj(acc.fy), fz = sum_over_jj(acc.fz);
+        if (jj == 0) {
+          double* fp = fpart + (size_t)item * (CL * 3) + ii * 3;
+          fp[0] = -A.df_scale * (double)fx;
+          fp[1] = -A.df_scale * (double)fy;
+          fp[2] = -A.df_scale * (double)fz;
+        }
+      }
+      float tv[7];
+      tv[0] = wave_sum_f(acc.sv);
+      if constexpr (!VO) {
+        tv[1] = wave_sum_f(acc.vxx);
+        tv[2] = wave_sum_f(acc.vxy);
+        tv[3] = wave_sum_f(acc.vxz);
+        tv[4] = wave_sum_f(acc.vyy);
+        tv[5] = wave_sum_f(acc.vyz);
+        tv[6] = wave_sum_f(acc.vzz);
+      } else {
+        tv[1] = tv[2] = tv[3] = tv[4] = tv[5] = tv[6] = 0.f;
+      }
+      if (lane < 8) {
+        // lane q forms row entry q: value, six virial products, pair count
+        float tq = tv[0];
+        for (int q = 1; q < 7; q++)
+          if (lane == q) tq = tv[q];
+        const double o = lane == 0 ? A.sv_scale * (double)tq + A.sv_shift * (double)npair
+                                   : (lane < 7 ? A.df_scale * (double)tq : (double)npair);
+        if (A.item_partials || nrec > 0) A.ipart[(size_t)item * 8 + lane] = o;
+        else stot[w][lane] += o;
+      }
     }
     // the next item overwrites the staging read above
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
+  }
+  if (!A.item_partials) {
+    __syncthreads();
+    if (w == 0 && lane < 8) {
+      double s = 0.0;
+      for (int v = 0; v < NW; v++) s += stot[v][lane];
+      partial[(size_t)blockIdx.x * 8 + lane] = s;
+    }
   }
 }
 
@@ -1325,21 +1420,20 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
 // (1) Fix-up: one wave per item of A.fix.  Lane r < nrec takes band record r
 //     (program order of the FP32 pass), rebuilds the pair's FP32 r^2 exactly
 //     as the staging did, decides the cut on the reference's FP64
-//     Pbc::distance (exact_r2) and forms the pair's FP32 terms; lane 0 adds
-//     them to the item's outputs in record order.
+//     Pbc::distance (exact_r2) and forms the pair's FP32 terms; the records'
+//     terms are added to the item's outputs in record order.
 // (2) Redo: the items of A.redo (more than kBandSlots band pairs), each
 //     shared by the four waves of a workgroup: the FP32 pass's staging and
 //     evaluation with the FP64 cut decision inline; the waves' partials are
-//     summed in wave order.  It needs ~50 more VGPRs than the FP32 pass,
-//     which is why it is a separate kernel.
+//     summed in wave order.  Its FP64 code needs ~50 more VGPRs than the FP32
+//     pass, which is why it is a separate kernel.
 template <int P, bool GENERIC, bool TWO, bool VO, int CH>
 __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(1))) void k_pairs_band(
     PairArgs A, const int4* __restrict__ items, const Entry* __restrict__ ents, const float4* __restrict__ posI,
     const float4* __restrict__ posJ, const float4* __restrict__ bbI, const float4* __restrict__ bbJ,
-    const int* __restrict__ absI, const int* __restrict__ absJ, const double* __restrict__ pos64,
-    double* __restrict__ fpart, double* __restrict__ partial) {
+    const int* __restrict__ absI, const int* __restrict__ absJ, double* __restrict__ fpart) {
   constexpr int NW = PAIR_BLOCK / WAVE;
-  constexpr int CAP = CH * CL + 4 * CL;
+  constexpr int CAP = stage_cap<CH>();
   __shared__ P3 sj[NW][CAP];
   __shared__ short sm[NW][CAP];
   __shared__ int scode[NW][CH];
@@ -1374,72 +1468,50 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(1)))
       d[1] = qy - pi.y;
       d[2] = qz - pi.z;
       const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
-      const bool cut = switch_inside(A.sw, exact_r2(pbc, pos64, __float_as_int(pi.w), __float_as_int(v.w)));
-      if (cut) {
+      if (switch_inside(A.sw, exact_r2(pbc, A.pos64, __float_as_int(pi.w), __float_as_int(v.w)))) {
         float dfu;
-        if constexpr (GENERIC) {
-          double dfd;
-          sv = (float)switch_calc_sqr(A.sw, (double)r2, dfd);
-          dfu = (float)dfd;
-        } else {
-          sw_fixed<P>(A.invr0_2, r2, sv, dfu);
-        }
+        pair_switch<P, GENERIC>(A, r2, sv, dfu);
         if (!VO) {
           g[0] = dfu * d[0];
           g[1] = dfu * d[1];
           g[2] = dfu * d[2];
         }
+        // the slot of a self pair is its staged index
         cnt = (kind == 1 || (kind == 0 && t > rii)) ? 1 : 0;
+      } else {
+        sv = 0.f;
       }
     }
-    // lane 0 adds the records in order
-    double add[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    double fadd[CL][3];
-#pragma unroll
-    for (int a = 0; a < CL; a++) fadd[a][0] = fadd[a][1] = fadd[a][2] = 0.0;
-    unsigned fmask = 0;
+    // the records' terms in record order, each output by one lane
+    double add = 0.0;
     for (int r = 0; r < nr; r++) {
-      const int ri = __shfl(rii, r, 64);
-      const int rc = __shfl(cnt, r, 64);
-      const float rs = __shfl(sv, r, 64);
+      const int ri = __builtin_amdgcn_readlane(rii, r);
+      const int rc = __builtin_amdgcn_readlane(cnt, r);
+      const float rs = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sv), r));
       float rg[3], rd[3];
       for (int c = 0; c < 3; c++) {
-        rg[c] = __shfl(g[c], r, 64);
-        rd[c] = __shfl(d[c], r, 64);
+        rg[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g[c]), r));
+        rd[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d[c]), r));
       }
-      if (!VO) {
-        for (int a = 0; a < CL; a++)
-          if (a == ri)
-            for (int c = 0; c < 3; c++) fadd[a][c] += -A.df_scale * (double)rg[c];
-        fmask |= 1u << ri;
-      }
-      if (rc) {
-        add[0] += A.sv_scale * (double)rs + A.sv_shift;
-        add[1] += A.df_scale * (double)(rg[0] * rd[0]);
-        add[2] += A.df_scale * (double)(rg[0] * rd[1]);
-        add[3] += A.df_scale * (double)(rg[0] * rd[2]);
-        add[4] += A.df_scale * (double)(rg[1] * rd[1]);
-        add[5] += A.df_scale * (double)(rg[1] * rd[2]);
-        add[6] += A.df_scale * (double)(rg[2] * rd[2]);
-        add[7] += 1.0;
+      if (!VO && lane < 3 * CL && lane / 3 == ri) add += -A.df_scale * (double)rg[lane % 3];
+      if (rc && lane >= 3 * CL && lane < 3 * CL + 8) {
+        const int q2 = lane - 3 * CL;
+        if (q2 == 0) add += A.sv_scale * (double)rs + A.sv_shift;
+        else if (q2 == 7) add += 1.0;
+        else if (!VO) {
+          const int a = q2 <= 3 ? 0 : (q2 <= 5 ? 1 : 2);
+          const int b = q2 <= 3 ? q2 - 1 : (q2 <= 5 ? q2 - 3 : 2);
+          add += A.df_scale * (double)(rg[a] * rd[b]);
+        }
       }
     }
-    if (lane == 0) {
-      double* out = partial + (size_t)item * 8;
-      for (int c = 0; c < 8; c++) out[c] += add[c];
-      if (!VO)
-        for (int a = 0; a < CL; a++)
-          if ((fmask >> a) & 1)
-            for (int c = 0; c < 3; c++) fpart[(size_t)item * (CL * 3) + a * 3 + c] += fadd[a][c];
-    }
+    if (!VO && lane < 3 * CL) fpart[(size_t)item * (CL * 3) + lane] += add;
+    if (lane >= 3 * CL && lane < 3 * CL + 8) A.ipart[(size_t)item * 8 + lane - 3 * CL] += add;
   }
 
   // (2) redo: the waves of a workgroup share one item (uniform over the workgroup)
   const int nredo = A.counts[0];
-  P3* sjw = sj[w];
-  short* smw = sm[w];
-  int* scw = scode[w];
-  int* saw = TWO ? sabs[w] : nullptr;
+  const WaveLds L = {sj[w], sm[w], scode[w], TWO ? sabs[w] : nullptr};
   for (int k = blockIdx.x; k < nredo; k += gridDim.x) {
     const int item = A.redo[k];
     const int4 it = items[item];
@@ -1453,23 +1525,22 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(1)))
     Entry my_e = {0, 0.f, 0.f, 0.f};
     if (lane < CH && it.y + lane < e1) my_e = ents[it.y + lane];
     int n0, b1, n1, b2, n2;
-    stage_item<TWO, VO, CH>(A, my_e, lane, it.y, e1, ic, ih, posJ, bbJ, absJ, sjw, smw, scw, saw, n0, b1, n1, b2, n2);
+    stage_item<TWO, VO, CH>(A, my_e, lane, it.y, e1, ic, ih, posJ, bbJ, absJ, L, n0, b1, n1, b2, n2);
     PairAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     int npair = 0;
     // every wave staged the whole item; wave w evaluates groups w, w + NW, ...
     if (n0 && w == 0)
-      pair_term_exact<P, GENERIC, TWO, VO, 0>(A, sjw[jj], pi, ii, jj, 0, reqi, ai, saw, smw, scw, posJ, pbc, pos64,
-                                              npair, acc);
+      pair_term_exact<P, GENERIC, TWO, VO, 0>(A, L.sj[jj], pi, ii, jj, 0, reqi, ai, L, posJ, pbc, npair, acc);
     int gi = n0 ? 1 : 0;
     for (int g = b1; g < b1 + n1; g += CL, gi++)
       if (gi % NW == w)
-        pair_term_exact<P, GENERIC, TWO, VO, 1>(A, sjw[g + jj], pi, ii, jj, g, reqi, ai, saw, smw, scw, posJ, pbc,
-                                                pos64, npair, acc);
+        pair_term_exact<P, GENERIC, TWO, VO, 1>(A, L.sj[g + jj], pi, ii, jj, g, reqi, ai, L, posJ, pbc, npair,
+                                                acc);
     if constexpr (!VO)
       for (int g = b2; g < b2 + n2; g += CL, gi++)
         if (gi % NW == w)
-          pair_term_exact<P, GENERIC, TWO, VO, 2>(A, sjw[g + jj], pi, ii, jj, g, reqi, ai, saw, smw, scw, posJ,
-                                                  pbc, pos64, npair, acc);
+          pair_term_exact<P, GENERIC, TWO, VO, 2>(A, L.sj[g + jj], pi, ii, jj, g, reqi, ai, L, posJ, pbc, npair,
+                                                  acc);
     double* red = sred[w];
     if constexpr (!VO) {
       const float fx = sum_over_jj(acc.fx), fy = sum_over_jj(acc.fy), fz = sum_over_jj(acc.fz);
@@ -1513,7 +1584,7 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(1)))
         } else if (lane < 7) {
           o = A.df_scale * s;
         }
-        partial[(size_t)item * 8 + lane] = o;
+        A.ipart[(size_t)item * 8 + lane] = o;
       }
     }
     // the next item overwrites the staging and sred
@@ -1529,44 +1600,118 @@ __device__ __forceinline__ void write_virial(double* virial, const double* s6) {
   virial[6] = xz; virial[7] = yz; virial[8] = zz;
 }
 
-constexpr int kRedBlocks = 128;
+// Inputs of the final value / virial / pair-count sum: the workgroup rows of
+// the pair kernels, then, per list direction, the rows of the items with band
+// pairs (their bits in bandmap, in item order).
+struct RedArgs {
+  int nred;
+  const double* partial;
+  int ndir;
+  const unsigned* bandmap[2];
+  int nwords[2];
+  const double* ipart[2];
+  double* value;
+  double* virial;
+  double* stats;
+};
 
-// After the pair passes, in one launch: workgroups [0, ngather) form the
-// derivatives of the atoms of every i cluster (fixed-order sums of the
-// cluster's item partials); the kRedBlocks workgroups after them sum
-// contiguous ranges of the item partials (value, virial, pair count) for
-// k_finalize.  Two roles in one grid save a kernel boundary.
+// After the pair kernels, in one launch.  With do_red, workgroup 0 forms the
+// value, virial and pair count from the rows of R in a fixed order (each
+// thread a fixed set of rows, then a fixed tree); the other ngather
+// workgroups form the derivatives of the atoms of every i cluster (fixed-order
+// sums of the cluster's item partials).  The reduction workgroup comes first
+// so that it starts with the first wave of workgroups.
 __global__ __launch_bounds__(256) void k_gather_reduce(int ngather, int ncl, const int* __restrict__ ncl_dev,
                                                        const float4* __restrict__ p,
                                                        const int* __restrict__ item_start, int items_cap,
                                                        const double* __restrict__ fpart, double* __restrict__ deriv,
-                                                       int nitems, const double* __restrict__ partial,
-                                                       double* __restrict__ out) {
+                                                       int do_red, RedArgs R) {
   const int t = threadIdx.x;
-  if ((int)blockIdx.x < ngather) {
+  const int red = do_red ? 1 : 0;
+  if ((int)blockIdx.x >= red) {
     if (!deriv) return;
-    const int g = blockIdx.x * blockDim.x + t;
+    const int g = (blockIdx.x - red) * blockDim.x + t;
     if (ncl_dev) ncl = *ncl_dev;
     if (g >= ncl * CL * 3) return;
     const int c = g / (CL * 3), r = g % (CL * 3);
+    // every load that depends only on c goes out before the first wait
+    const int i0 = item_start[c];
+    const int i1 = min(item_start[c + 1], items_cap);
     const int req = __float_as_int(p[c * CL + r / 3].w);
     if (req < 0) return;
     double sum = 0.0;
-    const int i1 = min(item_start[c + 1], items_cap);
-    for (int it = item_start[c]; it < i1; it++) sum += fpart[(size_t)it * (CL * 3) + r];
+    for (int it = i0; it < i1; it += 4) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) v[u] = it + u < i1 ? fpart[(size_t)(it + u) * (CL * 3) + r] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 4; u++) sum += v[u];
+    }
     deriv[3 * (size_t)req + r % 3] = sum;
     return;
   }
   __shared__ double s[8][256];
-  const int blk = blockIdx.x - ngather, nblk = gridDim.x - ngather;
-  const int per = (nitems + nblk - 1) / nblk;
-  const int b = blk * per, e = min(nitems, b + per);
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const double4* p4 = reinterpret_cast<const double4*>(partial);
-  for (int i = b + t; i < e; i += 256) {
-    double4 x = p4[2 * (size_t)i], y = p4[2 * (size_t)i + 1];
-    acc[0] += x.x; acc[1] += x.y; acc[2] += x.z; acc[3] += x.w;
-    acc[4] += y.x; acc[5] += y.y; acc[6] += y.z; acc[7] += y.w;
+  // the first band bitmap words go out together with the first rows, so that
+  // the workgroup waits on memory about twice in all (rows and words, then the
+  // rows of band items, which are few)
+  constexpr int kPre = 4;
+  unsigned wpre[2][kPre];
+#pragma unroll
+  for (int d = 0; d < 2; d++)
+#pragma unroll
+    for (int u = 0; u < kPre; u++) {
+      const int w = t + u * 256;
+      wpre[d][u] = (d < R.ndir && w < R.nwords[d]) ? R.bandmap[d][w] : 0u;
+    }
+  const double4* p4 = reinterpret_cast<const double4*>(R.partial);
+  // rows t, t + 256, t + 512, ... of each thread, eight in flight at once
+  for (int i0 = t; i0 < R.nred; i0 += 256 * 8) {
+    double4 x[8], y[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int i = i0 + u * 256;
+      if (i < R.nred) {
+        x[u] = p4[2 * (size_t)i];
+        y[u] = p4[2 * (size_t)i + 1];
+      } else {
+        x[u] = make_double4(0.0, 0.0, 0.0, 0.0);
+        y[u] = x[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      acc[0] += x[u].x; acc[1] += x[u].y; acc[2] += x[u].z; acc[3] += x[u].w;
+      acc[4] += y[u].x; acc[5] += y[u].y; acc[6] += y[u].z; acc[7] += y[u].w;
+    }
+  }
+  // items with band pairs, in item order per thread (words t, t + 256, ...)
+#pragma unroll
+  for (int d = 0; d < 2; d++) {
+    if (d >= R.ndir) break;
+    const unsigned* bm = R.bandmap[d];
+    const double* ip = R.ipart[d];
+    const int nw = R.nwords[d];
+    for (int w0 = t; w0 < nw; w0 += 256 * kPre) {
+      unsigned b[kPre];
+#pragma unroll
+      for (int u = 0; u < kPre; u++) {
+        const int w = w0 + u * 256;
+        b[u] = w0 == t ? wpre[d][u] : (w < nw ? bm[w] : 0u);
+      }
+#pragma unroll
+      for (int u = 0; u < kPre; u++) {
+        unsigned bits = b[u];
+        while (bits) {
+          const int item = (w0 + u * 256) * 32 + __builtin_ctz(bits);
+          bits &= bits - 1;
+          const double4* row = reinterpret_cast<const double4*>(ip + (size_t)item * 8);
+          const double4 x = row[0], y = row[1];
+          acc[0] += x.x; acc[1] += x.y; acc[2] += x.z; acc[3] += x.w;
+          acc[4] += y.x; acc[5] += y.y; acc[6] += y.z; acc[7] += y.w;
+        }
+      }
+    }
   }
   for (int q = 0; q < 8; q++) s[q][t] = acc[q];
   __syncthreads();
@@ -1575,7 +1720,17 @@ __global__ __launch_bounds__(256) void k_gather_reduce(int ngather, int ncl, con
       for (int q = 0; q < 8; q++) s[q][t] += s[q][t + o];
     __syncthreads();
   }
-  if (t < 8) out[(size_t)blk * 8 + t] = s[t][0];
+  if (t == 0) {
+    if (R.value) *R.value = s[0][0];
+    if (R.virial) {
+      double v6[6] = {s[1][0], s[2][0], s[3][0], s[4][0], s[5][0], s[6][0]};
+      write_virial(R.virial, v6);
+    }
+    if (R.stats) {
+      R.stats[0] = s[7][0];
+      R.stats[1] += s[7][0];  // running total since the last reset
+    }
+  }
 }
 
 __global__ void k_finalize(int nitems, const double* __restrict__ partial, double* __restrict__ value,
@@ -1640,10 +1795,15 @@ __global__ __launch_bounds__(256) void k_sys_reduce(int nsys, const int* __restr
 // FP64 fallback kernels
 // ------------------------------------------------------------------------
 // Every requested atom accumulates its own derivative from all its partners
-// (no atomics); the value and the virial are taken once per pair.
+// (no atomics); the value and the virial are taken once per pair.  The
+// partners are every atom of the other group (two groups) or of the group,
+// or, with a neighbour list (nbr_start non-null), the atoms nbr[nbr_start[i]
+// .. nbr_start[i+1]) listed for atom i in both directions.
 __global__ void k_allpairs(int nA, int nB, int use_pbc, PbcArg P, p2a_switch sw, const double* __restrict__ pos,
                            const long long* __restrict__ absidx, double* __restrict__ deriv,
-                           double* __restrict__ partial) {
+                           double* __restrict__ partial, const int* __restrict__ nbr_start = nullptr,
+                           const int* __restrict__ nbr = nullptr) {
+#pragma clang fp contract(off)
   const int natoms = nA + nB;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double cv = 0, v[6] = {0, 0, 0, 0, 0, 0}, f[3] = {0, 0, 0};
@@ -1651,29 +1811,39 @@ __global__ void k_allpairs(int nA, int nB, int use_pbc, PbcArg P, p2a_switch sw,
   if (i < natoms) {
     const bool two = nB > 0;
     int j0, j1;
-    if (two) {
+    if (nbr_start) {
+      j0 = nbr_start[i];
+      j1 = nbr_start[i + 1];
+    } else if (two) {
       if (i < nA) { j0 = nA; j1 = natoms; } else { j0 = 0; j1 = nA; }
     } else {
       j0 = 0;
       j1 = nA;
     }
     const double* xi = pos + 3 * (size_t)i;
-    for (int j = j0; j < j1; j++) {
+    for (int jj = j0; jj < j1; jj++) {
+      const int j = nbr_start ? nbr[jj] : jj;
       // absidx null: the requested atoms are distinct atoms
       if (j == i || (absidx && absidx[i] == absidx[j])) continue;
       const double* xj = pos + 3 * (size_t)j;
-      // pair (p,q) is oriented p -> q with p in GROUPA (two groups) or p<q
-      bool own = two ? (i < nA) : (i < j);
+      // the pair is measured once, as the reference lists it: p -> q with p
+      // in GROUPA (two groups) or p < q, d = Pbc::distance(x_p, x_q); atom
+      // i is p ("own") or q.  Measured from q, a pair at half a cell could
+      // take another of its tied images.
+      const bool own = two ? (i < nA) : (i < j);
+      const double* xp = own ? xi : xj;
+      const double* xq = own ? xj : xi;
       double d[3];
-      if (use_pbc) pbc_distance(P, xi, xj, d);
-      else { d[0] = xj[0] - xi[0]; d[1] = xj[1] - xi[1]; d[2] = xj[2] - xi[2]; }
+      if (use_pbc) pbc_distance(P, xp, xq, d);
+      else { d[0] = xq[0] - xp[0]; d[1] = xq[1] - xp[1]; d[2] = xq[2] - xp[2]; }
       double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
       double df;
       double s = switch_calc_sqr(sw, r2, df);
-      // deriv[i] -= df*d for d = x_j - x_i, the same for either orientation
-      f[0] -= df * d[0];
-      f[1] -= df * d[1];
-      f[2] -= df * d[2];
+      // CoordinationBase: deriv[p] -= df*d, deriv[q] += df*d
+      const double g = own ? -df : df;
+      f[0] += g * d[0];
+      f[1] += g * d[1];
+      f[2] += g * d[2];
       if (own) {
         cv += s;
         if (switch_inside(sw, r2)) np += 1;
@@ -1707,9 +1877,11 @@ __global__ void k_allpairs(int nA, int nB, int use_pbc, PbcArg P, p2a_switch sw,
 }
 
 // PAIR: the k-th atom of GROUPA with the k-th of GROUPB
+// (with a neighbour list, only the pairs k whose nl_keep[k] is set)
 __global__ void k_pairmode(int nA, int use_pbc, PbcArg P, p2a_switch sw, const double* __restrict__ pos,
                            const long long* __restrict__ absidx, double* __restrict__ deriv,
-                           double* __restrict__ partial) {
+                           double* __restrict__ partial, const int* __restrict__ nl_keep = nullptr) {
+#pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (i < nA) {
@@ -1718,7 +1890,7 @@ __global__ void k_pairmode(int nA, int use_pbc, PbcArg P, p2a_switch sw, const d
     double* dj = deriv + 3 * (size_t)j;
     di[0] = di[1] = di[2] = 0.0;
     dj[0] = dj[1] = dj[2] = 0.0;
-    if (absidx[i] != absidx[j]) {
+    if (absidx[i] != absidx[j] && (!nl_keep || nl_keep[i])) {
       const double* xi = pos + 3 * (size_t)i;
       const double* xj = pos + 3 * (size_t)j;
       double d[3];
@@ -1753,6 +1925,148 @@ __global__ void k_pairmode(int nA, int use_pbc, PbcArg P, p2a_switch sw, const d
       for (int w = 0; w < (int)(blockDim.x / 64); w++) t += red[w][q];
       partial[(size_t)blockIdx.x * 8 + q] = t;
     }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Neighbour lists of the FP64 path, with the candidate sets of the
+// reference (NeighborList::update, NeighborList.cpp:177-308), stored as a
+// symmetric CSR: atom i lists every partner j of a listed pair (i,j) or (j,i)
+// ------------------------------------------------------------------------
+
+// LinkCells grid of the cells branch: cell of an atom from its scaled
+// coordinates (LinkCells::findMyCell, LinkCells.cpp:277-292); `origin` is
+// subtracted first when the grid box was built around the atoms (no cell
+// given, LinkCells.cpp:49-73)
+struct CellGrid {
+  double inv[9];  // inverse of the grid box (rows), as Pbc::setBox forms it
+  double origin[3];
+  int shift_origin;
+  int nc[3];
+  int use_pbc;  // stencil wraps around (LinkCells::addRequiredCells)
+};
+
+#pragma clang fp contract(off)
+__device__ __forceinline__ void grid_cell(const CellGrid& G, const double* x, int* c) {
+  double m[3] = {x[0], x[1], x[2]};
+  if (G.shift_origin)
+    for (int k = 0; k < 3; k++) m[k] = m[k] - G.origin[k];
+  for (int j = 0; j < 3; j++) {
+    // Pbc::realToScaled: matmul(transpose(invBox), d)
+    const double f = m[0] * G.inv[j] + m[1] * G.inv[3 + j] + m[2] * G.inv[6 + j];
+    int cj = (int)floor((pbc1(f) + 0.5) * G.nc[j]);
+    c[j] = min(max(cj, 0), G.nc[j] - 1);
+  }
+}
+#pragma clang fp contract(fast)
+
+// cells a and b are within one stencil (LinkCells::addRequiredCells,
+// LinkCells.cpp:183-239: cells c-1..c+1 along each direction, wrapped with
+// pbc, every cell when there are fewer than three; without pbc, clipped)
+__device__ __forceinline__ bool grid_adjacent(const CellGrid& G, const int* a, const int* b) {
+  for (int k = 0; k < 3; k++) {
+    const int n = G.nc[k];
+    if (G.use_pbc) {
+      if (n >= 3) {
+        const int d = (b[k] - a[k] + n) % n;
+        if (d != 0 && d != 1 && d != n - 1) return false;
+      }
+    } else if (abs(a[k] - b[k]) > 1) {
+      return false;
+    }
+  }
+  return true;
+}
+
+// pair (p,q) as the reference lists it: GROUPA atom first with two groups,
+// the lower index first in one group
+__device__ __forceinline__ bool nl_member(int mode, int p, int q, int use_pbc, const PbcArg& P, double cut2,
+                                          const CellGrid& G, const double* __restrict__ pos,
+                                          const int* __restrict__ cellof) {
+#pragma clang fp contract(off)
+  if (mode == P2A_NL_CELLS) {
+    const int a[3] = {cellof[3 * p], cellof[3 * p + 1], cellof[3 * p + 2]};
+    const int b[3] = {cellof[3 * q], cellof[3 * q + 1], cellof[3 * q + 2]};
+    return grid_adjacent(G, a, b);
+  }
+  const double* xa = pos + 3 * (size_t)p;
+  const double* xb = pos + 3 * (size_t)q;
+  double d[3];
+  if (use_pbc) pbc_distance(P, xa, xb, d);
+  else { d[0] = xb[0] - xa[0]; d[1] = xb[1] - xa[1]; d[2] = xb[2] - xa[2]; }
+  return d[0] * d[0] + d[1] * d[1] + d[2] * d[2] <= cut2;
+}
+
+__global__ void k_nl_cells(int natoms, CellGrid G, const double* __restrict__ pos, int* __restrict__ cellof) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= natoms) return;
+  int c[3];
+  grid_cell(G, pos + 3 * (size_t)i, c);
+  cellof[3 * i] = c[0];
+  cellof[3 * i + 1] = c[1];
+  cellof[3 * i + 2] = c[2];
+}
+
+// FILL = 0: count[i] = listed partners of atom i; FILL = 1: write them, in
+// index order, from start[i]
+template <int FILL>
+__global__ void k_nl_atoms(int nA, int nB, int mode, int use_pbc, PbcArg P, double cut2, CellGrid G,
+                           const double* __restrict__ pos, const int* __restrict__ cellof,
+                           int* __restrict__ count, const int* __restrict__ start, int* __restrict__ nbr) {
+  const int natoms = nA + nB;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= natoms) return;
+  const bool two = nB > 0;
+  const int j0 = two ? (i < nA ? nA : 0) : 0;
+  const int j1 = two ? (i < nA ? natoms : nA) : nA;
+  int n = 0;
+  const int base = FILL ? start[i] : 0;
+  for (int j = j0; j < j1; j++) {
+    if (!two && j == i) continue;
+    const int p = two ? (i < nA ? i : j) : min(i, j);
+    const int q = two ? (i < nA ? j : i) : max(i, j);
+    if (nl_member(mode, p, q, use_pbc, P, cut2, G, pos, cellof)) {
+      if (FILL) nbr[base + n] = j;
+      n++;
+    }
+  }
+  if (!FILL) count[i] = n;
+}
+
+// PAIR with a list: keep[k] = pair k within NL_CUTOFF (classic branch)
+__global__ void k_nl_pairs_keep(int nA, int use_pbc, PbcArg P, double cut2, const double* __restrict__ pos,
+                                int* __restrict__ keep) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nA) return;
+  keep[i] = nl_member(P2A_NL_CLASSIC, i, nA + i, use_pbc, P, cut2, CellGrid{}, pos, nullptr) ? 1 : 0;
+}
+
+// exclusive scan of n counts in one workgroup of 1024 threads (contiguous
+// chunks per thread); start[n] = total
+__global__ __launch_bounds__(1024) void k_scan_small(int n, const int* __restrict__ count, int* __restrict__ start,
+                                                     long long* __restrict__ total) {
+  __shared__ long long part[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int a = min(n, t * per), b = min(n, a + per);
+  long long sum = 0;
+  for (int k = a; k < b; k++) sum += count[k];
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    long long v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  long long run = part[t] - sum;
+  for (int k = a; k < b; k++) {
+    start[k] = (int)run;
+    run += count[k];
+  }
+  if (t == 1023) {
+    start[n] = (int)part[1023];
+    *total = part[1023];
   }
 }
 
@@ -1809,27 +2123,28 @@ inline void sys_columns(const double* box9, int n, SysArg& S) {
 // the chunk size (entries per work item) and the grid of the pair kernel
 struct PairTuning {
   int chunk = 64;        // 32 or 64
-  int blocks_per_cu = 0; // 0: one wave per item; > 0: persistent grid of that many blocks per CU
+  // 0: one wave per item; > 0: persistent grid of that many blocks per CU.
+  // The persistent grid also bounds the workgroup rows of the final sum.
+  int blocks_per_cu = 8;
 };
 
-// workgroups per CU of the band kernel (its register use allows 3)
-constexpr int kExactBlocksPerCu = 3;
+// workgroups of the band kernel: it loops over the few items the FP32 pass
+// queues (one wave per fixed-up item, one workgroup per item redone whole)
+constexpr int kBandGrid = 64;
 
 // launches of the pair kernels for a switch, group mode and tuning: the FP32
 // pass over all items, then the band kernel over the items it queued
-// (A.counts / A.redo / A.fix / A.recs must be set; exact_grid workgroups
-// loop over them)
-inline void launch_pairs(const PairArgs& A, int grid, int exact_grid, hipStream_t s, bool value_only, int chunk,
-                         const int4* items, const Entry* ents, const float4* posI,
-                         const float4* posJ, const float4* bbI, const float4* bbJ, const int* absI, const int* absJ,
-                         const double* pos64, double* fpart, double* partial) {
+// (A.counts / A.redo / A.fix / A.recs / A.ipart / A.bandmap must be set)
+inline void launch_pairs(const PairArgs& A, int grid, hipStream_t s, bool value_only, int chunk, const int4* items,
+                         const Entry* ents, const float4* posI, const float4* posJ, const float4* bbI,
+                         const float4* bbJ, const int* absI, const int* absJ, double* fpart, double* partial) {
   const p2a_switch& sw = A.sw;
 #define P2A_K(P, GEN, TWO, VO, CH)                                                                                \
   do {                                                                                                            \
     hipLaunchKernelGGL((k_pairs<P, GEN, TWO, VO, CH>), dim3(grid), dim3(PAIR_BLOCK), 0, s, A, items, ents, posI,    \
                        posJ, bbI, bbJ, absI, absJ, fpart, partial);                                              \
-    hipLaunchKernelGGL((k_pairs_band<P, GEN, TWO, VO, CH>), dim3(exact_grid), dim3(PAIR_BLOCK), 0, s, A, items,    \
-                       ents, posI, posJ, bbI, bbJ, absI, absJ, pos64, fpart, partial);                            \
+    hipLaunchKernelGGL((k_pairs_band<P, GEN, TWO, VO, CH>), dim3(kBandGrid), dim3(PAIR_BLOCK), 0, s, A, items,     \
+                       ents, posI, posJ, bbI, bbJ, absI, absJ, fpart);                                           \
   } while (0)
 #define P2A_KCH(P, GEN, TWO, VO)          \
   do {                                    \

@@ -23,6 +23,13 @@
 
 namespace p2a {
 
+// the reference's host build forms every product and sum on its own: no
+// fused multiply-add contraction in this file, so that cut decisions and
+// values round as the reference's do
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+
 P2A_HD double ipow(double base, int exp) {
   // square-and-multiply, the evaluation order of Tools::fastpow (Tools.h:582-597)
   if (exp < 0) { exp = -exp; base = 1.0 / base; }
@@ -211,5 +218,9 @@ P2A_HD bool switch_inside(const p2a_switch& s, double d2) {
   if (s.type == P2A_SW_RATIONAL_FIXED || (s.type == P2A_SW_RATIONAL && s.fast)) return d2 <= s.dmax_2;
   return sqrt(d2) <= s.dmax;
 }
+
+#if defined(__clang__)
+#pragma clang fp contract(fast)
+#endif
 
 }  // namespace p2a

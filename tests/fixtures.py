@@ -77,3 +77,19 @@ def rt_backup_grid_cvs():
             c.append(float(np.sqrt(d @ d)))
         out.append(c)
     return out
+
+
+def read_forces(fname):
+    """plumed driver --dump-forces --dump-full-virial output -> list of
+    (virial[3,3], forces[n,3]) per frame."""
+    with open(fname) as f:
+        lines = [l for l in f.read().splitlines() if l.strip()]
+    out = []
+    i = 0
+    while i < len(lines):
+        n = int(lines[i].split()[0])
+        vir = np.array([float(v) for v in lines[i + 1].split()]).reshape(3, 3)
+        frc = np.array([[float(v) for v in lines[i + 2 + k].split()[1:4]] for k in range(n)])
+        out.append((vir, frc))
+        i += 2 + n
+    return out

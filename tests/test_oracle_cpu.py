@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from fixtures import path, read_colvar, read_deriv, read_xyz
+from fixtures import path, read_colvar, read_deriv, read_forces, read_xyz
 
 needs_ref = pytest.mark.skipif(O.ref() is None, reason="compiled reference subset (oracle/_ref) not built here")
 
@@ -161,6 +161,55 @@ def test_oracle_rt42c_golden():
         pos, box = frames[k]
         v, _, _, _ = O.coordination(sw, box, pos, 108, 0)
         assert abs(v - ref[k, 1]) <= 5.01e-5
+
+
+# regtest/basic/rt42 biases: RESTRAINT SLOPE=1 on s = c + d and SLOPE=0.001 on
+# c - 0.5 d - 0.5 dfast + e - f + g - h.  d and dfast are CUSTOM 1/(1+x^6)
+# (the rational form without D_MAX), f is CUSTOM 1/(1+x^7) with D_0=0.1, and
+# h = exp(-x) equals g, so the total bias is
+# 1.001 c + 0.999 d + 0.001 (e - f).
+RT42_BIAS_TERMS = [(1.001, dict(nn=6, mm=0, r0=1.0)), (0.999, "RATIONAL R_0=1 NN=6 MM=12"),
+                   (0.001, dict(nn=7, mm=0, r0=1.0, d0=0.1)), (-0.001, "RATIONAL R_0=1 NN=7 MM=14 D_0=0.1")]
+
+
+def rt42_forces_virial(coords):
+    """Forces on the 108 atoms and the virial the driver dumps, from a list of
+    (coefficient, (value, deriv[216,3], virial[3,3])) of the four terms."""
+    F = np.zeros((108, 3))
+    W = np.zeros((3, 3))
+    for c, (_, d, w) in coords:
+        F -= c * (d[:108] + d[108:])
+        W -= c * w
+    return F, W
+
+
+def test_oracle_rt42_forces_full_virial():
+    """regtest/basic/rt42 ff.reference (--dump-forces --dump-full-virial,
+    %10.5f): forces -dV/dx and the full virial -dV/ds * (box derivative of s)
+    of every frame, from the oracle's derivatives and virials."""
+    frames = read_xyz(path("rt42", "trajectory.xyz"))
+    ff = read_forces(path("rt42", "ff.reference"))
+    assert len(ff) == len(frames)
+    absidx = np.concatenate([np.arange(108), np.arange(108)])
+    sws = [(c, O.Switch(**a) if isinstance(a, dict) else O.Switch(a)) for c, a in RT42_BIAS_TERMS]
+    for k in range(0, len(frames), 3):
+        pos, box = frames[k]
+        p2 = np.concatenate([pos, pos])
+        terms = [(c, O.coordination(sw, box, p2, 108, 108, absidx=absidx)[:3]) for c, sw in sws]
+        F, W = rt42_forces_virial(terms)
+        vir, frc = ff[k]
+        np.testing.assert_allclose(F, frc, atol=5.01e-6, rtol=0, err_msg=f"forces frame {k}")
+        np.testing.assert_allclose(W, vir, atol=5.01e-6, rtol=0, err_msg=f"virial frame {k}")
+
+
+def test_oracle_rt42_cells_forces_zero():
+    """regtest/basic/rt42-cells ff.reference: the bias 0.001 (d - c) of the
+    NLISTCELLS and NLIST forms puts zero force and virial on every frame."""
+    frames = read_xyz(path("rt42-cells", "trajectory.xyz"))
+    ff = read_forces(path("rt42-cells", "ff.reference"))
+    assert len(ff) == len(frames)
+    for vir, frc in ff:
+        assert np.all(vir == 0) and np.all(frc == 0)
 
 
 def test_oracle_rt42_cells_golden():
