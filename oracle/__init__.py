@@ -419,6 +419,8 @@ def ref():
         R.ref_coordctx_destroy.argtypes = [C.c_void_p]
         R.ref_coordctx_update.argtypes = [C.c_void_p, _dp]
         R.ref_coordctx_update.restype = C.c_int64
+        R.ref_coordctx_reduce.argtypes = [C.c_void_p]
+        R.ref_coordctx_reduce.restype = C.c_int64
         R.ref_coordctx_calc.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
         R.ref_coordctx_calc.restype = C.c_int64
         _REF = R
@@ -507,6 +509,11 @@ class RefCoordination:
 
     def update(self, pos):
         return ref().ref_coordctx_update(self.h, np.ascontiguousarray(pos, dtype=np.float64).reshape(-1))
+
+    def reduce(self):
+        """NeighborList::getReducedAtomList after a rebuild; returns the
+        number of requested atoms."""
+        return ref().ref_coordctx_reduce(self.h)
 
     def calc(self, pos, nthreads=1, want_deriv=False):
         val = C.c_double()

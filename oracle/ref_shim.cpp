@@ -239,6 +239,15 @@ REF_EXPORT int64_t ref_coordctx_update(void* h, const double* pos) {
   return c->nl->size();
 }
 
+// NeighborList::getReducedAtomList (NeighborList.cpp:330-355): the remap of
+// the pair indices onto the sorted list of requested atoms that
+// CoordinationBase::prepare requests after each rebuild; returns the number
+// of requested atoms
+REF_EXPORT int64_t ref_coordctx_reduce(void* h) {
+  auto* c = static_cast<RefCoordCtx*>(h);
+  return (int64_t)c->nl->getReducedAtomList().size();
+}
+
 REF_EXPORT int64_t ref_coordctx_calc(void* h, const double* pos, int nthreads, double* value, double* deriv,
                                      double* virial) {
   auto* c = static_cast<RefCoordCtx*>(h);
