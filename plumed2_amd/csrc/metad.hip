@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -138,23 +139,54 @@ __device__ bool spline(const GridArg& G, const double* __restrict__ val, const d
   return true;
 }
 
-// Gaussian of one hill at the node point xx (MetaD.cpp:2244-2275, diagonal)
-__device__ __forceinline__ double gaussian(const GridArg& G, const double* xx, const double* c, double h,
-                                           double* d) {
-  double dp[kMaxCv], dp2 = 0.0;
-  for (int i = 0; i < G.ncv; i++) {
+// Gaussian of one hill at the node point xx (MetaD.cpp:2244-2275, diagonal),
+// in two parts: the shape (scaled distances dp and exp(-dp^2/2); false
+// beyond the cutoff, where the hill adds nothing), and its value and
+// gradient for the height h.  gaussian() is the two together.
+// NCV > 0: that many CVs, known at compile time (the loops unroll and the
+// small arrays stay in registers); 0: G.ncv.  The arithmetic is the same.
+template <int NCV = 0>
+__device__ __forceinline__ bool gauss_shape(const GridArg& G, const double* xx, const double* c, double* dp,
+                                            double& e) {
+  const int ncv = NCV > 0 ? NCV : G.ncv;
+  double dp2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < (NCV > 0 ? NCV : kMaxCv); i++) {
+    if (i >= ncv) break;
     dp[i] = vdiff(G, i, c[i], xx[i]) * G.invsigma[i];
     dp2 += dp[i] * dp[i];
   }
   dp2 *= 0.5;
-  double bias = 0.0;
-  for (int i = 0; i < G.ncv; i++) d[i] = 0.0;
-  if (dp2 < 6.25) {
-    bias = h * exp(-dp2);
-    for (int i = 0; i < G.ncv; i++) d[i] = -bias * dp[i] * G.invsigma[i] * G.stretchA;
-    bias = G.stretchA * bias + h * G.stretchB;
+  e = 0.0;
+  if (!(dp2 < 6.25)) return false;
+  e = exp(-dp2);
+  return true;
+}
+
+template <int NCV = 0>
+__device__ __forceinline__ double gauss_apply(const GridArg& G, const double* dp, double e, double h, double* d) {
+  const int ncv = NCV > 0 ? NCV : G.ncv;
+  double bias = h * e;
+#pragma unroll
+  for (int i = 0; i < (NCV > 0 ? NCV : kMaxCv); i++) {
+    if (i >= ncv) break;
+    d[i] = -bias * dp[i] * G.invsigma[i] * G.stretchA;
   }
+  bias = G.stretchA * bias + h * G.stretchB;
   return bias;
+}
+
+template <int NCV = 0>
+__device__ __forceinline__ double gaussian(const GridArg& G, const double* xx, const double* c, double h,
+                                           double* d) {
+  double dp[kMaxCv], e;
+#pragma unroll
+  for (int i = 0; i < (NCV > 0 ? NCV : kMaxCv); i++) {
+    if (i >= (NCV > 0 ? NCV : G.ncv)) break;
+    d[i] = 0.0;
+  }
+  if (!gauss_shape<NCV>(G, xx, c, dp, e)) return 0.0;
+  return gauss_apply<NCV>(G, dp, e, h, d);
 }
 
 // node of support offset `s` around the bin `ind` (GridBase::getNeighbors,
@@ -334,12 +366,25 @@ __global__ __launch_bounds__(kSeqThreads) void k_deposit_seq(GridArg G, double* 
 }
 
 // Two non-periodic CVs: the same sequential deposition with the grid around
-// the walk held in LDS.  A tile of kTileX x kTileY nodes (value, two
-// gradient components) serves every hill whose support and spline corners
-// lie inside it; a hill outside writes the tile back and loads the one
-// centred on it.  Every node receives the same FP64 additions in the same
-// order as in k_deposit_seq, so the grids are identical.
-constexpr int kTileX = 64, kTileY = 80;
+// the walk held in LDS (a tile of tw x th <= kTileNodes nodes, shaped like
+// the hill support: value and the two gradient components), in runs of up to kRun consecutive hills that the
+// tile holds (support and upper spline corner):
+//   1. wave 0 decides the run: the current tile when it holds the first
+//      hill, else a tile centred on that hill (the old one is written back);
+//      the run ends before the first hill outside the grid or the tile;
+//   2. the workgroup evaluates the shape (scaled distances, exponential) of
+//      every hill of the run at the spline corners of every later hill;
+//      then wave 0 finds the heights in order, lane 4r+q holding corner q of
+//      hill r: hill k's height comes from its four corners, and every later
+//      corner in hill k's support adds hill k's Gaussian for that height;
+//   3. the whole workgroup adds the run to the tile, one node per thread,
+//      the hills of the run in order.
+// Every node, and every corner read for a height, receives the same FP64
+// additions in the same order as hill-by-hill deposition, so grids and
+// heights are identical to it; the workgroup meets two barriers per run
+// instead of two per hill.
+constexpr int kTileNodes = 5120;
+constexpr int kRun = 16;  // hills per run: 4 corners each on one wave
 
 __device__ __forceinline__ double spline_corner_vals(const GridArg& G, const double* x, const unsigned* ind,
                                                      const unsigned* ni, double grid, const double* der) {
@@ -357,59 +402,110 @@ __device__ __forceinline__ double spline_corner_vals(const GridArg& G, const dou
   return grid * ff;
 }
 
+// value of lane `l` (uniform) of a double, through the scalar unit
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __global__ __launch_bounds__(kSeqThreads) void k_deposit_seq_tile2(GridArg G, double* __restrict__ val,
                                                                    double* __restrict__ der, long long nhills,
                                                                    const double* __restrict__ centers,
                                                                    double* __restrict__ heights,
-                                                                   int* __restrict__ err) {
-  __shared__ double tv[kTileX * kTileY], tdx[kTileX * kTileY], tdy[kTileX * kTileY];
-  __shared__ double sh_h, sh_c[2];
-  __shared__ unsigned sh_ind[2];
-  __shared__ int sh_ok, sh_fits, sh_nox, sh_noy;
+                                                                   int* __restrict__ err, int tw, int th) {
+  __shared__ double tv[kTileNodes], tdx[kTileNodes], tdy[kTileNodes];
+  __shared__ double sh_c[kRun][2], sh_h[kRun];
+  __shared__ int sh_ind[kRun][2];
+  // shape of hill j at corner lane l (4r+q, r > j): e (negative: no
+  // contribution) and the scaled distances
+  __shared__ double sh_e[kRun][4 * kRun], sh_dp[kRun][4 * kRun][2];
+  __shared__ int sh_nb, sh_reload, sh_nox, sh_noy, sh_bx[4];
   const int t = threadIdx.x;
+  const int lane = t & (WAVE_M - 1);
+  const bool wave0 = t < WAVE_M;
   const int nx = (int)G.nbin[0], ny = (int)G.nbin[1];
-  const int tw = min(kTileX, nx), th = min(kTileY, ny);
+  const int nn0 = G.nneigh[0], nn1 = G.nneigh[1];
   int ox = -1, oy = -1;  // tile origin in nodes (-1: no tile loaded)
   auto flush = [&]() {
     if (ox < 0) return;
     for (int j = t; j < tw * th; j += blockDim.x) {
       const int lx = j % tw, ly = j / tw;
       const long long node = (long long)(oy + ly) * nx + (ox + lx);
-      const int l = lx + kTileX * ly;
+      const int l = lx + tw * ly;
       val[node] = tv[l];
       der[2 * node] = tdx[l];
       der[2 * node + 1] = tdy[l];
     }
   };
-  for (long long k = 0; k < nhills; k++) {
-    if (t == 0) {
-      double c[2] = {centers[2 * k], centers[2 * k + 1]};
+  long long k0 = 0;
+  while (k0 < nhills) {
+    // 1. the run
+    if (wave0) {
+      const long long k = k0 + lane;
+      const bool valid = k < nhills;
+      double c[2] = {0.0, 0.0};
       unsigned ind[2] = {0u, 0u};
-      const int ok = grid_index_of(G, c, ind) >= 0;
+      int ok = 0;
+      if (valid) {
+        c[0] = centers[2 * k];
+        c[1] = centers[2 * k + 1];
+        ok = grid_index_of(G, c, ind) >= 0;
+      }
       // nodes the hill touches: support, and the cell's upper spline corner
-      const int xl = max(0, (int)ind[0] - G.nneigh[0]), xh = min(nx - 1, (int)ind[0] + G.nneigh[0] + 1);
-      const int yl = max(0, (int)ind[1] - G.nneigh[1]), yh = min(ny - 1, (int)ind[1] + G.nneigh[1] + 1);
-      sh_fits = ox >= 0 && xl >= ox && xh < ox + tw && yl >= oy && yh < oy + th;
+      const int xl = max(0, (int)ind[0] - nn0), xh = min(nx - 1, (int)ind[0] + nn0 + 1);
+      const int yl = max(0, (int)ind[1] - nn1), yh = min(ny - 1, (int)ind[1] + nn1 + 1);
+      const int xl0 = __builtin_amdgcn_readfirstlane(xl), xh0 = __builtin_amdgcn_readfirstlane(xh);
+      const int yl0 = __builtin_amdgcn_readfirstlane(yl), yh0 = __builtin_amdgcn_readfirstlane(yh);
+      const int i00 = __builtin_amdgcn_readfirstlane((int)ind[0]);
+      const int i01 = __builtin_amdgcn_readfirstlane((int)ind[1]);
+      const bool fit0 = ox >= 0 && xl0 >= ox && xh0 < ox + tw && yl0 >= oy && yh0 < oy + th;
       // a tile centred on the hill holds it whenever nneigh + 2 <= tile / 2,
       // which the launch guarantees; checked anyway before any LDS index
-      const int nox = max(0, min(nx - tw, (int)ind[0] - tw / 2));
-      const int noy = max(0, min(ny - th, (int)ind[1] - th / 2));
-      const bool fits_new = xl >= nox && xh < nox + tw && yl >= noy && yh < noy + th;
-      sh_nox = nox;
-      sh_noy = noy;
-      sh_ok = ok && (sh_fits || fits_new);
-      sh_ind[0] = ind[0];
-      sh_ind[1] = ind[1];
-      sh_c[0] = c[0];
-      sh_c[1] = c[1];
-      if (!sh_ok) atomicCAS(err, 0, (int)(1 + (k < 0x3fffffff ? k : 0x3fffffff)));
+      const int nox = max(0, min(nx - tw, i00 - tw / 2));
+      const int noy = max(0, min(ny - th, i01 - th / 2));
+      const int bx = fit0 ? ox : nox, by = fit0 ? oy : noy;
+      const bool fits = lane < kRun && valid && ok && xl >= bx && xh < bx + tw && yl >= by && yh < by + th;
+      const unsigned long long bad = __ballot(!fits);
+      const int nb = min(kRun, __ffsll((long long)bad) - 1);
+      if (lane < nb) {
+        sh_c[lane][0] = c[0];
+        sh_c[lane][1] = c[1];
+        sh_ind[lane][0] = (int)ind[0];
+        sh_ind[lane][1] = (int)ind[1];
+      }
+      // nodes the run deposits on: ind -/+ nneigh, inside the grid
+      int dxl = lane < nb ? max(0, (int)ind[0] - nn0) : INT_MAX;
+      int dxh = lane < nb ? min(nx - 1, (int)ind[0] + nn0) : -1;
+      int dyl = lane < nb ? max(0, (int)ind[1] - nn1) : INT_MAX;
+      int dyh = lane < nb ? min(ny - 1, (int)ind[1] + nn1) : -1;
+      for (int o = 1; o < kRun; o++) {
+        dxl = min(dxl, __builtin_amdgcn_readlane(dxl, o));
+        dxh = max(dxh, __builtin_amdgcn_readlane(dxh, o));
+        dyl = min(dyl, __builtin_amdgcn_readlane(dyl, o));
+        dyh = max(dyh, __builtin_amdgcn_readlane(dyh, o));
+      }
+      if (lane == 0) {
+        sh_nb = nb;
+        sh_reload = !fit0;
+        sh_nox = nox;
+        sh_noy = noy;
+        sh_bx[0] = dxl;
+        sh_bx[1] = dxh;
+        sh_bx[2] = dyl;
+        sh_bx[3] = dyh;
+        // the first hill can be placed in no tile: outside the grid
+        if (nb == 0) atomicCAS(err, 0, (int)(1 + (k0 < 0x3fffffff ? k0 : 0x3fffffff)));
+      }
     }
     __syncthreads();
-    if (!sh_ok) {  // uniform
+    const int nb = sh_nb;  // uniform
+    if (nb == 0) {
       flush();
       return;
     }
-    if (!sh_fits) {
+    if (sh_reload) {
       flush();
       // the tile written above may overlap the one read below
       __threadfence();
@@ -419,50 +515,133 @@ __global__ __launch_bounds__(kSeqThreads) void k_deposit_seq_tile2(GridArg G, do
       for (int j = t; j < tw * th; j += blockDim.x) {
         const int lx = j % tw, ly = j / tw;
         const long long node = (long long)(oy + ly) * nx + (ox + lx);
-        const int l = lx + kTileX * ly;
+        const int l = lx + tw * ly;
         tv[l] = val[node];
         tdx[l] = der[2 * node];
         tdy[l] = der[2 * node + 1];
       }
       __syncthreads();
     }
-    const unsigned ind[2] = {sh_ind[0], sh_ind[1]};
-    const double c[2] = {sh_c[0], sh_c[1]};
-    if (t < WAVE_M) {
-      double v = 0.0;
-      if (G.welltemp && t < 4) {
-        // corner ip = t of the cell (Grid.cpp:442-505): skipped past the last node
-        const unsigned n0 = ind[0] + (t & 1), n1 = ind[1] + (t >> 1);
-        if (n0 < G.nbin[0] && n1 < G.nbin[1]) {
-          const unsigned ni[2] = {n0, n1};
-          const int l = (int)(n0 - ox) + kTileX * (int)(n1 - oy);
-          const double dv[2] = {tdx[l], tdy[l]};
-          v = spline_corner_vals(G, c, ind, ni, tv[l], dv);
+    // 2a. shapes of each hill j at the corners of the later hills
+    for (int p = t; p < kRun * 4 * kRun; p += blockDim.x) {
+      const int j = p / (4 * kRun), l = p % (4 * kRun);
+      const int r = l >> 2, q = l & 3;
+      double e = -1.0, dp[2] = {0.0, 0.0};
+      if (j < r && r < nb) {
+        const unsigned n0 = (unsigned)sh_ind[r][0] + (q & 1), n1 = (unsigned)sh_ind[r][1] + (q >> 1);
+        if (n0 < G.nbin[0] && n1 < G.nbin[1] && abs((int)n0 - sh_ind[j][0]) <= nn0 &&
+            abs((int)n1 - sh_ind[j][1]) <= nn1) {
+          const double xx[2] = {G.min[0] + (double)n0 * G.dx[0], G.min[1] + (double)n1 * G.dx[1]};
+          const double cj[2] = {sh_c[j][0], sh_c[j][1]};
+          double ej;
+          if (gauss_shape<2>(G, xx, cj, dp, ej)) e = ej;
         }
       }
-      double tot = 0.0;
-      for (int q = 0; q < 4; q++) tot += __shfl(v, q, WAVE_M);
-      if (t == 0) {
+      sh_e[j][l] = e;
+      sh_dp[j][l][0] = dp[0];
+      sh_dp[j][l][1] = dp[1];
+    }
+    __syncthreads();
+    // 2b. heights, in order (wave 0)
+    if (wave0) {
+      const int r = lane >> 2, q = lane & 3;
+      const bool mine = r < nb;
+      double c[2] = {0.0, 0.0};
+      unsigned ind[2] = {0u, 0u}, ni[2] = {0u, 0u};
+      bool has = false;
+      double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+      if (mine) {
+        c[0] = sh_c[r][0];
+        c[1] = sh_c[r][1];
+        ind[0] = (unsigned)sh_ind[r][0];
+        ind[1] = (unsigned)sh_ind[r][1];
+        // corner q of the cell (Grid.cpp:442-505): absent past the last node
+        ni[0] = ind[0] + (q & 1);
+        ni[1] = ind[1] + (q >> 1);
+        has = ni[0] < G.nbin[0] && ni[1] < G.nbin[1];
+        if (has) {
+          const int l = (int)(ni[0] - ox) + tw * (int)(ni[1] - oy);
+          acc0 = tv[l];
+          acc1 = tdx[l];
+          acc2 = tdy[l];
+        }
+      }
+      // the terms of spline_corner_vals that depend on the position only
+      double X[2] = {0.0, 0.0}, X2[2] = {0.0, 0.0}, X3[2] = {0.0, 0.0}, sg[2] = {1.0, 1.0};
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const int x0 = (ni[j] == ind[j]) ? 0 : 1;
+        const double dx = G.dx[j];
+        const double xfloor = G.min[j] + (double)ind[j] * dx;
+        X[j] = fabs((c[j] - xfloor) / dx - (double)x0);
+        X2[j] = X[j] * X[j];
+        X3[j] = X2[j] * X[j];
+        sg[j] = x0 ? -1.0 : 1.0;
+      }
+      for (int k = 0; k < nb; k++) {
+        // corner term of this lane (used by the lanes of hill k only): the
+        // arithmetic of spline_corner_vals
+        double v = 0.0;
+        if (G.welltemp && has && r == k) {
+          const double grid = acc0;
+          const double der[2] = {acc1, acc2};
+          double ff = 1.0;
+#pragma unroll
+          for (int j = 0; j < 2; j++) {
+            const double yy = fabs(grid) < 0.0000001 ? 0.0 : -der[j] / grid;
+            ff *= (1.0 - 3.0 * X2[j] + 2.0 * X3[j]) - sg[j] * yy * (X[j] - 2.0 * X2[j] + X3[j]) * G.dx[j];
+          }
+          v = grid * ff;
+        }
+        // the four corners of hill k in corner order, on every lane
+        double tot = 0.0;
+#pragma unroll
+        for (int qq = 0; qq < 4; qq++) tot += readlane_d(v, 4 * k + qq);
         const double h = G.welltemp ? wt_height(G, tot) : G.height0;
-        sh_h = h;
-        if (heights) heights[k] = h;
+        if (lane == 4 * k) {
+          sh_h[k] = h;
+          if (heights) heights[k0 + k] = h;
+        }
+        if (r > k && mine) {
+          const double e = sh_e[k][lane];
+          if (e >= 0.0) {
+            const double dp[2] = {sh_dp[k][lane][0], sh_dp[k][lane][1]};
+            double d[2];
+            const double b = gauss_apply<2>(G, dp, e, h, d);
+            acc0 += b;
+            acc1 += d[0];
+            acc2 += d[1];
+          }
+        }
       }
     }
     __syncthreads();
-    const double h = sh_h;
-    for (int sidx = t; sidx < G.support; sidx += blockDim.x) {
-      unsigned ni[kMaxCv];
-      if (!support_node(G, ind, sidx, ni)) continue;
-      double xx[2], d[2];
-      xx[0] = G.min[0] + (double)ni[0] * G.dx[0];
-      xx[1] = G.min[1] + (double)ni[1] * G.dx[1];
-      const double b = gaussian(G, xx, c, h, d);
-      const int l = (int)(ni[0] - ox) + kTileX * (int)(ni[1] - oy);
-      tv[l] += b;
-      tdx[l] += d[0];
-      tdy[l] += d[1];
+    // 3. the run on the tile: one node per thread, hills in order
+    {
+      const int bx0 = sh_bx[0], bx1 = sh_bx[1], by0 = sh_bx[2], by1 = sh_bx[3];
+      const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1;
+      for (int j = t; j < bw * bh; j += blockDim.x) {
+        const int n0 = bx0 + j % bw, n1 = by0 + j / bw;
+        const int l = (n0 - ox) + tw * (n1 - oy);
+        const double xx[2] = {G.min[0] + (double)n0 * G.dx[0], G.min[1] + (double)n1 * G.dx[1]};
+        double v = tv[l], gx = tdx[l], gy = tdy[l];
+        for (int k = 0; k < nb; k++) {
+          if (abs(n0 - sh_ind[k][0]) <= nn0 && abs(n1 - sh_ind[k][1]) <= nn1) {
+            const double ck[2] = {sh_c[k][0], sh_c[k][1]};
+            double d[2];
+            const double b = gaussian<2>(G, xx, ck, sh_h[k], d);
+            v += b;
+            gx += d[0];
+            gy += d[1];
+          }
+        }
+        tv[l] = v;
+        tdx[l] = gx;
+        tdy[l] = gy;
+      }
     }
     __syncthreads();
+    k0 += nb;
   }
   flush();
 }
@@ -727,12 +906,27 @@ int p2a_metad_deposit_device(p2a_metad* m, int64_t nhills, const double* centers
   if (nhills <= 0) return P2A_OK;
   P2A_HIPM(hipSetDevice(m->device));
   const GridArg& G = m->G;
-  const bool tile2 = G.ncv == 2 && !G.pbc[0] && !G.pbc[1] && !G.may_alias &&
-                     2 * G.nneigh[0] + 4 <= std::min<int>(kTileX, (int)G.nbin[0]) &&
-                     2 * G.nneigh[1] + 4 <= std::min<int>(kTileY, (int)G.nbin[1]);
+  // LDS tile of the 2-CV kernel: shaped like the hill support (a walk
+  // crosses it in a number of hills about equal along both CVs), at least
+  // two supports plus the spline corner wide, within the grid
+  int tw = 0, th = 0;
+  bool tile2 = G.ncv == 2 && !G.pbc[0] && !G.pbc[1] && !G.may_alias;
+  if (tile2) {
+    const int nx = (int)G.nbin[0], ny = (int)G.nbin[1];
+    const int need0 = 2 * G.nneigh[0] + 4, need1 = 2 * G.nneigh[1] + 4;
+    const double a0 = 2.0 * G.nneigh[0] + 1.0, a1 = 2.0 * G.nneigh[1] + 1.0;
+    tw = std::min(nx, std::max(need0, (int)std::floor(std::sqrt(kTileNodes * a0 / a1))));
+    th = std::min(ny, kTileNodes / std::max(tw, 1));
+    if (th < need1) {
+      th = std::min(ny, need1);
+      tw = std::min(nx, kTileNodes / th);
+    }
+    tile2 = tw >= std::min(nx, need0) && th >= std::min(ny, need1) && tw * th <= kTileNodes &&
+            need0 <= nx && need1 <= ny;
+  }
   if (tile2)
     hipLaunchKernelGGL(k_deposit_seq_tile2, dim3(1), dim3(kSeqThreads), 0, m->stream, m->G, m->val, m->der,
-                       (long long)nhills, centers_dev, heights_dev, m->err);
+                       (long long)nhills, centers_dev, heights_dev, m->err, tw, th);
   else
     hipLaunchKernelGGL(k_deposit_seq, dim3(1), dim3(kSeqThreads), 0, m->stream, m->G, m->val, m->der,
                        (long long)nhills, centers_dev, heights_dev, m->err);
