@@ -1,21 +1,21 @@
 """COORDINATION as a differentiable torch function (learned-CV training).
 
-The forward runs the value-only pair kernel; the backward is the library's
-backward entry point, which recomputes the pair terms with derivatives on the
-neighbour structure the forward built, so no per-pair data is kept between
-the two.  The forward records which build of the structure it used; when
-another forward on the same action has replaced it before this backward (two
-forwards, then the backward of both), the backward first rebuilds the
-structure at its own positions.  Positions may be float32 or float64 CUDA tensors; the gradient comes
-back in the input dtype.  Both passes run on torch's current stream.
+When a gradient is wanted, the forward runs the pair kernels once for the
+value and the derivatives and keeps the derivatives; the backward is then
+grad * dCV/dx, with no second pass over the pairs.  Without a gradient
+(torch.no_grad, or positions that do not require one) the forward runs the
+cheaper value-only kernel.  Every forward keeps its own derivatives, so two
+forwards followed by the backward of both give each its own gradient.
+Positions may be float32 or float64 CUDA tensors; the gradient comes back in
+the input dtype.  Both passes run on torch's current stream.
 """
 from __future__ import annotations
 
-import ctypes as C
+import ctypes as C  # noqa: F401  (kept for callers that bind the C ABI through this module)
 
 import torch
 
-from . import _capi as A
+from . import _capi as A  # noqa: F401
 from .coordination import Coordination, _box9
 
 
@@ -29,26 +29,22 @@ class CoordinationFunction(torch.autograd.Function):
             raise ValueError(f"expected positions of shape ({action.natoms}, 3)")
         val = torch.empty(1, dtype=torch.float64, device=positions.device)
         action.set_stream(torch.cuda.current_stream(positions.device).cuda_stream)
-        action.calculate_device(pos64.data_ptr(), box, step, val.data_ptr(), 0, 0)
-        ctx.generation = action.list_generation()
-        ctx.action = action
-        ctx.box = _box9(box)
+        want_grad = ctx.needs_input_grad[0]
+        der = torch.empty_like(pos64) if want_grad else None
+        # CoordinationBase::calculate: value and derivatives in one pass
+        action.calculate_device(pos64.data_ptr(), box, step, val.data_ptr(), der.data_ptr() if want_grad else 0, 0)
         ctx.in_dtype = positions.dtype
-        ctx.save_for_backward(pos64)
+        if want_grad:
+            ctx.save_for_backward(der)
         return val.to(positions.dtype).reshape(())
 
     @staticmethod
     def backward(ctx, grad):
-        (pos64,) = ctx.saved_tensors
-        action = ctx.action
-        der = torch.empty_like(pos64)
-        action.set_stream(torch.cuda.current_stream(pos64.device).cuda_stream)
-        if action.list_generation() != ctx.generation:
-            A.check(A.lib().p2a_coord_rebuild_device(action._ctx, ctx.box.ctypes.data, C.c_void_p(pos64.data_ptr())))
-        A.check(A.lib().p2a_coord_backward_device(action._ctx, ctx.box.ctypes.data, C.c_void_p(pos64.data_ptr()),
-                                                  C.c_void_p(der.data_ptr()), None))
-        g = grad.to(torch.float64) * der
-        return g.to(ctx.in_dtype), None, None, None
+        (der,) = ctx.saved_tensors
+        # one elementwise kernel: grad * der, written in the input dtype
+        g = torch.empty(der.shape, dtype=ctx.in_dtype, device=der.device)
+        torch.mul(der, grad.to(torch.float64), out=g)
+        return g, None, None, None
 
 
 def coordination(positions, action: Coordination, box, step: int = 0):

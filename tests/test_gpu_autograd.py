@@ -131,10 +131,11 @@ def test_module_training_step():
 
 @pytest.mark.parametrize("skin,sigma", [(0.0, 0.05), (0.1, 0.05), (0.1, 0.005)])
 def test_two_forwards_before_one_backward(skin, sigma):
-    """loss = m(x1) + m(x2), then one backward.  When the second forward
-    replaced the pair structure, the backward of the first rebuilds it at x1;
-    each gradient is checked against the oracle derivatives at its own
-    positions (sigma 0.05 nm moves pairs across D_MAX between x1 and x2)."""
+    """loss = m(x1) + m(x2), then one backward.  Each forward keeps the
+    derivatives at its own positions, whatever the second forward does to the
+    pair structure; each gradient is checked against the oracle derivatives
+    at its own positions (sigma 0.05 nm moves pairs across D_MAX between x1
+    and x2)."""
     import torch
 
     import plumed2_amd as P
@@ -157,3 +158,26 @@ def test_two_forwards_before_one_backward(skin, sigma):
         _, od, _, _ = O.coordination(sw, box, xx, na, nb)
         g = t.grad.cpu().numpy()
         assert np.max(np.abs(g - od)) <= 1e-5 * np.max(np.abs(od))
+
+
+def test_forward_without_gradient_uses_value_only_pass():
+    """Under torch.no_grad (and for positions that need no gradient) the
+    forward takes the value-only kernels; the value equals the one of the
+    forward that also forms the derivatives."""
+    import torch
+
+    import plumed2_amd as P
+    from plumed2_amd.autograd import coordination
+    na, nb = 500, 4000
+    x, L = system(na, nb, 100.0, 11)
+    box = np.diag([L] * 3)
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{na} GROUPB={na + 1}-{na + nb} SWITCH={{{SWITCH}}}")
+    t = torch.tensor(x, dtype=torch.float64, device="cuda", requires_grad=True)
+    v_grad = coordination(t, act, box, 0)
+    with torch.no_grad():
+        v_nograd = coordination(t, act, box, 1)
+    v_plain = coordination(t.detach(), act, box, 2)
+    assert v_nograd.requires_grad is False and v_plain.requires_grad is False
+    ov = O.coordination(O.Switch(SWITCH), box, x, na, nb)[0]
+    for v in (v_grad, v_nograd, v_plain):
+        assert abs(float(v.detach()) - ov) <= 1e-6 * abs(ov)
