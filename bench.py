@@ -553,7 +553,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": measured_traffic(),
-            "kernel": f"k_pairs<3,false,false,false,{tuned[0]},*> (FP32 pass + FP64-decision pass, value/derivative/virial)",
+            "kernel": f"k_pairs<3,false,false,false,{tuned[0]}> + k_pairs_band (FP32 pass + FP64 cut decisions of the band pairs, value/derivative/virial)",
             "kernel_ms": kern_ms,
             "flop_per_pair": FLOP_PER_PAIR,
         },
