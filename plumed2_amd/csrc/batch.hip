@@ -98,18 +98,16 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
   const size_t nslot = (size_t)clc * CL;
   const int chunk = b->tune.chunk;
   const int* ncl_dev = b->totals.p + 1;
-  hipLaunchKernelGGL(k_fill_int, dim3(grid1(ncol + 1, 256)), dim3(256), 0, s, b->col_count.p, ncol + 1, 0);
-  hipLaunchKernelGGL(k_fill_int, dim3(grid1(ncol + 1, 256)), dim3(256), 0, s, b->fill.p, ncol + 1, 0);
-  hipLaunchKernelGGL(k_fill_int, dim3(grid1(nslot, 256)), dim3(256), 0, s, b->slot_loc.p, (int)nslot, -1);
+  hipLaunchKernelGGL(k_fill_int3, dim3(std::max(1, std::min(grid1(nslot, 256), 4096))), dim3(256), 0, s,
+                     b->col_count.p, ncol + 1, 0, b->fill.p, ncol + 1, 0, b->slot_loc.p, (int)nslot, -1);
   hipLaunchKernelGGL(k_bin, dim3(grid1(N, 256)), dim3(256), 0, s, N, pos, 0, (const SysArg*)b->sys.p,
                      (const int*)b->atom_sys.p, b->info.p, b->key.p, b->col_count.p);
   hipLaunchKernelGGL(k_scan_cols, dim3(1), dim3(1024), 0, s, ncol, b->col_count.p, b->col_start.p, b->col_cl.p,
-                     b->totals.p);
+                     b->totals.p, b->cl_col.p);
   hipLaunchKernelGGL(k_scatter, dim3(grid1(N, 256)), dim3(256), 0, s, N, b->info.p, b->col_start.p, b->fill.p,
                      b->tmp.p);
   hipLaunchKernelGGL(k_rank, dim3(ncol), dim3(256), 0, s, ncol, b->key.p, b->col_start.p, b->col_cl.p, b->tmp.p,
                      b->slot_loc.p);
-  hipLaunchKernelGGL(k_cl_col, dim3(grid1(ncol, 256)), dim3(256), 0, s, ncol, b->col_cl.p, b->cl_col.p);
   hipLaunchKernelGGL(k_origin, dim3(grid1(clc, 128)), dim3(128), 0, s, clc, ncl_dev, b->slot_loc.p, b->info.p,
                      b->key.p, pos, 0, (const SysArg*)b->sys.p, (const int*)b->atom_sys.p, b->origin.p, b->bb0.p,
                      b->kr.p, b->cl_sys.p);
@@ -341,7 +339,8 @@ int p2a_batch_set_boxes(p2a_batch* b, const double* boxes) {
     fill_pbc_arg(ph, Pb[k]);
   }
   colbase[b->nsys] = (int)ncol;
-  if (ncol > INT32_MAX / 2 || clc * CL > INT32_MAX / 2) return fail(P2A_ERR_UNSUPPORTED, "batch too large");
+  if (ncol > INT32_MAX / 2 || clc * CL > INT32_MAX / 2 || clc > kMaxClusters)
+    return fail(P2A_ERR_UNSUPPORTED, "batch too large");
   if (b->stream) P2A_HIP(hipStreamSynchronize(b->stream));
   int rc;
   if ((rc = b->sys.reserve(b->nsys)) || (rc = b->pbc.reserve(b->nsys)) || (rc = b->sys_colbase.reserve(b->nsys + 1)) ||

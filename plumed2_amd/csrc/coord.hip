@@ -186,22 +186,20 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
   const int wslot = 12 + (int)(c->rebuilds & 1);
   for (int gi = 0; gi < ngroups; gi++) {
     Group& G = c->g[gi];
-    hipLaunchKernelGGL(k_fill_int, dim3(grid1(G.ncol + 1, 256)), dim3(256), 0, s, G.col_count.p, G.ncol + 1, 0);
-    hipLaunchKernelGGL(k_fill_int, dim3(grid1(G.ncol + 1, 256)), dim3(256), 0, s, G.fill.p, G.ncol + 1, 0);
-    hipLaunchKernelGGL(k_fill_int, dim3(grid1(G.slot_loc.n, 256)), dim3(256), 0, s, G.slot_loc.p, (int)G.slot_loc.n, -1);
+    hipLaunchKernelGGL(k_fill_int3, dim3(std::max(1, std::min(grid1(G.slot_loc.n, 256), 1024))), dim3(256), 0, s,
+                       G.col_count.p, G.ncol + 1, 0, G.fill.p, G.ncol + 1, 0, G.slot_loc.p, (int)G.slot_loc.n, -1);
     if (G.n > 0)
       hipLaunchKernelGGL(k_bin, dim3(grid1(G.n, 256)), dim3(256), 0, s, G.n, pos_dev, G.off, (const SysArg*)G.sys.p,
                          (const int*)nullptr, G.info.p, G.key.p, G.col_count.p, c->totals.p + wslot,
                          c->totals.p + (25 - wslot));
     hipLaunchKernelGGL(k_scan_cols, dim3(1), dim3(1024), 0, s, G.ncol, G.col_count.p, G.col_start.p, G.col_cl.p,
-                       c->totals.p + 4 + 2 * gi);
+                       c->totals.p + 4 + 2 * gi, G.cl_col.p);
     if (G.n > 0) {
       hipLaunchKernelGGL(k_scatter, dim3(grid1(G.n, 256)), dim3(256), 0, s, G.n, G.info.p, G.col_start.p, G.fill.p,
                          G.tmp.p);
       hipLaunchKernelGGL(k_rank, dim3(G.ncol), dim3(256), 0, s, G.ncol, G.key.p, G.col_start.p, G.col_cl.p,
                          G.tmp.p, G.slot_loc.p);
     }
-    hipLaunchKernelGGL(k_cl_col, dim3(grid1(G.ncol, 256)), dim3(256), 0, s, G.ncol, G.col_cl.p, G.cl_col.p);
     if ((rc = launch_check("rebuild: binning"))) return rc;
   }
   // cluster counts stay on the device (totals[5 + 2 g]) until the one host

@@ -238,8 +238,9 @@ __global__ __launch_bounds__(256) void k_bin(int n, const double* __restrict__ p
 
 // exclusive scans of column counts (atoms) and of ceil(count/8) (clusters),
 // one workgroup
+// (with cl_col, also the column of every cluster)
 __global__ void k_scan_cols(int ncol, const int* __restrict__ cnt, int* __restrict__ col_start,
-                            int* __restrict__ col_cl, int* __restrict__ totals) {
+                            int* __restrict__ col_cl, int* __restrict__ totals, int* __restrict__ cl_col = nullptr) {
   __shared__ int sa[1024], sc[1024];
   const int t = threadIdx.x, T = blockDim.x;
   int chunk = (ncol + T - 1) / T;
@@ -263,8 +264,11 @@ __global__ void k_scan_cols(int ncol, const int* __restrict__ cnt, int* __restri
   for (int i = b; i < e; i++) {
     col_start[i] = ra;
     col_cl[i] = rc;
+    const int ncl = (cnt[i] + CL - 1) / CL;
+    if (cl_col)
+      for (int k = 0; k < ncl; k++) cl_col[rc + k] = i;
     ra += cnt[i];
-    rc += (cnt[i] + CL - 1) / CL;
+    rc += ncl;
   }
   if (t == T - 1) {
     col_start[ncol] = sa[t];
@@ -295,6 +299,49 @@ __device__ __forceinline__ int block_excl_scan256(int v, int* sh, int& total) {
   const int r = sh[t] - v;
   __syncthreads();
   return r;
+}
+
+// the same two scans in one workgroup of 1024 threads (contiguous chunks per
+// thread), for counts that one workgroup scans faster than three launches
+constexpr int kScanOneMax = 1 << 16;
+__global__ __launch_bounds__(1024) void k_scan2_one(int n, const int* __restrict__ n_dev, const int* __restrict__ in,
+                                                    int chunk, int* __restrict__ out0, int* __restrict__ out1,
+                                                    int* __restrict__ totals, int slot0, int slot1) {
+  __shared__ int sa[1024], sc[1024];
+  if (n_dev) n = *n_dev;
+  const int t = threadIdx.x, T = blockDim.x;
+  const int per = (n + T - 1) / T;
+  const int b = min(n, t * per), e = min(n, b + per);
+  int la = 0, lc = 0;
+  for (int i = b; i < e; i++) {
+    const int v = in[i];
+    la += v;
+    lc += (v + chunk - 1) / chunk;
+  }
+  sa[t] = la;
+  sc[t] = lc;
+  __syncthreads();
+  for (int o = 1; o < T; o <<= 1) {
+    const int va = (t >= o) ? sa[t - o] : 0, vc = (t >= o) ? sc[t - o] : 0;
+    __syncthreads();
+    sa[t] += va;
+    sc[t] += vc;
+    __syncthreads();
+  }
+  int ra = sa[t] - la, rc = sc[t] - lc;
+  for (int i = b; i < e; i++) {
+    const int v = in[i];
+    out0[i] = ra;
+    out1[i] = rc;
+    ra += v;
+    rc += (v + chunk - 1) / chunk;
+  }
+  if (t == T - 1) {
+    out0[n] = sa[t];
+    out1[n] = sc[t];
+    totals[slot0] = sa[t];
+    totals[slot1] = sc[t];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_scan2_sums(int n, const int* __restrict__ n_dev, const int* __restrict__ in,
@@ -391,6 +438,14 @@ __global__ __launch_bounds__(256) void k_scan2_apply(int n, const int* __restric
 __global__ void k_fill_int(int* p, int n, int v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// three fills in one launch (grid-stride over the longest)
+__global__ void k_fill_int3(int* p0, int n0, int v0, int* p1, int n1, int v1, int* p2, int n2, int v2) {
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x, st = gridDim.x * blockDim.x;
+  for (int i = i0; i < n0; i += st) p0[i] = v0;
+  for (int i = i0; i < n1; i += st) p1[i] = v1;
+  for (int i = i0; i < n2; i += st) p2[i] = v2;
 }
 
 // flag[0] = 1 when some atom lies farther than sqrt(lim2) from its position
@@ -905,6 +960,41 @@ __device__ __forceinline__ float xor32_sum(float v) {
   const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
   return __uint_as_float(p[0]) + __uint_as_float(p[1]);
 }
+// the same moves for a double (both halves take the same path)
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double xor16_sum_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const auto pl = __builtin_amdgcn_permlane16_swap((unsigned)(b & 0xffffffffll), (unsigned)(b & 0xffffffffll), false,
+                                                   false);
+  const auto ph = __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+  const double x = __longlong_as_double(((long long)ph[0] << 32) | pl[0]);
+  const double y = __longlong_as_double(((long long)ph[1] << 32) | pl[1]);
+  return x + y;
+}
+__device__ __forceinline__ double xor32_sum_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const auto pl = __builtin_amdgcn_permlane32_swap((unsigned)(b & 0xffffffffll), (unsigned)(b & 0xffffffffll), false,
+                                                   false);
+  const auto ph = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+  const double x = __longlong_as_double(((long long)ph[0] << 32) | pl[0]);
+  const double y = __longlong_as_double(((long long)ph[1] << 32) | pl[1]);
+  return x + y;
+}
+// sum over the 64 lanes of a double, without LDS (the value of lane 0 is used)
+__device__ __forceinline__ double wave_sum_d(double v) {
+  v += dpp_d<kRor8>(v);
+  v += dpp_d<kRor4>(v);
+  v += dpp_d<kRor2>(v);
+  v += dpp_d<kRor1>(v);
+  return xor32_sum_d(xor16_sum_d(v));
+}
+
 // sum over the 8 lanes l, l^8, l^16, ..., l^56 (lane = ii + 8 jj: the sum over jj)
 __device__ __forceinline__ float sum_over_jj(float v) {
   v += dpp_f<kRor8>(v);
@@ -1650,7 +1740,7 @@ __global__ __launch_bounds__(256) void k_gather_reduce(int ngather, int ncl, con
     deriv[3 * (size_t)req + r % 3] = sum;
     return;
   }
-  __shared__ double s[8][256];
+  __shared__ double s[8][4];
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // the first band bitmap words go out together with the first rows, so that
   // the workgroup waits on memory about twice in all (rows and words, then the
@@ -1713,22 +1803,23 @@ __global__ __launch_bounds__(256) void k_gather_reduce(int ngather, int ncl, con
       }
     }
   }
-  for (int q = 0; q < 8; q++) s[q][t] = acc[q];
+  // wave sums on the VALU, then the four waves in order
+#pragma unroll
+  for (int q = 0; q < 8; q++) acc[q] = wave_sum_d(acc[q]);
+  if ((t & 63) == 0)
+    for (int q = 0; q < 8; q++) s[q][t >> 6] = acc[q];
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (t < o)
-      for (int q = 0; q < 8; q++) s[q][t] += s[q][t + o];
-    __syncthreads();
-  }
   if (t == 0) {
-    if (R.value) *R.value = s[0][0];
+    double r[8];
+    for (int q = 0; q < 8; q++) r[q] = ((s[q][0] + s[q][1]) + s[q][2]) + s[q][3];
+    if (R.value) *R.value = r[0];
     if (R.virial) {
-      double v6[6] = {s[1][0], s[2][0], s[3][0], s[4][0], s[5][0], s[6][0]};
+      double v6[6] = {r[1], r[2], r[3], r[4], r[5], r[6]};
       write_virial(R.virial, v6);
     }
     if (R.stats) {
-      R.stats[0] = s[7][0];
-      R.stats[1] += s[7][0];  // running total since the last reset
+      R.stats[0] = r[7];
+      R.stats[1] += r[7];  // running total since the last reset
     }
   }
 }
@@ -2183,6 +2274,11 @@ inline void launch_pairs(const PairArgs& A, int grid, hipStream_t s, bool value_
 // n = *n_dev (or n) i clusters; bsum holds 2 * ceil(ncap / kScanBlock) ints
 inline void launch_list_scans(hipStream_t s, int ncap, const int* n_dev, const int* count, int chunk, int* bsum,
                               int* out0, int* out1, int* totals, int slot0, int slot1) {
+  if (ncap <= kScanOneMax) {
+    hipLaunchKernelGGL(k_scan2_one, dim3(1), dim3(1024), 0, s, ncap, n_dev, count, chunk, out0, out1, totals, slot0,
+                       slot1);
+    return;
+  }
   const int nb = std::max(1, grid1(ncap, kScanBlock));
   hipLaunchKernelGGL(k_scan2_sums, dim3(nb), dim3(256), 0, s, ncap, n_dev, count, chunk, bsum);
   hipLaunchKernelGGL(k_scan2_blocks, dim3(1), dim3(1024), 0, s, nb, ncap, n_dev, bsum, out0, out1, totals, slot0,
