@@ -400,6 +400,23 @@ int p2a_cdist_calculate_device(p2a_cdist *ctx, void *stream, const double *box9,
 int p2a_cdist_apply_force_device(p2a_cdist *ctx, void *stream, const double *f_dev, const double *deriv_dev,
                                  const double *virial_cv_dev, double *forces_dev, double *virial_dev);
 
+/* COORDINATIONNUMBER on the cluster pair list of a COORDINATION context
+ * (src/symfunc/CoordinationNumbers.cpp:326-372: CONTACT_MATRIX rows times
+ * ONES, adjmat/ContactMatrix.h:95-121): the rows are the GROUPA atoms, the
+ * environment GROUPA itself (one group, SPECIES=) or GROUPB (SPECIESA /
+ * SPECIESB).  rows_dev[na] = sum over the environment of s(r), pairs at one
+ * place (|r|^2 < DBL_EPSILON) left out.  With weights_dev[na] also
+ * vjp_dev[na+nb][3] = sum_k w_k dcn_k/dx (the derivative layout of
+ * p2a_coord_calculate_device) and virial_dev[9] (may be NULL) = the matching
+ * box derivative.  The list follows the context's rules (NLIST/NLISTCELLS,
+ * NL_STRIDE, skin).  On the FP64 path (cells narrower than twice the list
+ * radius, NOPBC, duplicate atoms) the rows are computed over every pair and
+ * weights are refused (P2A_ERR_UNSUPPORTED); PAIR is refused.  Device arrays,
+ * host box, asynchronous on the context stream.  No counterpart call in the
+ * reference's C API: this is what the COORDINATIONNUMBER action computes. */
+int p2a_coord_rows_device(p2a_coord *ctx, int64_t step, const double *box9, const double *pos_dev, double *rows_dev,
+                          const double *weights_dev, double *vjp_dev, double *virial_dev);
+
 /* COORDINATIONNUMBER (src/symfunc/CoordinationNumbers.cpp:326-372, a shortcut
  * for CONTACT_MATRIX (src/adjmat/ContactMatrix.cpp) times ONES): per-atom
  * coordination numbers cn[k] = sum over env atoms j != rows[k] of

@@ -5,8 +5,8 @@ The arithmetic lives in the HIP library ``libplumed2amd.so`` (C ABI in
 reference's action interface.
 """
 from ._capi import Plumed2AmdError, lib  # noqa: F401
-from .coordination import (Coordination, CoordinationSpec, coordination_numbers_torch,  # noqa: F401
-                           describe_switch, parse_switch, rational_switch)
+from .coordination import (Coordination, CoordinationNumbers, CoordinationSpec,  # noqa: F401
+                           coordination_numbers_torch, describe_switch, parse_switch, rational_switch)
 from .batch import CoordinationBatch  # noqa: F401
 from .metad import MetaD  # noqa: F401
 from .chain import CenterDistance, MetadStep  # noqa: F401

@@ -79,6 +79,8 @@ _SIGS = {
     "p2a_coord_list_generation": ([C.c_void_p, C.POINTER(C.c_int64)], C.c_int),
     "p2a_coord_rebuild_device": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "p2a_coord_get_stats": ([C.c_void_p, C.POINTER(p2a_coord_stats)], C.c_int),
+    "p2a_coord_rows_device": ([C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_void_p], C.c_int),
     "p2a_coordination_numbers_device": ([C.c_void_p, C.POINTER(p2a_switch), C.c_void_p, C.c_int, C.c_int64,
                                          C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p],
                                         C.c_int),

@@ -209,3 +209,100 @@ def coordination_numbers_torch(switch, pos, box=None, species=None, species_a=No
         rows_t.data_ptr() if rows else None, len(env), env_t.data_ptr() if env else None, pos.data_ptr(),
         cn.data_ptr()))
     return cn
+
+
+def _atoms_arg(idx):
+    """0-based indices, in the given order, as a PLUMED atom list (1-based,
+    consecutive runs written as ranges)."""
+    idx = [int(i) for i in idx]
+    out = []
+    k = 0
+    while k < len(idx):
+        j = k
+        while j + 1 < len(idx) and idx[j + 1] == idx[j] + 1:
+            j += 1
+        out.append(f"{idx[k] + 1}" if j == k else f"{idx[k] + 1}-{idx[j] + 1}")
+        k = j + 1
+    return ",".join(out)
+
+
+class CoordinationNumbers:
+    """COORDINATIONNUMBER (src/symfunc/CoordinationNumbers.cpp:326-372) on the
+    cluster pair list: cn[k] = sum over the environment atoms j of s(r_kj)
+    (ContactMatrix rows times ONES), with the vector-Jacobian product
+    sum_k w_k dcn_k/dx for given row weights (what a function of the vector,
+    e.g. MORE_THAN summed, sends back as forces).
+
+    SPECIES=list gives rows = environment = list; SPECIESA/SPECIESB give rows =
+    A, environment = B.  Indices are 0-based rows of the positions passed to
+    the call.  It runs as a COORDINATION context whose GROUPA is the rows and
+    GROUPB the environment (none for SPECIES), so the same list rules apply
+    (``nl`` takes e.g. "NLISTCELLS NL_CUTOFF=1.5 NL_STRIDE=10")."""
+
+    def __init__(self, switch: str, species=None, species_a=None, species_b=None, *, pbc: bool = True,
+                 nl: str = "", device: int = 0):
+        if species is not None:
+            if species_a is not None or species_b is not None:
+                raise ValueError("give SPECIES or SPECIESA/SPECIESB, not both")
+            rows, env = [int(i) for i in species], None
+        elif species_a is not None and species_b is not None:
+            rows, env = [int(i) for i in species_a], [int(i) for i in species_b]
+        else:
+            raise ValueError("missing atoms input use SPECIES or SPECIESA/SPECIESB")
+        if not rows or (env is not None and not env):
+            raise ValueError("empty species list")
+        if min(rows + (env or [])) < 0:
+            raise ValueError("atom index outside the positions array")
+        line = f"cn: COORDINATION GROUPA={_atoms_arg(rows)} "
+        if env is not None:
+            line += f"GROUPB={_atoms_arg(env)} "
+        line += f"SWITCH={{{switch}}}" + ("" if pbc else " NOPBC") + (f" {nl}" if nl else "")
+        self.action = Coordination(line, device=device)
+        self.rows = rows
+        self.env = env
+        self.pbc = pbc
+        self._idx = None
+
+    def _req(self, device):
+        import torch
+
+        if self._idx is None or self._idx.device != device:
+            self._idx = torch.tensor(self.rows + (self.env or []), dtype=torch.long, device=device)
+        return self._idx
+
+    def __call__(self, pos, box=None, step: int = 0, weights=None):
+        """pos: float64 CUDA tensor [natoms,3]; box 3x3 rows = lattice vectors
+        (ignored with pbc=False).  Returns cn [nrows]; with weights [nrows]
+        also (cn, vjp [natoms,3], box derivative [3,3]).  Enqueued on torch's
+        current stream."""
+        import torch
+
+        if not (pos.is_cuda and pos.dtype == torch.float64 and pos.dim() == 2 and pos.shape[1] == 3):
+            raise ValueError("positions must be a float64 CUDA tensor [natoms,3]")
+        idx = self._req(pos.device)
+        if int(idx.max()) >= pos.shape[0]:
+            raise ValueError("atom index outside the positions array")
+        req = pos.index_select(0, idx).contiguous()
+        b9 = _box9(box if box is not None else np.eye(3))
+        self.action.set_stream(torch.cuda.current_stream(pos.device).cuda_stream)
+        cn = torch.empty(len(self.rows), dtype=torch.float64, device=pos.device)
+        if weights is None:
+            A.check(A.lib().p2a_coord_rows_device(self.action._ctx, int(step), b9.ctypes.data,
+                                                  C.c_void_p(req.data_ptr()), C.c_void_p(cn.data_ptr()), None, None,
+                                                  None))
+            return cn
+        w = torch.as_tensor(weights, dtype=torch.float64, device=pos.device).contiguous()
+        if w.shape != (len(self.rows),):
+            raise ValueError(f"expected {len(self.rows)} weights")
+        vjp_req = torch.empty((len(idx), 3), dtype=torch.float64, device=pos.device)
+        vir = torch.empty((3, 3), dtype=torch.float64, device=pos.device)
+        A.check(A.lib().p2a_coord_rows_device(self.action._ctx, int(step), b9.ctypes.data,
+                                              C.c_void_p(req.data_ptr()), C.c_void_p(cn.data_ptr()),
+                                              C.c_void_p(w.data_ptr()), C.c_void_p(vjp_req.data_ptr()),
+                                              C.c_void_p(vir.data_ptr())))
+        vjp = torch.zeros_like(pos)
+        vjp.index_add_(0, idx, vjp_req)
+        return cn, vjp, vir
+
+    def close(self):
+        self.action.close()

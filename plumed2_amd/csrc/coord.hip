@@ -24,6 +24,7 @@ struct Dir {  // one cluster-pair list: i clusters of group I against group J
   DBuf<Entry> ents;
   DBuf<int4> items;
   DBuf<double> fpart;  // per item: derivative partials of the 8 i atoms
+  DBuf<double> rpart;  // per item: per-atom coordination-number rows of the 8 i atoms
   DBuf<double> ipart;  // per item with band pairs: value, virial, pair count
   DBuf<int> redo;      // items redone whole with FP64 cut decisions
   DBuf<int2> fix;      // items with a few band pairs, fixed up
@@ -108,6 +109,10 @@ struct p2a_coord {
   p2a::DBuf<int> nl_count, nl_start, nl_nbr, nl_cell;
   p2a::DBuf<long long> nl_total;
   bool nl_valid = false;
+  // per-atom coordination numbers: weights per requested atom; row and
+  // environment indices for the FP64 path
+  p2a::DBuf<float> wreq;
+  p2a::DBuf<int> cn_rows, cn_env;
 };
 
 namespace p2a {
@@ -374,7 +379,7 @@ static int reimage(p2a_coord* c, const double* box9) {
 // items of ndir lists), in one workgroup and a fixed order; with
 // gather_dir >= 0 the same launch also forms that direction's derivatives
 static int reduce_partials(p2a_coord* c, int nred, int ndir, double* value_dev, double* virial_dev,
-                           int gather_dir = -1, double* deriv_dev = nullptr) {
+                           int gather_dir = -1, double* deriv_dev = nullptr, bool with_stats = true) {
   hipStream_t s = c->stream;
   int ngather = 0, ncl = 0;
   const float4* gp = nullptr;
@@ -403,7 +408,7 @@ static int reduce_partials(p2a_coord* c, int nred, int ndir, double* value_dev, 
   }
   R.value = value_dev;
   R.virial = virial_dev;
-  R.stats = c->stats.p;
+  R.stats = with_stats ? c->stats.p : nullptr;
   hipLaunchKernelGGL(k_gather_reduce, dim3(ngather + 1), dim3(256), 0, s, ngather, ncl, (const int*)nullptr, gp,
                      istart, items_cap, fpart, deriv_dev, 1, R);
   return P2A_OK;
@@ -641,6 +646,128 @@ static int run_fallback_nl(p2a_coord* c, bool build, const double* box9, const d
   return launch_check("finalize");
 }
 
+// COORDINATIONNUMBER rows of GROUPA over the environment (GROUPA itself, or
+// GROUPB) on the cluster list, and with weights their vector-Jacobian
+// product and box derivative
+static int run_rows(p2a_coord* c, const double* box9, const double* pos_dev, double* rows_dev,
+                    const double* weights_dev, double* vjp_dev, double* virial_dev) {
+  int rc;
+  hipStream_t s = c->stream;
+  const int ngroups = c->spec.nb > 0 ? 2 : 1;
+  for (int gi = 0; gi < ngroups; gi++) {
+    Group& G = c->g[gi];
+    if (G.ncl == 0) continue;
+    hipLaunchKernelGGL(k_pack_bbox, dim3(grid1((size_t)G.ncl * CL, 256)), dim3(256), 0, s, G.ncl * CL,
+                       (const int*)nullptr, G.slot_loc.p, G.info.p, pos_dev, G.off, (const SysArg*)G.sys.p,
+                       (const int*)nullptr, G.origin.p, G.p.p, G.bb.p, (int*)nullptr, 0);
+  }
+  PairArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.twogroup = ngroups == 2;
+  pair_args_switch(c->spec.sw, A);
+  if ((rc = upload_pbc(c, box9))) return rc;
+  A.pbc = c->pbc_dev.p;
+  A.pos64 = pos_dev;
+  const bool weighted = weights_dev != nullptr;
+  if (weighted) {
+    if ((rc = c->wreq.reserve(std::max(c->natoms, 1)))) return rc;
+    hipLaunchKernelGGL(k_row_weights, dim3(grid1(c->natoms, 256)), dim3(256), 0, s, (int)c->spec.na, c->natoms,
+                       weights_dev, c->wreq.p);
+  }
+  // the rows come from the first direction (i = GROUPA); the weighted
+  // product needs every direction for the derivatives of all atoms
+  const int ndir_run = weighted ? c->ndir : 1;
+  int grids[2] = {0, 0}, nred = 0;
+  for (int d = 0; d < ndir_run; d++) {
+    Dir& D = c->dir[d];
+    if (D.nitems == 0) continue;
+    int grid = grid1(D.nitems, PAIR_BLOCK / WAVE);
+    if (c->tune.blocks_per_cu > 0) grid = std::min(grid, c->tune.blocks_per_cu * c->cu_count);
+    grids[d] = grid;
+    nred += grid;
+  }
+  if ((rc = c->partial.reserve(8 * (size_t)std::max(nred, 1)))) return rc;
+  int off = 0;
+  for (int d = 0; d < ndir_run; d++) {
+    Dir& D = c->dir[d];
+    if (grids[d] == 0) continue;
+    Group& GI = c->g[D.I];
+    Group& GJ = c->g[D.J];
+    if ((rc = D.rpart.reserve((size_t)CL * std::max(D.nitems, 1)))) return rc;
+    A.nitems = D.nitems;
+    A.items_cap = (int)D.items.n;
+    A.ent_cap = (int)D.ents.n;
+    launch_rows(A, weighted, grids[d], s, c->wreq.p, D.items.p, D.ents.p, GI.p.p, GJ.p.p, GI.bb.p, GJ.bb.p,
+                GI.abs_slot.p, GJ.abs_slot.p, D.rpart.p, D.fpart.p, c->partial.p + 8 * (size_t)off);
+    off += grids[d];
+  }
+  if ((rc = launch_check("row kernel"))) return rc;
+  {
+    Dir& D = c->dir[0];
+    Group& GI = c->g[D.I];
+    if (GI.ncl > 0 && D.nitems > 0)
+      hipLaunchKernelGGL(k_gather_rows, dim3(grid1((size_t)GI.ncl * CL, 256)), dim3(256), 0, s, GI.ncl * CL, GI.p.p,
+                         (const int*)D.istart.p, (int)D.items.n, (const double*)D.rpart.p, (int)c->spec.na,
+                         rows_dev);
+    else
+      hipLaunchKernelGGL(k_fill_double, dim3(grid1((size_t)c->spec.na, 256)), dim3(256), 0, s, rows_dev,
+                         (size_t)c->spec.na);
+  }
+  if (!weighted) return launch_check("rows");
+  if ((rc = c->out_buf.reserve(16))) return rc;
+  const int last = c->ndir - 1;
+  for (int d = 0; d < last; d++) {
+    Dir& D = c->dir[d];
+    Group& GI = c->g[D.I];
+    if (GI.ncl == 0) continue;
+    const int ng = grid1((size_t)GI.ncl * CL * 3, 256);
+    RedArgs R;
+    std::memset(&R, 0, sizeof(R));
+    hipLaunchKernelGGL(k_gather_reduce, dim3(ng), dim3(256), 0, s, ng, GI.ncl, (const int*)nullptr, GI.p.p,
+                       D.istart.p, (int)D.items.n, D.fpart.p, vjp_dev, 0, R);
+  }
+  double* wdev = virial_dev ? virial_dev : c->out_buf.p + 1;
+  if ((rc = reduce_partials(c, nred, 0, c->out_buf.p, wdev, last, vjp_dev, false))) return rc;
+  return launch_check("row derivatives");
+}
+
+// The cluster list for this step: NeighborList::prepare (NeighborList.cpp:
+// 433-456) rebuilds on the first step and every NL_STRIDE steps.  Without a
+// neighbour list every step, or, with a skin, when some atom has moved more
+// than skin/2 since the build: every pair within D_MAX now was within D_MAX +
+// skin then, so the pairs evaluated are those of a list rebuilt now.
+static int prepare_cluster_list(p2a_coord* c, int64_t step, const double* box9, const double* pos_dev) {
+  int rc;
+  bool need = !c->built || c->firsttime;
+  const bool new_cell = !need && std::memcmp(c->box_at_build, box9, sizeof(c->box_at_build)) != 0;
+  if (c->spec.nl_mode == P2A_NL_NONE) {
+    // the skin test measures motion in one cell: a new cell rebuilds
+    if (new_cell) need = true;
+    if (!need && c->skin_active > 0.0) {
+      if ((rc = c->moved.reserve(1))) return rc;
+      const double half = 0.5 * c->skin_active;
+      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->moved.p, 1, 0);
+      hipLaunchKernelGGL(k_moved, dim3(grid1((size_t)c->natoms, 256)), dim3(256), 0, c->stream, c->natoms, pos_dev,
+                         (const double*)c->pos_ref.p, half * half, c->moved.p);
+      P2A_HIP(hipMemcpyAsync(c->host_ro + 32, c->moved.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      P2A_HIP(hipStreamSynchronize(c->stream));
+      need = c->host_ro[32] != 0;
+    } else {
+      need = true;
+    }
+  } else if (c->spec.nl_stride <= 1 || step % c->spec.nl_stride == 0) {
+    need = true;
+  }
+  c->firsttime = false;
+  if (!need && new_cell && !reimage_exact(c, box9)) need = true;
+  if (need) {
+    if ((rc = rebuild(c, box9, pos_dev))) return rc;
+  } else if (new_cell) {
+    if ((rc = reimage(c, box9))) return rc;
+  }
+  return P2A_OK;
+}
+
 static int calc_device(p2a_coord* c, int64_t step, const double* box9, const double* pos_dev, double* value_dev,
                        double* deriv_dev, double* virial_dev) {
   int rc;
@@ -677,38 +804,7 @@ static int calc_device(p2a_coord* c, int64_t step, const double* box9, const dou
     }
     return run_fallback(c, box9, pos_dev, vdev, ddev, wdev);
   }
-  // NeighborList::prepare (NeighborList.cpp:433-456): rebuild on the first
-  // step and every NL_STRIDE steps.  Without a neighbour list every step, or,
-  // with a skin, when some atom has moved more than skin/2 since the build:
-  // every pair within D_MAX now was within D_MAX + skin then, so the pairs
-  // evaluated are those of a list rebuilt now.
-  bool need = !c->built || c->firsttime;
-  const bool new_cell = !need && std::memcmp(c->box_at_build, box9, sizeof(c->box_at_build)) != 0;
-  if (c->spec.nl_mode == P2A_NL_NONE) {
-    // the skin test measures motion in one cell: a new cell rebuilds
-    if (new_cell) need = true;
-    if (!need && c->skin_active > 0.0) {
-      if ((rc = c->moved.reserve(1))) return rc;
-      const double half = 0.5 * c->skin_active;
-      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->moved.p, 1, 0);
-      hipLaunchKernelGGL(k_moved, dim3(grid1((size_t)c->natoms, 256)), dim3(256), 0, c->stream, c->natoms, pos_dev,
-                         (const double*)c->pos_ref.p, half * half, c->moved.p);
-      P2A_HIP(hipMemcpyAsync(c->host_ro + 32, c->moved.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      P2A_HIP(hipStreamSynchronize(c->stream));
-      need = c->host_ro[32] != 0;
-    } else {
-      need = true;
-    }
-  } else if (c->spec.nl_stride <= 1 || step % c->spec.nl_stride == 0) {
-    need = true;
-  }
-  c->firsttime = false;
-  if (!need && new_cell && !reimage_exact(c, box9)) need = true;
-  if (need) {
-    if ((rc = rebuild(c, box9, pos_dev))) return rc;
-  } else if (new_cell) {
-    if ((rc = reimage(c, box9))) return rc;
-  }
+  if ((rc = prepare_cluster_list(c, step, box9, pos_dev))) return rc;
   return run_pairs(c, box9, pos_dev, vdev, ddev, wdev, value_only);
 }
 
@@ -1062,6 +1158,49 @@ int p2a_coord_set_profiling(p2a_coord* c, int on) {
     }
   }
   return P2A_OK;
+}
+
+int p2a_coord_rows_device(p2a_coord* c, int64_t step, const double* box9, const double* pos_dev, double* rows_dev,
+                          const double* weights_dev, double* vjp_dev, double* virial_dev) {
+  if (!c || !box9 || !pos_dev || !rows_dev) return fail(P2A_ERR_ARG, "null argument");
+  if (weights_dev && !vjp_dev) return fail(P2A_ERR_ARG, "weights need an output for their derivatives");
+  if (c->spec.pair) return fail(P2A_ERR_UNSUPPORTED, "PAIR gives no per-atom rows");
+  P2A_HIP(hipSetDevice(c->device));
+  int rc;
+  if (!c->stats.p) {
+    if ((rc = c->stats.reserve(8))) return rc;
+    hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(64), 0, c->stream, c->stats.p, (size_t)8);
+  }
+  const int path = choose_path(c, box9);
+  if (path != c->path) {
+    c->built = false;
+    c->nl_valid = false;
+  }
+  c->path = path;
+  if (path != 0) {
+    if (weights_dev)
+      return fail(P2A_ERR_UNSUPPORTED,
+                  "derivatives of per-atom coordination numbers need the cluster path (a periodic cell wider than "
+                  "twice the list radius, distinct atoms)");
+    // every environment atom of every row, in FP64
+    const int na = (int)c->spec.na, nb = (int)c->spec.nb;
+    const int nenv = nb > 0 ? nb : na;
+    if ((rc = c->cn_rows.reserve(na)) || (rc = c->cn_env.reserve(nenv))) return rc;
+    std::vector<int> r(na), e(nenv);
+    for (int k = 0; k < na; k++) r[k] = k;
+    for (int k = 0; k < nenv; k++) e[k] = nb > 0 ? na + k : k;
+    P2A_HIP(hipMemcpyAsync(c->cn_rows.p, r.data(), na * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    P2A_HIP(hipMemcpyAsync(c->cn_env.p, e.data(), nenv * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if ((rc = upload_pbc(c, box9))) return rc;
+    hipLaunchKernelGGL(k_cn_rows, dim3(grid1((size_t)na, 256)), dim3(256), 0, c->stream, c->natoms, na,
+                       (const int*)c->cn_rows.p, nenv, (const int*)c->cn_env.p, c->spec.pbc ? 1 : 0, c->pbc_stage,
+                       c->spec.sw, pos_dev, rows_dev);
+    // the host vectors are read by the copies above before they return
+    P2A_HIP(hipStreamSynchronize(c->stream));
+    return launch_check("coordination numbers");
+  }
+  if ((rc = prepare_cluster_list(c, step, box9, pos_dev))) return rc;
+  return run_rows(c, box9, pos_dev, rows_dev, weights_dev, vjp_dev, virial_dev);
 }
 
 int p2a_coordination_numbers_device(void* stream, const p2a_switch* sw, const double* box9, int use_pbc,

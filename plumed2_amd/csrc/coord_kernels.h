@@ -2219,6 +2219,227 @@ struct PairTuning {
   int blocks_per_cu = 8;
 };
 
+// ------------------------------------------------------------------------
+// COORDINATIONNUMBER rows on the cluster list
+// ------------------------------------------------------------------------
+// cn_i = sum over the environment atoms j of s(r_ij) (ContactMatrix row times
+// ONES, symfunc/CoordinationNumbers.cpp:326-372; ContactMatrix::
+// calculateWeight, adjmat/ContactMatrix.h:95-121, skips |r|^2 < DBL_EPSILON).
+// In the full cluster list every row atom i meets all its partners as the i
+// atom of its own items, so lane (ii, jj) sums the s of its pairs over every
+// entry kind.  With weights w (wreq: the weight of each requested atom, 0 for
+// atoms that are no row) the kernel also forms the vector-Jacobian product
+// sum_k w_k dcn_k/dx: F = sum_k w_k cn_k = sum over pairs (w_i + w_j) s_ij,
+// so atom i takes (w_i + w_j)(-dfunc d) from each partner and the box
+// derivative takes (w_i + w_j) dfunc d (x) d once per unordered pair.
+// Band pairs (r^2 within `band` of D_MAX^2) are decided on the spot on the
+// reference's FP64 Pbc::distance.
+// Outputs: rpart[item][8] (row sums of the i atoms), and with WT fpart
+// [item][8][3] and per-workgroup rows (0, six box-derivative products, 0).
+constexpr float kContactEps2 = 2.220446049250313e-16f;  // DBL_EPSILON
+
+template <int P, bool GENERIC, bool TWO, bool WT, int KIND>
+__device__ __forceinline__ void row_term(const PairArgs& A, const P3& pj, const float4& pi, int ii, int jj, int g,
+                                         int ai, float wi, const float* __restrict__ swt, const WaveLds& L,
+                                         const float4* __restrict__ posJ, const PbcArg* __restrict__ pbc,
+                                         PairAcc& acc, float& cnt) {
+  const float dx = pj.x - pi.x;
+  const float dy = pj.y - pi.y;
+  const float dz = pj.z - pi.z;
+  const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+  const float t = r2 - A.dmax2;
+  bool keep = r2 >= kContactEps2;
+  if constexpr (KIND == 0) keep = keep && jj != ii;
+  if constexpr (TWO) keep = keep && (L.sa[g + jj] != ai);
+  bool cut = keep && t <= 0.f;
+  if (keep && fabsf(t) < A.band) {
+    const int meta = L.sm[g + jj];
+    const int reqi = __float_as_int(pi.w);
+    if (reqi >= 0 && meta >= 0) {
+      const int cj = L.sc[meta >> 3] & kCjMask;
+      const int reqj = __float_as_int(posJ[cj * CL + (meta & 7)].w);
+      if (reqj >= 0) cut = switch_inside(A.sw, exact_r2(pbc, A.pos64, reqi, reqj));
+    }
+  }
+  float sv, dfu;
+  pair_switch<P, GENERIC>(A, r2, sv, dfu);
+  acc.sv += cut ? sv : 0.f;
+  cnt += cut ? 1.f : 0.f;
+  if constexpr (WT) {
+    const float wg = cut ? (wi + swt[g + jj]) * dfu : 0.f;
+    const float gx = wg * dx, gy = wg * dy, gz = wg * dz;
+    acc.fx += gx;
+    acc.fy += gy;
+    acc.fz += gz;
+    bool once = KIND == 1;
+    if constexpr (KIND == 0) once = jj > ii;
+    if (KIND != 2) {
+      const float h = once ? 1.f : 0.f;
+      acc.vxx = fmaf(h * gx, dx, acc.vxx);
+      acc.vxy = fmaf(h * gx, dy, acc.vxy);
+      acc.vxz = fmaf(h * gx, dz, acc.vxz);
+      acc.vyy = fmaf(h * gy, dy, acc.vyy);
+      acc.vyz = fmaf(h * gy, dz, acc.vyz);
+      acc.vzz = fmaf(h * gz, dz, acc.vzz);
+    }
+  }
+}
+
+template <int P, bool GENERIC, bool TWO, bool WT>
+__global__ __launch_bounds__(PAIR_BLOCK) void k_rows(PairArgs A, const float* __restrict__ wreq,
+                                                     const int4* __restrict__ items, const Entry* __restrict__ ents,
+                                                     const float4* __restrict__ posI, const float4* __restrict__ posJ,
+                                                     const float4* __restrict__ bbI, const float4* __restrict__ bbJ,
+                                                     const int* __restrict__ absI, const int* __restrict__ absJ,
+                                                     double* __restrict__ rpart, double* __restrict__ fpart,
+                                                     double* __restrict__ partial) {
+  constexpr int CH = 64;
+  constexpr int NW = PAIR_BLOCK / WAVE;
+  constexpr int CAP = stage_cap<CH>();
+  __shared__ P3 sj[NW][CAP];
+  __shared__ short sm[NW][CAP];
+  __shared__ int scode[NW][CH];
+  __shared__ int sabs[TWO ? NW : 1][TWO ? CAP : 1];
+  __shared__ float swt[WT ? NW : 1][WT ? CAP : 1];
+  __shared__ double stot[NW][8];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int ii = lane & 7, jj = lane >> 3;
+  const int nitems = A.nitems_dev ? min(*A.nitems_dev, A.items_cap) : A.nitems;
+  const WaveLds L = {sj[w], sm[w], scode[w], TWO ? sabs[w] : nullptr};
+  if (lane < 8) stot[w][lane] = 0.0;
+  for (int k = blockIdx.x * NW + w; k < nitems; k += gridDim.x * NW) {
+    const int item = k;
+    const int4 it = items[item];
+    const int ci = it.x;
+    const int e1 = min(it.z, A.ent_cap);
+    const float4 pi = posI[ci * CL + ii];
+    const int ai = TWO ? absI[ci * CL + ii] : 0;
+    const float4 ic = bbI[2 * ci], ih = bbI[2 * ci + 1];
+    const PbcArg* pbc = A.pbc + nsys_of(A.cl_sys, ci);
+    Entry my_e = {0, 0.f, 0.f, 0.f};
+    if (lane < CH && it.y + lane < e1) my_e = ents[it.y + lane];
+    int n0, b1, n1, b2, n2;
+    stage_item<TWO, false, CH>(A, my_e, lane, it.y, e1, ic, ih, posJ, bbJ, absJ, L, n0, b1, n1, b2, n2);
+    n1 = __builtin_amdgcn_readfirstlane(n1);
+    n2 = __builtin_amdgcn_readfirstlane(n2);
+    b1 = __builtin_amdgcn_readfirstlane(b1);
+    b2 = __builtin_amdgcn_readfirstlane(b2);
+    float wi = 0.f;
+    if constexpr (WT) {
+      // weights of the staged atoms (0 for padding)
+      const int nst = b2 + n2;
+      for (int p = lane; p < nst; p += WAVE) {
+        const int meta = L.sm[p];
+        float wv = 0.f;
+        if (meta >= 0) {
+          const int cj = L.sc[meta >> 3] & kCjMask;
+          const int req = __float_as_int(posJ[cj * CL + (meta & 7)].w);
+          if (req >= 0) wv = wreq[req];
+        }
+        swt[w][p] = wv;
+      }
+      const int reqi = __float_as_int(pi.w);
+      wi = reqi >= 0 ? wreq[reqi] : 0.f;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    const float* sw = WT ? swt[w] : nullptr;
+    PairAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float cnt = 0.f;
+    if (n0) row_term<P, GENERIC, TWO, WT, 0>(A, L.sj[jj], pi, ii, jj, 0, ai, wi, sw, L, posJ, pbc, acc, cnt);
+    for (int g = b1; g < b1 + n1; g += CL)
+      row_term<P, GENERIC, TWO, WT, 1>(A, L.sj[g + jj], pi, ii, jj, g, ai, wi, sw, L, posJ, pbc, acc, cnt);
+    for (int g = b2; g < b2 + n2; g += CL)
+      row_term<P, GENERIC, TWO, WT, 2>(A, L.sj[g + jj], pi, ii, jj, g, ai, wi, sw, L, posJ, pbc, acc, cnt);
+    // rows of the 8 i atoms: s = stretch * s~ + shift per contributing pair
+    const float cs = sum_over_jj(acc.sv), cc = sum_over_jj(cnt);
+    if (jj == 0) rpart[(size_t)item * CL + ii] = A.sv_scale * (double)cs + A.sv_shift * (double)cc;
+    if constexpr (WT) {
+      const float fx = sum_over_jj(acc.fx), fy = sum_over_jj(acc.fy), fz = sum_over_jj(acc.fz);
+      if (jj == 0) {
+        double* fp = fpart + (size_t)item * (CL * 3) + ii * 3;
+        fp[0] = -A.df_scale * (double)fx;
+        fp[1] = -A.df_scale * (double)fy;
+        fp[2] = -A.df_scale * (double)fz;
+      }
+      float tv[6];
+      tv[0] = wave_sum_f(acc.vxx);
+      tv[1] = wave_sum_f(acc.vxy);
+      tv[2] = wave_sum_f(acc.vxz);
+      tv[3] = wave_sum_f(acc.vyy);
+      tv[4] = wave_sum_f(acc.vyz);
+      tv[5] = wave_sum_f(acc.vzz);
+      if (lane >= 1 && lane < 7) {
+        float tq = tv[0];
+        for (int q = 1; q < 6; q++)
+          if (lane == q + 1) tq = tv[q];
+        stot[w][lane] += A.df_scale * (double)tq;
+      }
+    }
+    // the next item overwrites the staging read above
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  if constexpr (WT) {
+    __syncthreads();
+    if (w == 0 && lane < 8) {
+      double s = 0.0;
+      for (int v = 0; v < NW; v++) s += stot[v][lane];
+      partial[(size_t)blockIdx.x * 8 + lane] = s;
+    }
+  }
+}
+
+// rows of the requested atoms 0..nrows-1 (the i group): the fixed-order sum of
+// their cluster's item rows; atoms beyond nrows (the environment) are skipped
+__global__ __launch_bounds__(256) void k_gather_rows(int nslot, const float4* __restrict__ p,
+                                                     const int* __restrict__ item_start, int items_cap,
+                                                     const double* __restrict__ rpart, int nrows,
+                                                     double* __restrict__ rows) {
+  const int sidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sidx >= nslot) return;
+  const int c = sidx / CL, r = sidx % CL;
+  const int i0 = item_start[c];
+  const int i1 = min(item_start[c + 1], items_cap);
+  const int req = __float_as_int(p[sidx].w);
+  if (req < 0 || req >= nrows) return;
+  double sum = 0.0;
+  for (int it = i0; it < i1; it++) sum += rpart[(size_t)it * CL + r];
+  rows[req] = sum;
+}
+
+// FP32 weight of every requested atom: w[k] for the first nrows, 0 after
+__global__ void k_row_weights(int nrows, int natoms, const double* __restrict__ w, float* __restrict__ wreq) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < natoms) wreq[k] = k < nrows ? (float)w[k] : 0.f;
+}
+
+inline void launch_rows(const PairArgs& A, bool weighted, int grid, hipStream_t s, const float* wreq,
+                        const int4* items, const Entry* ents, const float4* posI, const float4* posJ,
+                        const float4* bbI, const float4* bbJ, const int* absI, const int* absJ, double* rpart,
+                        double* fpart, double* partial) {
+  const p2a_switch& sw = A.sw;
+#define P2A_R(P, GEN, TWO, WT)                                                                                   \
+  hipLaunchKernelGGL((k_rows<P, GEN, TWO, WT>), dim3(grid), dim3(PAIR_BLOCK), 0, s, A, wreq, items, ents, posI,   \
+                     posJ, bbI, bbJ, absI, absJ, rpart, fpart, partial)
+#define P2A_RSW(P, GEN)                                        \
+  do {                                                         \
+    if (A.twogroup) {                                          \
+      if (weighted) P2A_R(P, GEN, true, true);                 \
+      else P2A_R(P, GEN, true, false);                         \
+    } else {                                                   \
+      if (weighted) P2A_R(P, GEN, false, true);                \
+      else P2A_R(P, GEN, false, false);                        \
+    }                                                          \
+  } while (0)
+  if (sw.type == P2A_SW_RATIONAL_FIXED && sw.nn / 2 == 3) P2A_RSW(3, false);
+  else if (sw.type == P2A_SW_RATIONAL_FIXED && sw.nn / 2 == 4) P2A_RSW(4, false);
+  else P2A_RSW(1, true);
+#undef P2A_RSW
+#undef P2A_R
+}
+
 // workgroups of the band kernel: it loops over the few items the FP32 pass
 // queues (one wave per fixed-up item, one workgroup per item redone whole)
 constexpr int kBandGrid = 64;
