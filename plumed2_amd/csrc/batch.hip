@@ -107,10 +107,8 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
   hipLaunchKernelGGL(k_scatter, dim3(grid1(N, 256)), dim3(256), 0, s, N, b->info.p, b->col_start.p, b->fill.p,
                      b->tmp.p);
   hipLaunchKernelGGL(k_rank, dim3(ncol), dim3(256), 0, s, ncol, b->key.p, b->col_start.p, b->col_cl.p, b->tmp.p,
-                     b->slot_loc.p);
-  hipLaunchKernelGGL(k_origin, dim3(grid1(clc, 128)), dim3(128), 0, s, clc, ncl_dev, b->slot_loc.p, b->info.p,
-                     b->key.p, pos, 0, (const SysArg*)b->sys.p, (const int*)b->atom_sys.p, b->origin.p, b->bb0.p,
-                     b->kr.p, b->cl_sys.p);
+                     b->slot_loc.p, b->info.p, pos, 0, (const SysArg*)b->sys.p, (const int*)b->atom_sys.p,
+                     b->origin.p, b->bb0.p, b->kr.p, b->cl_sys.p);
   ListArgs L;
   L.nclI = clc;
   L.nclI_dev = ncl_dev;

@@ -203,18 +203,13 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
       hipLaunchKernelGGL(k_scatter, dim3(grid1(G.n, 256)), dim3(256), 0, s, G.n, G.info.p, G.col_start.p, G.fill.p,
                          G.tmp.p);
       hipLaunchKernelGGL(k_rank, dim3(G.ncol), dim3(256), 0, s, G.ncol, G.key.p, G.col_start.p, G.col_cl.p,
-                         G.tmp.p, G.slot_loc.p);
+                         G.tmp.p, G.slot_loc.p, G.info.p, pos_dev, G.off, (const SysArg*)G.sys.p,
+                         (const int*)nullptr, G.origin.p, G.bb0.p, G.kr.p, (int*)nullptr);
     }
     if ((rc = launch_check("rebuild: binning"))) return rc;
   }
   // cluster counts stay on the device (totals[5 + 2 g]) until the one host
   // read below; the kernels in between are sized by the cluster capacity
-  for (int gi = 0; gi < ngroups; gi++) {
-    Group& G = c->g[gi];
-    hipLaunchKernelGGL(k_origin, dim3(grid1(G.maxcl, 128)), dim3(128), 0, s, G.maxcl, (const int*)c->totals.p + 5 + 2 * gi,
-                       G.slot_loc.p, G.info.p, G.key.p, pos_dev, G.off, (const SysArg*)G.sys.p, (const int*)nullptr,
-                       G.origin.p, G.bb0.p, G.kr.p, (int*)nullptr);
-  }
   // full cluster-pair lists: one direction for a single group, A->B (the
   // counting side) and B->A for two groups
   c->ndir = ngroups;

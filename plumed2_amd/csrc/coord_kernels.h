@@ -501,34 +501,56 @@ __global__ __launch_bounds__(256) void k_scatter(int n, const int4* __restrict__
   if (col >= 0) tmp[col_start[col] + hist[col - lo] + local] = i;
 }
 
-// rank of each atom inside its column by (key, index): deterministic order.
-// One workgroup per column; the column's keys are staged in LDS when they fit.
+// Slots of one column and the bounds of its clusters; one workgroup per column.
+// (1) Rank of each atom inside its column by (key, index): a deterministic
+//     order.  The column's keys and indices are staged in LDS (padded to a
+//     multiple of 4 with keys that rank last) when they fit; each thread
+//     counts the keys below its own, four at a time.
+// (2) Per cluster (8 lanes, one per slot): FP64 origin (bounding-box centre
+//     of the wrapped positions at the rebuild), FP32 bounding box, range of
+//     fractional c (widened outwards so that the FP32 range contains the
+//     FP64 keys) and system.  Padding slots come last in a cluster, so its
+//     first slot always holds an atom.
 constexpr int kRankLds = 2048;
 __global__ __launch_bounds__(256) void k_rank(int ncol, const double* __restrict__ key,
                                               const int* __restrict__ col_start, const int* __restrict__ col_cl,
-                                              const int* __restrict__ tmp, int* __restrict__ slot_loc) {
-  __shared__ double sk[kRankLds];
-  __shared__ int si[kRankLds];
+                                              const int* __restrict__ tmp, int* __restrict__ slot_loc,
+                                              const int4* __restrict__ info, const double* __restrict__ pos, int off,
+                                              const SysArg* __restrict__ sys, const int* __restrict__ atom_sys,
+                                              double4* __restrict__ origin, float4* __restrict__ bb0,
+                                              float2* __restrict__ kr, int* __restrict__ cl_sys) {
+  __shared__ __align__(16) double sk[kRankLds];
+  __shared__ __align__(16) int si[kRankLds];
+  __shared__ int ss[kRankLds];
   const int col = blockIdx.x;
   if (col >= ncol) return;
   const int b = col_start[col], n = col_start[col + 1] - b;
-  const int base = col_cl[col] * CL;
-  if (n <= kRankLds) {
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
-      int o = tmp[b + t];
+  const int c0 = col_cl[col], ncl = col_cl[col + 1] - c0;
+  const int base = c0 * CL;
+  const int n4 = (n + 3) & ~3;
+  const bool lds = n4 <= kRankLds;
+  if (lds) {
+    for (int t = threadIdx.x; t < n4; t += blockDim.x) {
+      const int o = t < n ? tmp[b + t] : INT_MAX;
       si[t] = o;
-      sk[t] = key[o];
+      sk[t] = t < n ? key[o] : __builtin_huge_val();
     }
     __syncthreads();
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
       const double ka = sk[t];
       const int ia = si[t];
       int rank = 0;
-      for (int u = 0; u < n; u++) {
-        const double kb = sk[u];
-        rank += (kb < ka) || (kb == ka && si[u] < ia);
+      for (int u = 0; u < n4; u += 4) {
+        const double2 k01 = *reinterpret_cast<const double2*>(sk + u);
+        const double2 k23 = *reinterpret_cast<const double2*>(sk + u + 2);
+        const int4 iv = *reinterpret_cast<const int4*>(si + u);
+        rank += (k01.x < ka) || (k01.x == ka && iv.x < ia);
+        rank += (k01.y < ka) || (k01.y == ka && iv.y < ia);
+        rank += (k23.x < ka) || (k23.x == ka && iv.z < ia);
+        rank += (k23.y < ka) || (k23.y == ka && iv.w < ia);
       }
       slot_loc[base + rank] = ia;
+      ss[rank] = ia;
     }
   } else {
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
@@ -543,6 +565,47 @@ __global__ __launch_bounds__(256) void k_rank(int ncol, const double* __restrict
       slot_loc[base + rank] = ia;
     }
   }
+  __syncthreads();
+  // (2) the column's clusters; the loop bound is uniform over the workgroup
+  for (int t0 = 0; t0 < ncl * CL; t0 += blockDim.x) {
+    const int t = t0 + (int)threadIdx.x;
+    int a = -1;
+    if (t < n) a = lds ? ss[t] : slot_loc[base + t];
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    double klo = DBL_MAX, khi = -DBL_MAX;
+    int sy = 0;
+    if (a >= 0) {
+      sy = nsys_of(atom_sys, a);
+      const SysArg& S = sys[sy];
+      const int4 m = info[a];
+      const double* x = pos + 3 * (size_t)(off + a);
+      for (int d = 0; d < 3; d++) {
+        const double w = x[d] - (m.x * S.a[0 * 3 + d] + m.y * S.a[1 * 3 + d] + m.z * S.a[2 * 3 + d]);
+        lo[d] = w;
+        hi[d] = w;
+      }
+      klo = khi = key[a];
+    }
+    // the 8 lanes of a cluster are aligned in the wave
+    for (int o = 1; o < CL; o <<= 1) {
+      for (int d = 0; d < 3; d++) {
+        lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
+        hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
+      }
+      klo = fmin(klo, __shfl_xor(klo, o, 64));
+      khi = fmax(khi, __shfl_xor(khi, o, 64));
+    }
+    if ((t & (CL - 1)) == 0 && t < ncl * CL) {
+      const int c = c0 + t / CL;
+      if (cl_sys) cl_sys[c] = sy;
+      const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
+      origin[c] = make_double4(cx, cy, cz, 0.0);
+      bb0[2 * c] = make_float4((float)cx, (float)cy, (float)cz, 0.f);
+      bb0[2 * c + 1] = make_float4((float)(0.5 * (hi[0] - lo[0])), (float)(0.5 * (hi[1] - lo[1])),
+                                   (float)(0.5 * (hi[2] - lo[2])), 0.f);
+      kr[c] = make_float2((float)klo - 1.0e-6f, (float)khi + 1.0e-6f);
+    }
+  }
 }
 
 // absolute atom index of every slot of a group (two groups: an atom listed in
@@ -554,50 +617,6 @@ __global__ void k_abs_slot(int nslot, const int* __restrict__ ncl_dev, const int
   if (s >= nslot) return;
   const int a = slot_loc[s];
   abs_slot[s] = (a >= 0) ? (int)absidx[off + a] : -1 - s;
-}
-
-__global__ void k_cl_col(int ncol, const int* __restrict__ col_cl, int* __restrict__ cl_col) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncol) return;
-  for (int k = col_cl[c]; k < col_cl[c + 1]; k++) cl_col[k] = c;
-}
-
-// FP64 origin (bounding-box centre at the rebuild), rebuild bounding box,
-// range of fractional c and system of every cluster
-__global__ void k_origin(int ncl, const int* __restrict__ ncl_dev, const int* __restrict__ slot_loc,
-                         const int4* __restrict__ info, const double* __restrict__ key,
-                         const double* __restrict__ pos, int off, const SysArg* __restrict__ sys,
-                         const int* __restrict__ atom_sys, double4* __restrict__ origin, float4* __restrict__ bb0,
-                         float2* __restrict__ kr, int* __restrict__ cl_sys) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ncl_dev) ncl = *ncl_dev;
-  if (c >= ncl) return;
-  // the first slot of a cluster always holds an atom (padding comes last)
-  const int s = nsys_of(atom_sys, slot_loc[c * CL]);
-  const SysArg& S = sys[s];
-  if (cl_sys) cl_sys[c] = s;
-  double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-  double klo = DBL_MAX, khi = -DBL_MAX;
-  for (int k = 0; k < CL; k++) {
-    int a = slot_loc[c * CL + k];
-    if (a < 0) continue;
-    int4 m = info[a];
-    const double* x = pos + 3 * (size_t)(off + a);
-    for (int d = 0; d < 3; d++) {
-      double w = x[d] - (m.x * S.a[0 * 3 + d] + m.y * S.a[1 * 3 + d] + m.z * S.a[2 * 3 + d]);
-      lo[d] = fmin(lo[d], w);
-      hi[d] = fmax(hi[d], w);
-    }
-    klo = fmin(klo, key[a]);
-    khi = fmax(khi, key[a]);
-  }
-  double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
-  origin[c] = make_double4(cx, cy, cz, 0.0);
-  bb0[2 * c] = make_float4((float)cx, (float)cy, (float)cz, 0.f);
-  bb0[2 * c + 1] = make_float4((float)(0.5 * (hi[0] - lo[0])), (float)(0.5 * (hi[1] - lo[1])),
-                               (float)(0.5 * (hi[2] - lo[2])), 0.f);
-  // widened outwards so that the FP32 range always contains the FP64 keys
-  kr[c] = make_float2((float)klo - 1.0e-6f, (float)khi + 1.0e-6f);
 }
 
 // candidates kept per i cluster between the count and fill passes of the
@@ -1046,6 +1065,33 @@ __device__ __forceinline__ void sw_fixed(float invr0_2, float r2, float& sv, flo
   dfu = xp * res * res;
 }
 
+// x for a pair outside the cut: x^(P-1) stays finite and 1 + x^P overflows,
+// so that sv = 1/(1+x^P) and dfu = x^(P-1) sv^2 are exactly 0 (x = 2^ceil(128/P))
+template <int P>
+__device__ __forceinline__ constexpr float sw_far() {
+  return P == 1 ? __builtin_huge_valf()
+                : (P == 2 ? 0x1p64f : (P == 3 ? 0x1p43f : (P == 4 ? 0x1p32f : (P == 5 ? 0x1p26f : 0x1p22f))));
+}
+static_assert(sw_far<6>() == 0x1p22f, "far values cover P <= 6");
+
+template <int P>
+__device__ __forceinline__ void sw_fixed_x(float x, float& sv, float& dfu) {
+  float xp = 1.0f;
+#pragma unroll
+  for (int k = 1; k < P; k++) xp *= x;
+  const float res = __builtin_amdgcn_rcpf(fmaf(xp, x, 1.0f));
+  sv = res;
+  dfu = xp * res * res;
+}
+
+template <int P>
+__device__ __forceinline__ float sw_fixed_value_x(float x) {
+  float xp = 1.0f;
+#pragma unroll
+  for (int k = 1; k < P; k++) xp *= x;
+  return __builtin_amdgcn_rcpf(fmaf(xp, x, 1.0f));
+}
+
 template <int P>
 __device__ __forceinline__ float sw_fixed_value(float invr0_2, float r2) {
   const float x = r2 * invr0_2;
@@ -1067,6 +1113,7 @@ struct PairArgs {
   int items_cap, ent_cap; // capacities of the item and entry arrays
   int twogroup;
   float dmax2, prune2, band;
+  float cut_lo, cut_hi;   // FP32 r^2 at or below cut_lo: inside; in (cut_lo, cut_hi): the band
   float invr0_2;          // FP32 constant of fixedRational
   double sv_scale, sv_shift, df_scale;  // per-item FP32 sums -> outputs
   p2a_switch sw;          // FP64 parameters (generic switch and the cut band)
@@ -1148,35 +1195,41 @@ __device__ __forceinline__ void pair_term(const PairArgs& A, const P3& pj, const
   const float dy = pj.y - pi.y;
   const float dz = pj.z - pi.z;
   const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-  const float t = r2 - A.dmax2;  // exact sign: t <= 0 iff r2 <= dmax2
   bool keep = true;
   if constexpr (KIND == 0) keep = jj != ii;  // an atom is not paired with itself
   if constexpr (TWO) keep = keep && (L.sa[g + jj] != ai);
   if constexpr (VO && KIND == 0) keep = keep && jj > ii;  // value only: each pair once
-  // inside D_MAX and outside the band (t <= 0 and not |t| < band)
-  const bool cut = keep && t <= -A.band;
-  const bool inb = keep && fabsf(t) < A.band;
-  const unsigned long long bm = __ballot(inb);
+  // inside D_MAX and outside the band: r2 <= cut_lo; in the band: cut_lo < r2 < cut_hi
+  const bool cut = keep && r2 <= A.cut_lo;
+  const unsigned long long bcut = __ballot(cut);
+  const unsigned long long bm = __ballot(keep && r2 < A.cut_hi) & ~bcut;
   if (__builtin_expect(bm != 0ull, 0)) {
-    if (inb) {
+    if ((bm >> (ii + CL * jj)) & 1ull) {
       const int k = nrec + bits_below(bm);
       if (k < kBandSlots) A.recs[(size_t)item * kBandSlots + k] = (int)L.sm[g + jj] | (ii << 9) | (KIND << 12);
     }
     nrec += __popcll(bm);
   }
+  // a pair outside the cut gets sv = dfu = 0 from x = sw_far (fixed P) or by selection
   float sv, dfu;
-  if constexpr (VO && !GENERIC) {
-    sv = sw_fixed_value<P>(A.invr0_2, r2);
-    dfu = 0.f;
-  } else {
+  if constexpr (GENERIC) {
     pair_switch<P, GENERIC>(A, r2, sv, dfu);
+    sv = cut ? sv : 0.f;
+    dfu = cut ? dfu : 0.f;
+  } else {
+    const float x = cut ? r2 * A.invr0_2 : sw_far<P>();
+    if constexpr (VO) {
+      sv = sw_fixed_value_x<P>(x);
+      dfu = 0.f;
+    } else {
+      sw_fixed_x<P>(x, sv, dfu);
+    }
   }
   if constexpr (VO) {
-    npair += __popcll(__ballot(cut));
-    acc.sv += cut ? sv : 0.f;
+    npair += __popcll(bcut);
+    acc.sv += sv;
     return;
   }
-  dfu = cut ? dfu : 0.f;
   const float gx = dfu * dx, gy = dfu * dy, gz = dfu * dz;
   // derivative of atom i over every partner (the full list holds both
   // orientations of each cluster pair)
@@ -1185,8 +1238,8 @@ __device__ __forceinline__ void pair_term(const PairArgs& A, const P3& pj, const
   acc.fz += gz;
   if constexpr (KIND == 1) {
     // value and virial once per unordered pair
-    npair += __popcll(__ballot(cut));
-    acc.sv += cut ? sv : 0.f;
+    npair += __popcll(bcut);
+    acc.sv += sv;
     acc.vxx = fmaf(gx, dx, acc.vxx);
     acc.vxy = fmaf(gx, dy, acc.vxy);
     acc.vxz = fmaf(gx, dz, acc.vxz);
@@ -1226,8 +1279,10 @@ __device__ __forceinline__ void pair_groups(const PairArgs& A, int gb, int ge, c
   if (g < ge) pair_term<P, GENERIC, TWO, VO, KIND>(A, pa, pi, ii, jj, g, ai, L, item, nrec, npair, acc);
 }
 
-// The same evaluation with the FP64 cut decision taken on the spot for the
-// pairs in the band (the whole-item redo of the band kernel).
+// The band pairs of one group (the FP32 pass's test: cut_lo < r2 < cut_hi)
+// with the FP64 cut decision taken on the spot; every other pair adds
+// nothing (the item redo of the band kernel, for items whose band pairs
+// overflowed their record slots).
 template <int P, bool GENERIC, bool TWO, bool VO, int KIND>
 __device__ __forceinline__ void pair_term_exact(const PairArgs& A, const P3& pj, const float4& pi, int ii, int jj,
                                                 int g, int reqi, int ai, const WaveLds& L,
@@ -1237,13 +1292,12 @@ __device__ __forceinline__ void pair_term_exact(const PairArgs& A, const P3& pj,
   const float dy = pj.y - pi.y;
   const float dz = pj.z - pi.z;
   const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-  const float t = r2 - A.dmax2;
   bool keep = true;
   if constexpr (KIND == 0) keep = jj != ii;
   if constexpr (TWO) keep = keep && (L.sa[g + jj] != ai);
   if constexpr (VO && KIND == 0) keep = keep && jj > ii;
-  bool cut = keep && t <= 0.f;
-  if (keep && fabsf(t) < A.band) {
+  bool cut = false;
+  if (keep && r2 > A.cut_lo && r2 < A.cut_hi) {
     const int meta = L.sm[g + jj];  // (entry << 3 | slot), -1 for padding
     if (reqi >= 0 && meta >= 0) {
       const int cj = L.sc[meta >> 3] & kCjMask;
@@ -1276,6 +1330,30 @@ __device__ __forceinline__ void pair_term_exact(const PairArgs& A, const P3& pj,
     acc.vyz = fmaf(hy, dz, acc.vyz);
     acc.vzz = fmaf(hz, dz, acc.vzz);
   }
+}
+
+// Row entry `lane` (< 8) of FP32 per-lane sums: value, six virial products,
+// pair count (npair uniform); the wave sums are formed in every lane.
+template <bool VO>
+__device__ __forceinline__ double item_row(const PairArgs& A, const PairAcc& acc, int npair, int lane) {
+  float tv[7];
+  tv[0] = wave_sum_f(acc.sv);
+  if constexpr (!VO) {
+    tv[1] = wave_sum_f(acc.vxx);
+    tv[2] = wave_sum_f(acc.vxy);
+    tv[3] = wave_sum_f(acc.vxz);
+    tv[4] = wave_sum_f(acc.vyy);
+    tv[5] = wave_sum_f(acc.vyz);
+    tv[6] = wave_sum_f(acc.vzz);
+  } else {
+    tv[1] = tv[2] = tv[3] = tv[4] = tv[5] = tv[6] = 0.f;
+  }
+  float tq = tv[0];
+#pragma unroll
+  for (int q = 1; q < 7; q++)
+    if (lane == q) tq = tv[q];
+  return lane == 0 ? A.sv_scale * (double)tq + A.sv_shift * (double)npair
+                   : (lane < 7 ? A.df_scale * (double)tq : (double)npair);
 }
 
 // Staging of one work item by one wave (i cluster ci, entries [e0, e1)):
@@ -1387,13 +1465,15 @@ constexpr int stage_cap() {
 // staged atoms: lane l evaluates the pair (i = l&7 of cluster ci, j = staged
 // atom 8g + (l>>3)).  VO (value only): no derivatives and no virial;
 // derivative-only entries are skipped, so each unordered pair is visited
-// once.  Pairs within the band around D_MAX^2 are recorded: an item with up
-// to kBandSlots of them is queued in A.fix (its outputs are written here and
-// completed by the band kernel), an item with more in A.redo (its outputs
-// come from the band kernel alone).
+// once.  Pairs within the band around D_MAX^2 are left out and recorded: an
+// item with up to kBandSlots of them is queued in A.fix, an item with more
+// in A.redo, for the band kernel.
 // Outputs: fpart[item][8][3], the derivative of each i atom over the item,
-// and the value / virial / pair count rows described at PairArgs; the
-// workgroup rows are the FP64 sums of each wave's items, added in wave order.
+// and the value / virial / pair count rows described at PairArgs, all over
+// the pairs outside the band (the band kernel adds the band pairs).  For the
+// workgroup rows each lane keeps FP32 running sums over all of the wave's
+// items; their wave sums are taken once at the end, converted to FP64, and
+// the waves' rows are added in wave order.
 template <int P, bool GENERIC, bool TWO, bool VO, int CH>
 __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_pairs(
     PairArgs A, const int4* __restrict__ items, const Entry* __restrict__ ents, const float4* __restrict__ posI,
@@ -1412,7 +1492,8 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
   const int ii = lane & 7, jj = lane >> 3;
   const int nitems = A.nitems_dev ? min(*A.nitems_dev, A.items_cap) : A.nitems;
   const WaveLds L = {sj[w], sm[w], scode[w], TWO ? sabs[w] : nullptr};
-  if (lane < 8) stot[w][lane] = 0.0;
+  PairAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  int ntot = 0;
 
   // the next item's descriptor and entries are loaded while the current item
   // computes (a wave handles several items when the grid is persistent)
@@ -1447,7 +1528,6 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
     n2 = __builtin_amdgcn_readfirstlane(n2);
     b1 = __builtin_amdgcn_readfirstlane(b1);
     b2 = __builtin_amdgcn_readfirstlane(b2);
-    PairAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     int nrec = 0, npair = 0;
     if (n0) pair_term<P, GENERIC, TWO, VO, 0>(A, L.sj[jj], pi, ii, jj, 0, ai, L, item, nrec, npair, acc);
     pair_groups<P, GENERIC, TWO, VO, 1>(A, b1, b1 + n1, pi, ii, jj, ai, L, item, nrec, npair, acc);
@@ -1458,45 +1538,35 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
       else A.fix[atomicAdd(A.counts + 1, 1)] = make_int2(item, nrec);
       if (!A.item_partials) atomicOr(&A.bandmap[item >> 5], 1u << (item & 31));
     }
-    // an item redone whole gets all of its outputs from the band kernel
-    if (nrec <= kBandSlots) {
-      if constexpr (!VO) {
-        const float fx = sum_over_jj(acc.fx), fy = sum_over_jj(acc.fy), fz = sum_over_jj(acc.fz);
-        if (jj == 0) {
-          double* fp = fpart + (size_t)item * (CL * 3) + ii * 3;
-          fp[0] = -A.df_scale * (double)fx;
-          fp[1] = -A.df_scale * (double)fy;
-          fp[2] = -A.df_scale * (double)fz;
-        }
+    // the item's outputs over its pairs outside the band; the band kernel
+    // adds the band pairs
+    if constexpr (!VO) {
+      const float fx = sum_over_jj(acc.fx), fy = sum_over_jj(acc.fy), fz = sum_over_jj(acc.fz);
+      if (jj == 0) {
+        double* fp = fpart + (size_t)item * (CL * 3) + ii * 3;
+        fp[0] = -A.df_scale * (double)fx;
+        fp[1] = -A.df_scale * (double)fy;
+        fp[2] = -A.df_scale * (double)fz;
       }
-      float tv[7];
-      tv[0] = wave_sum_f(acc.sv);
-      if constexpr (!VO) {
-        tv[1] = wave_sum_f(acc.vxx);
-        tv[2] = wave_sum_f(acc.vxy);
-        tv[3] = wave_sum_f(acc.vxz);
-        tv[4] = wave_sum_f(acc.vyy);
-        tv[5] = wave_sum_f(acc.vyz);
-        tv[6] = wave_sum_f(acc.vzz);
-      } else {
-        tv[1] = tv[2] = tv[3] = tv[4] = tv[5] = tv[6] = 0.f;
-      }
-      if (lane < 8) {
-        // lane q forms row entry q: value, six virial products, pair count
-        float tq = tv[0];
-        for (int q = 1; q < 7; q++)
-          if (lane == q) tq = tv[q];
-        const double o = lane == 0 ? A.sv_scale * (double)tq + A.sv_shift * (double)npair
-                                   : (lane < 7 ? A.df_scale * (double)tq : (double)npair);
-        if (A.item_partials || nrec > 0) A.ipart[(size_t)item * 8 + lane] = o;
-        else stot[w][lane] += o;
-      }
+      acc.fx = acc.fy = acc.fz = 0.f;
+    }
+    if (A.item_partials) {
+      const double o = item_row<VO>(A, acc, npair, lane);
+      if (lane < 8) A.ipart[(size_t)item * 8 + lane] = o;
+      acc = PairAcc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    } else {
+      // value and virial stay in the lanes' running sums over the wave's
+      // items; an item with band pairs gets a row for the band kernel's terms
+      ntot += npair;
+      if (nrec > 0 && lane < 8) A.ipart[(size_t)item * 8 + lane] = 0.0;
     }
     // the next item overwrites the staging read above
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
   if (!A.item_partials) {
+    const double o = item_row<VO>(A, acc, ntot, lane);
+    if (lane < 8) stot[w][lane] = o;
     __syncthreads();
     if (w == 0 && lane < 8) {
       double s = 0.0;
@@ -1513,9 +1583,9 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
 //     Pbc::distance (exact_r2) and forms the pair's FP32 terms; the records'
 //     terms are added to the item's outputs in record order.
 // (2) Redo: the items of A.redo (more than kBandSlots band pairs), each
-//     shared by the four waves of a workgroup: the FP32 pass's staging and
-//     evaluation with the FP64 cut decision inline; the waves' partials are
-//     summed in wave order.  Its FP64 code needs ~50 more VGPRs than the FP32
+//     shared by the four waves of a workgroup: the FP32 pass's staging, then
+//     the band pairs with the FP64 cut decision inline; the waves' partials
+//     are summed in wave order and added to the item's outputs.  Its FP64 code needs ~50 more VGPRs than the FP32
 //     pass, which is why it is a separate kernel.
 template <int P, bool GENERIC, bool TWO, bool VO, int CH>
 __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(1))) void k_pairs_band(
@@ -1661,7 +1731,7 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(1)))
       if (!VO && lane < 3 * CL) {
         double s = 0.0;
         for (int v = 0; v < NW; v++) s += sred[v][lane];
-        fpart[(size_t)item * (CL * 3) + lane] = -A.df_scale * s;
+        fpart[(size_t)item * (CL * 3) + lane] += -A.df_scale * s;
       }
       if (lane < 8) {
         double s = 0.0;
@@ -1674,7 +1744,7 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(1)))
         } else if (lane < 7) {
           o = A.df_scale * s;
         }
-        A.ipart[(size_t)item * 8 + lane] = o;
+        A.ipart[(size_t)item * 8 + lane] += o;
       }
     }
     // the next item overwrites the staging and sred
@@ -2519,6 +2589,9 @@ inline void pair_args_switch(const p2a_switch& sw, PairArgs& A) {
   A.prune2 = (float)(pr * pr);
   // 1.8x the worst-case FP32 error of r^2 from cluster-local coordinates
   A.band = 4.0e-6f * A.dmax2;
+  // the band as two thresholds, each rounded outwards
+  A.cut_lo = std::nextafter((float)((double)A.dmax2 - (double)A.band), 0.0f);
+  A.cut_hi = std::nextafter((float)((double)A.dmax2 + (double)A.band), HUGE_VALF);
   A.invr0_2 = (float)sw.invr0_2;
   const bool fixed = sw.type == P2A_SW_RATIONAL_FIXED && sw.nn / 2 >= 1 && sw.nn / 2 <= 6;
   A.sv_scale = fixed ? sw.stretch : 1.0;
