@@ -54,8 +54,7 @@ struct p2a_batch {
   p2a::DBuf<double4> origin;
   p2a::DBuf<float4> bb0, bb, p;
   p2a::DBuf<float2> kr;
-  p2a::DBuf<int> ecount, estart, istart, totals, redo, recs, scan_bsum;
-  p2a::DBuf<int2> fix;
+  p2a::DBuf<int> ecount, estart, istart, totals, scan_bsum, bflag;
   p2a::DBuf<unsigned> ell;  // candidates the list count pass keeps for the fill pass
   p2a::DBuf<p2a::Entry> ents;
   p2a::DBuf<int4> items;
@@ -80,8 +79,7 @@ static int batch_reserve_lists(p2a_batch* b, int ent_cap, int items_cap) {
   if ((rc = b->ents.reserve(std::max(ent_cap, 1))) || (rc = b->items.reserve(std::max(items_cap, 1))) ||
       (rc = b->fpart.reserve((size_t)CL * 3 * std::max(items_cap, 1))) ||
       (rc = b->partial.reserve(8 * (size_t)std::max(items_cap, 1))) ||
-      (rc = b->redo.reserve(std::max(items_cap, 1))) || (rc = b->fix.reserve(std::max(items_cap, 1))) ||
-      (rc = b->recs.reserve((size_t)kBandSlots * std::max(items_cap, 1))))
+      (rc = b->bflag.reserve(std::max(items_cap, 1))))
     return rc;
   b->ent_cap = ent_cap;
   b->items_cap = items_cap;
@@ -115,6 +113,7 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
   L.mode = 0;
   L.ent_cap = b->ent_cap;
   L.rlist2 = (float)(b->rlist * b->rlist);
+  L.pscale = pack_scale(b->sw);
   const int wpb = 256 / WAVE;
   hipLaunchKernelGGL(k_list<0>, dim3(grid1(clc, wpb)), dim3(256), 0, s, L, b->cl_col.p, (const int*)b->cl_sys.p,
                      (const SysArg*)b->sys.p, (const SysArg*)b->sys.p, b->bb0.p, b->bb0.p, b->kr.p, b->kr.p,
@@ -127,7 +126,7 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
                      b->istart.p, b->items.p, chunk, b->items_cap);
   hipLaunchKernelGGL(k_pack_bbox, dim3(grid1(nslot, 256)), dim3(256), 0, s, (int)nslot, ncl_dev, b->slot_loc.p,
                      b->info.p, pos, 0, (const SysArg*)b->sys.p, (const int*)b->atom_sys.p, b->origin.p, b->p.p,
-                     b->bb.p, b->totals.p + 4, 2);
+                     b->bb.p, pack_scale(b->sw));
   int rc;
   if ((rc = launch_check("batch: list build"))) return rc;
   PairArgs A;
@@ -143,10 +142,7 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
   A.pos64 = pos;
   A.item_partials = 1;  // per-system sums need the items' own partials
   A.ipart = b->partial.p;
-  A.counts = b->totals.p + 4;  // cleared by k_pack_bbox
-  A.redo = b->redo.p;
-  A.fix = b->fix.p;
-  A.recs = b->recs.p;
+  A.bflag = b->bflag.p;
   int grid = grid1(std::max(b->items_cap, 1), PAIR_BLOCK / WAVE);
   if (b->tune.blocks_per_cu > 0) grid = std::min(grid, b->tune.blocks_per_cu * b->cu_count);
   launch_pairs(A, grid, s, false, chunk, b->items.p, b->ents.p, b->p.p, b->p.p, b->bb.p, b->bb.p, (const int*)nullptr,

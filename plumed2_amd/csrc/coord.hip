@@ -25,10 +25,7 @@ struct Dir {  // one cluster-pair list: i clusters of group I against group J
   DBuf<int4> items;
   DBuf<double> fpart;  // per item: derivative partials of the 8 i atoms
   DBuf<double> rpart;  // per item: per-atom coordination-number rows of the 8 i atoms
-  DBuf<double> ipart;  // per item with band pairs: value, virial, pair count
-  DBuf<int> redo;      // items redone whole with FP64 cut decisions
-  DBuf<int2> fix;      // items with a few band pairs, fixed up
-  DBuf<int> recs;      // band records, kBandSlots per item
+  DBuf<int> bflag;     // per item: band pairs found by the pair kernel
   DBuf<unsigned> ell;  // candidates the list count pass keeps for the fill pass
   int nitems = 0, item_offset = 0;
   int64_t nentries = 0;
@@ -78,8 +75,6 @@ struct p2a_coord {
   int cu_count = 256;
   p2a::DBuf<double> partial, red, stats, pos_buf, deriv_buf, out_buf;
   p2a::DBuf<int> scan_bsum;   // block sums of the list scans
-  p2a::DBuf<int> bandz;       // band counters (two per direction), then one bitmap of items per direction
-  int band_words[2] = {0, 0};
   p2a::DBuf<long long> abs_dev;
   int64_t rebuilds = 0;
   bool profiling = false;
@@ -230,6 +225,7 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     L.mode = (ngroups == 1) ? 0 : (d == 0 ? 1 : 2);
     L.ent_cap = 0;
     L.rlist2 = (float)(c->rlist * c->rlist);
+    L.pscale = pack_scale(c->spec.sw);
     if ((rc = D.ecount.reserve(GI.maxcl + 1)) || (rc = D.estart.reserve(GI.maxcl + 1)) ||
         (rc = D.istart.reserve(GI.maxcl + 1)) || (rc = c->scan_bsum.reserve(2 * (size_t)grid1(GI.maxcl, kScanBlock) + 2)) ||
         (rc = D.ell.reserve((size_t)GI.maxcl * kEll)))
@@ -258,9 +254,7 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     tot_entries += D.nentries;
     if ((rc = D.ents.reserve(std::max<int64_t>(D.nentries, 1))) || (rc = D.items.reserve(std::max(D.nitems, 1))) ||
         (rc = D.fpart.reserve((size_t)CL * 3 * std::max(D.nitems, 1))) ||
-        (rc = D.ipart.reserve(8 * (size_t)std::max(D.nitems, 1))) || (rc = D.redo.reserve(std::max(D.nitems, 1))) ||
-        (rc = D.fix.reserve(std::max(D.nitems, 1))) ||
-        (rc = D.recs.reserve((size_t)kBandSlots * std::max(D.nitems, 1))))
+        (rc = D.bflag.reserve(std::max(D.nitems, 1))))
       return rc;
     Ls[d].ent_cap = (int)D.ents.n;
     if (GI.ncl > 0) {
@@ -273,8 +267,6 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
     }
   }
   c->nentries = tot_entries;
-  for (int d = 0; d < 2; d++) c->band_words[d] = d < c->ndir ? (c->dir[d].nitems + 31) / 32 : 0;
-  if ((rc = c->bandz.reserve(4 + (size_t)c->band_words[0] + c->band_words[1]))) return rc;
   // absolute index per slot (two groups: pairs of one atom with itself are skipped)
   if (ngroups == 2) {
     for (int gi = 0; gi < 2; gi++) {
@@ -364,16 +356,16 @@ static int reimage(p2a_coord* c, const double* box9) {
     if (GI.ncl == 0) continue;
     hipLaunchKernelGGL(k_reimage, dim3(grid1(GI.ncl, wpb)), dim3(256), 0, s, GI.ncl, (const int*)D.estart.p,
                        (const int*)D.ecount.p, (int)D.ents.n, (const double4*)GI.origin.p,
-                       (const double4*)GJ.origin.p, B, D.ents.p);
+                       (const double4*)GJ.origin.p, B, pack_scale(c->spec.sw), D.ents.p);
   }
   std::memcpy(c->box_at_build, box9, sizeof(c->box_at_build));
   return launch_check("reimage");
 }
 
-// value / virial / pair count from nred rows of c->partial (and the band
-// items of ndir lists), in one workgroup and a fixed order; with
-// gather_dir >= 0 the same launch also forms that direction's derivatives
-static int reduce_partials(p2a_coord* c, int nred, int ndir, double* value_dev, double* virial_dev,
+// value / virial / pair count from nred rows of c->partial, in one workgroup
+// and a fixed order; with gather_dir >= 0 the same launch also forms that
+// direction's derivatives
+static int reduce_partials(p2a_coord* c, int nred, double* value_dev, double* virial_dev,
                            int gather_dir = -1, double* deriv_dev = nullptr, bool with_stats = true) {
   hipStream_t s = c->stream;
   int ngather = 0, ncl = 0;
@@ -395,12 +387,6 @@ static int reduce_partials(p2a_coord* c, int nred, int ndir, double* value_dev, 
   std::memset(&R, 0, sizeof(R));
   R.nred = nred;
   R.partial = c->partial.p;
-  R.ndir = ndir;
-  for (int d = 0; d < ndir; d++) {
-    R.bandmap[d] = reinterpret_cast<const unsigned*>(c->bandz.p + 4 + (d == 0 ? 0 : c->band_words[0]));
-    R.nwords[d] = c->band_words[d];
-    R.ipart[d] = c->dir[d].ipart.p;
-  }
   R.value = value_dev;
   R.virial = virial_dev;
   R.stats = with_stats ? c->stats.p : nullptr;
@@ -417,12 +403,9 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
   for (int gi = 0; gi < ngroups; gi++) {
     Group& G = c->g[gi];
     if (G.ncl == 0) continue;
-    // the first launch also clears the band counters and bitmaps
-    const bool first = gi == 0 || c->g[0].ncl == 0;
     hipLaunchKernelGGL(k_pack_bbox, dim3(grid1((size_t)G.ncl * CL, 256)), dim3(256), 0, s, G.ncl * CL,
                        (const int*)nullptr, G.slot_loc.p, G.info.p, pos_dev, G.off, (const SysArg*)G.sys.p,
-                       (const int*)nullptr, G.origin.p, G.p.p, G.bb.p, first ? c->bandz.p : (int*)nullptr,
-                       first ? 4 + c->band_words[0] + c->band_words[1] : 0);
+                       (const int*)nullptr, G.origin.p, G.p.p, G.bb.p, pack_scale(c->spec.sw));
   }
   PairArgs A;
   std::memset(&A, 0, sizeof(A));
@@ -471,12 +454,7 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     A.nitems = D.nitems;
     A.items_cap = (int)D.items.n;
     A.ent_cap = (int)D.ents.n;
-    A.counts = c->bandz.p + 2 * d;
-    A.bandmap = reinterpret_cast<unsigned*>(c->bandz.p + 4 + (d == 0 ? 0 : c->band_words[0]));
-    A.ipart = D.ipart.p;
-    A.redo = D.redo.p;
-    A.fix = D.fix.p;
-    A.recs = D.recs.p;
+    A.bflag = D.bflag.p;
     launch_pairs(A, grids[d], s, value_only, c->tune.chunk, D.items.p, D.ents.p, GI.p.p, GJ.p.p, GI.bb.p, GJ.bb.p,
                  GI.abs_slot.p, GJ.abs_slot.p, D.fpart.p, c->partial.p + 8 * (size_t)off);
     off += grids[d];
@@ -498,7 +476,7 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     hipLaunchKernelGGL(k_gather_reduce, dim3(ng), dim3(256), 0, s, ng, GI.ncl, (const int*)nullptr, GI.p.p,
                        D.istart.p, (int)D.items.n, D.fpart.p, deriv_dev, 0, R);
   }
-  if ((rc = reduce_partials(c, nred, ndir_run, value_dev, virial_dev, last, deriv_dev))) return rc;
+  if ((rc = reduce_partials(c, nred, value_dev, virial_dev, last, deriv_dev))) return rc;
   return launch_check("finalize");
 }
 
@@ -523,7 +501,7 @@ static int run_fallback(p2a_coord* c, const double* box9, const double* pos_dev,
                        pos_dev, c->abs_dev.p, deriv_dev, c->partial.p);
   }
   if ((rc = launch_check("fp64 pair kernel"))) return rc;
-  if ((rc = reduce_partials(c, std::max(nblocks, 1), 0, value_dev, virial_dev))) return rc;
+  if ((rc = reduce_partials(c, std::max(nblocks, 1), value_dev, virial_dev))) return rc;
   return launch_check("finalize");
 }
 
@@ -637,7 +615,7 @@ static int run_fallback_nl(p2a_coord* c, bool build, const double* box9, const d
                        (const int*)c->nl_nbr.p);
   }
   if ((rc = launch_check("fp64 pair kernel"))) return rc;
-  if ((rc = reduce_partials(c, std::max(nblocks, 1), 0, value_dev, virial_dev))) return rc;
+  if ((rc = reduce_partials(c, std::max(nblocks, 1), value_dev, virial_dev))) return rc;
   return launch_check("finalize");
 }
 
@@ -654,7 +632,7 @@ static int run_rows(p2a_coord* c, const double* box9, const double* pos_dev, dou
     if (G.ncl == 0) continue;
     hipLaunchKernelGGL(k_pack_bbox, dim3(grid1((size_t)G.ncl * CL, 256)), dim3(256), 0, s, G.ncl * CL,
                        (const int*)nullptr, G.slot_loc.p, G.info.p, pos_dev, G.off, (const SysArg*)G.sys.p,
-                       (const int*)nullptr, G.origin.p, G.p.p, G.bb.p, (int*)nullptr, 0);
+                       (const int*)nullptr, G.origin.p, G.p.p, G.bb.p, pack_scale(c->spec.sw));
   }
   PairArgs A;
   std::memset(&A, 0, sizeof(A));
@@ -722,7 +700,7 @@ static int run_rows(p2a_coord* c, const double* box9, const double* pos_dev, dou
                        D.istart.p, (int)D.items.n, D.fpart.p, vjp_dev, 0, R);
   }
   double* wdev = virial_dev ? virial_dev : c->out_buf.p + 1;
-  if ((rc = reduce_partials(c, nred, 0, c->out_buf.p, wdev, last, vjp_dev, false))) return rc;
+  if ((rc = reduce_partials(c, nred, c->out_buf.p, wdev, last, vjp_dev, false))) return rc;
   return launch_check("row derivatives");
 }
 

@@ -626,7 +626,7 @@ constexpr unsigned kEllCj = 0x3FFFFFFu;
 
 // list entry of i cluster ci for j cluster cj seen through image (mx,my,mz)
 __device__ __forceinline__ Entry make_entry(int mode, int ci, int cj, int mx, int my, int mz, const SysArg& SJ,
-                                            const double4* __restrict__ orgJ, const double4& oi) {
+                                            const double4* __restrict__ orgJ, const double4& oi, double pscale) {
   const double4 oj = orgJ[cj];
   const double Tx = mx * SJ.a[0] + my * SJ.a[3] + mz * SJ.a[6];
   const double Ty = mx * SJ.a[1] + my * SJ.a[4] + mz * SJ.a[7];
@@ -638,9 +638,9 @@ __device__ __forceinline__ Entry make_entry(int mode, int ci, int cj, int mx, in
   Entry en;
   en.code = cj | (((mx + 1) + 3 * (my + 1) + 9 * (mz + 1)) << kImgShift) | (counted ? kCountBit : 0) |
             (diag ? (int)0x80000000u : 0);
-  en.dx = (float)(oj.x + Tx - oi.x);
-  en.dy = (float)(oj.y + Ty - oi.y);
-  en.dz = (float)(oj.z + Tz - oi.z);
+  en.dx = (float)((oj.x + Tx - oi.x) * pscale);
+  en.dy = (float)((oj.y + Ty - oi.y) * pscale);
+  en.dz = (float)((oj.z + Tz - oi.z) * pscale);
   return en;
 }
 
@@ -650,7 +650,7 @@ __device__ __forceinline__ Entry make_entry(int mode, int ci, int cj, int mx, in
 __global__ __launch_bounds__(256) void k_reimage(int ncl, const int* __restrict__ estart,
                                                  const int* __restrict__ ecount, int ent_cap,
                                                  const double4* __restrict__ orgI, const double4* __restrict__ orgJ,
-                                                 BoxArg box, Entry* __restrict__ ents) {
+                                                 BoxArg box, double pscale, Entry* __restrict__ ents) {
   const int lane = threadIdx.x & (WAVE - 1);
   const int ci = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
   if (ci >= ncl) return;
@@ -664,9 +664,9 @@ __global__ __launch_bounds__(256) void k_reimage(int ncl, const int* __restrict_
     const int im = (en.code >> kImgShift) & 31;
     const int mx = im % 3 - 1, my = (im / 3) % 3 - 1, mz = im / 9 - 1;
     const double4 oj = orgJ[cj];
-    en.dx = (float)(oj.x + (mx * box.a[0] + my * box.a[3] + mz * box.a[6]) - oi.x);
-    en.dy = (float)(oj.y + (mx * box.a[1] + my * box.a[4] + mz * box.a[7]) - oi.y);
-    en.dz = (float)(oj.z + (mx * box.a[2] + my * box.a[5] + mz * box.a[8]) - oi.z);
+    en.dx = (float)((oj.x + (mx * box.a[0] + my * box.a[3] + mz * box.a[6]) - oi.x) * pscale);
+    en.dy = (float)((oj.y + (mx * box.a[1] + my * box.a[4] + mz * box.a[7]) - oi.y) * pscale);
+    en.dz = (float)((oj.z + (mx * box.a[2] + my * box.a[5] + mz * box.a[8]) - oi.z) * pscale);
     ents[e] = en;
   }
 }
@@ -684,6 +684,7 @@ struct ListArgs {
   int mode;             // 0 one group, 1 two groups (counting side), 2 two groups (other side)
   int ent_cap;          // capacity of the entry array (fill pass)
   float rlist2;
+  double pscale;        // scale of the entry offsets (pack_scale)
 };
 
 // Cluster-pair search: one wave per i cluster.  FILL=0 counts and keeps the
@@ -738,7 +739,7 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
         const unsigned pk = ell[(size_t)ci * kEll + e];
         const int cj = (int)(pk & kEllCj), im = (int)(pk >> 26);
         if (base + e < L.ent_cap)
-          out[base + e] = make_entry(L.mode, ci, cj, im % 3 - 1, (im / 3) % 3 - 1, im / 9 - 1, SJ, orgJ, oi);
+          out[base + e] = make_entry(L.mode, ci, cj, im % 3 - 1, (im / 3) % 3 - 1, im / 9 - 1, SJ, orgJ, oi, L.pscale);
       }
       return;
     }
@@ -825,7 +826,7 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
       if (pass) {
         const int k = n + __popcll(bal & ((1ull << lane) - 1));
         if (FILL) {
-          if (base + k < L.ent_cap) out[base + k] = make_entry(L.mode, ci, cj, mx, my, mz, SJ, orgJ, oi);
+          if (base + k < L.ent_cap) out[base + k] = make_entry(L.mode, ci, cj, mx, my, mz, SJ, orgJ, oi, L.pscale);
         } else if (k < kEll) {
           ell[(size_t)ci * kEll + k] = (unsigned)cj | ((unsigned)((mx + 1) + 3 * (my + 1) + 9 * (mz + 1)) << 26);
         }
@@ -861,16 +862,14 @@ __global__ void k_items(int ncl, const int* __restrict__ ncl_dev, const int* __r
 
 // k_pack and k_bbox in one pass: one thread per slot, the 8 slots of a
 // cluster are 8 adjacent lanes, which reduce the cluster's box by shuffles.
-// Also clears nzero ints at `zero` (the pair kernels' band counters and
-// bitmaps).
+// Positions relative to the cluster origin, times pscale (pack_scale).
 __global__ __launch_bounds__(256) void k_pack_bbox(int nslot, const int* __restrict__ ncl_dev,
                                                    const int* __restrict__ slot_loc, const int4* __restrict__ info,
                                                    const double* __restrict__ pos, int off,
                                                    const SysArg* __restrict__ sys, const int* __restrict__ atom_sys,
                                                    const double4* __restrict__ origin, float4* __restrict__ out,
-                                                   float4* __restrict__ bb, int* __restrict__ zero, int nzero) {
+                                                   float4* __restrict__ bb, double pscale) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int z = s; z < nzero; z += gridDim.x * blockDim.x) zero[z] = 0;
   if (ncl_dev) nslot = *ncl_dev * CL;
   // whole groups of 8 lanes stay together (nslot is a multiple of 8)
   if (s >= nslot) return;
@@ -889,7 +888,8 @@ __global__ __launch_bounds__(256) void k_pack_bbox(int nslot, const int* __restr
     const double* x = pos + 3 * (size_t)(off + a);
     double w[3];
     for (int d = 0; d < 3; d++) w[d] = x[d] - (m.x * S.a[0 * 3 + d] + m.y * S.a[1 * 3 + d] + m.z * S.a[2 * 3 + d]);
-    q = make_float4((float)(w[0] - o.x), (float)(w[1] - o.y), (float)(w[2] - o.z), __int_as_float(off + a));
+    q = make_float4((float)((w[0] - o.x) * pscale), (float)((w[1] - o.y) * pscale), (float)((w[2] - o.z) * pscale),
+                    __int_as_float(off + a));
     lo[0] = hi[0] = q.x;
     lo[1] = hi[1] = q.y;
     lo[2] = hi[2] = q.z;
@@ -1053,7 +1053,8 @@ __device__ __forceinline__ int bits_below(unsigned long long m) {
 // without its stretch: sv = 1/(1+x^P) and dfu = x^(P-1) sv^2 with x = r^2/r0^2.
 // The stretched value is stretch*sv + shift and dfunc = -2P/r0^2 stretch dfu
 // (SwitchingFunction.cpp:203-215); both factors are applied once per work
-// item, in FP64 (PairArgs::sv_scale, sv_shift, df_scale).
+// item, in FP64 (PairArgs::sv_scale, sv_shift, df_scale, vir_scale).  The
+// cluster positions of these switches are stored in units of R_0 (x = r^2).
 template <int P>
 __device__ __forceinline__ void sw_fixed(float invr0_2, float r2, float& sv, float& dfu) {
   const float x = r2 * invr0_2;
@@ -1101,11 +1102,9 @@ __device__ __forceinline__ float sw_fixed_value(float invr0_2, float r2) {
   return __builtin_amdgcn_rcpf(fmaf(xp, x, 1.0f));
 }
 
-// band records a wave holds in LDS before it evaluates them (an item with
-// more is evaluated again whole with FP64 cut decisions)
-// band records kept per work item; an item with more band pairs is redone
-// whole with FP64 cut decisions
-constexpr int kBandSlots = 8;
+// band records a wave holds in LDS at once: the band pairs of an item are
+// decided in rounds of this many (one per lane)
+constexpr int kBandSlots = 64;
 
 struct PairArgs {
   int nitems;
@@ -1114,23 +1113,20 @@ struct PairArgs {
   int twogroup;
   float dmax2, prune2, band;
   float cut_lo, cut_hi;   // FP32 r^2 at or below cut_lo: inside; in (cut_lo, cut_hi): the band
-  float invr0_2;          // FP32 constant of fixedRational
-  double sv_scale, sv_shift, df_scale;  // per-item FP32 sums -> outputs
+  float contact2;         // FP32 r^2 of DBL_EPSILON (the contact-matrix rule)
+  float invr0_2;          // FP32 constant of fixedRational (1: positions in units of R_0)
+  double r2_real;         // FP32 r^2 -> squared distance (generic switches)
+  double sv_scale, sv_shift, df_scale, vir_scale;  // per-item FP32 sums -> outputs
   p2a_switch sw;          // FP64 parameters (generic switch and the cut band)
   const PbcArg* pbc;      // device copies of the prepared cells (FP64 cut decisions)
   const int* cl_sys;      // system of each i cluster (null: one system)
   const double* pos64;    // FP64 positions of the requested atoms (band pairs)
   // Value, six virial products and pair count: with item_partials every item
-  // writes its own row of ipart; otherwise the items without band pairs add
-  // into their workgroup's row of `partial`, and the items with band pairs
-  // write their row of ipart and set their bit in bandmap.
+  // writes its own row of ipart; otherwise every workgroup writes one row of
+  // `partial` over all of its items.
   int item_partials;
   double* ipart;
-  unsigned* bandmap;      // cleared before the pass
-  int* counts;            // [0] items redone whole, [1] items fixed up; cleared before the pass
-  int* redo;              // items redone with FP64 cut decisions (capacity: items)
-  int2* fix;              // {item, band records} of the items fixed up (capacity: items)
-  int* recs;              // kBandSlots band records per item
+  int* bflag;             // band pairs of each item from a wave's 65th item on
 };
 
 // FP64 re-evaluation of the squared distance of one pair, used only when the
@@ -1166,50 +1162,62 @@ struct WaveLds {
   short* sm;    // their origin: entry << 3 | slot, -1 for padding
   int* sc;      // entry codes of the item
   int* sa;      // absolute index of each staged atom (two groups)
+  int* sr;      // band records of the current round (kBandSlots)
 };
+
+// a band record: staged index (bits 0-9), i slot (10-12), kind (13-14)
+__device__ __forceinline__ int band_record(int staged, int ii, int kind) {
+  return staged | (ii << 10) | (kind << 13);
+}
 
 // FP32 switch terms of one pair on its FP32 r^2
 template <int P, bool GENERIC>
 __device__ __forceinline__ void pair_switch(const PairArgs& A, float r2, float& sv, float& dfu) {
   if constexpr (GENERIC) {
     double dfd;
-    sv = (float)switch_calc_sqr(A.sw, (double)r2, dfd);
+    sv = (float)switch_calc_sqr(A.sw, (double)r2 * A.r2_real, dfd);
     dfu = (float)dfd;
   } else {
     sw_fixed<P>(A.invr0_2, r2, sv, dfu);
   }
 }
 
-// One group of 8 staged j atoms against the 8 i atoms of the item: lane
-// (ii, jj) takes i = ii, j = staged atom g + jj.  KIND is uniform over the
-// group: 0 = the i cluster with itself (slot order), 1 = entries that count
-// the pair (value, virial, derivative), 2 = derivative only.
-// A pair whose FP32 r^2 lies within `band` of D_MAX^2 is left out here and
-// recorded in the item's band slots; the band kernel completes the item with
-// the FP64 cut decision.  nrec (uniform) counts the item's band pairs.
-template <int P, bool GENERIC, bool TWO, bool VO, int KIND>
-__device__ __forceinline__ void pair_term(const PairArgs& A, const P3& pj, const float4& pi, int ii, int jj, int g,
-                                          int ai, const WaveLds& L, int item, int& nrec, int& npair,
-                                          PairAcc& acc) {
-  const float dx = pj.x - pi.x;
-  const float dy = pj.y - pi.y;
-  const float dz = pj.z - pi.z;
-  const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+// FP32 separation and r^2 of staged atom pj from i atom pi (the one sequence
+// of operations for a pair in the FP32 pass and in its band decisions)
+__device__ __forceinline__ float pair_d(const P3& pj, const float4& pi, float& dx, float& dy, float& dz) {
+  dx = pj.x - pi.x;
+  dy = pj.y - pi.y;
+  dz = pj.z - pi.z;
+  return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+}
+
+// whether lane (ii, jj) evaluates its pair in a group of kind KIND
+template <bool TWO, bool VO, int KIND>
+__device__ __forceinline__ bool pair_keep(const WaveLds& L, int g, int ii, int jj, int ai) {
   bool keep = true;
   if constexpr (KIND == 0) keep = jj != ii;  // an atom is not paired with itself
   if constexpr (TWO) keep = keep && (L.sa[g + jj] != ai);
   if constexpr (VO && KIND == 0) keep = keep && jj > ii;  // value only: each pair once
+  return keep;
+}
+
+// One group of 8 staged j atoms against the 8 i atoms of the item: lane
+// (ii, jj) takes i = ii, j = staged atom g + jj.  KIND is uniform over the
+// group: 0 = the i cluster with itself (slot order), 1 = entries that count
+// the pair (value, virial, derivative), 2 = derivative only.
+// A pair whose FP32 r^2 lies in the band around D_MAX^2 (cut_lo < r2 <
+// cut_hi) is left out here and counted in nrec (uniform); band_terms
+// decides the item's band pairs on FP64 distances after the pass.
+template <int P, bool GENERIC, bool TWO, bool VO, int KIND>
+__device__ __forceinline__ void pair_term(const PairArgs& A, const P3& pj, const float4& pi, int ii, int jj, int g,
+                                          int ai, const WaveLds& L, int& nrec, int& npair, PairAcc& acc) {
+  float dx, dy, dz;
+  const float r2 = pair_d(pj, pi, dx, dy, dz);
+  const bool keep = pair_keep<TWO, VO, KIND>(L, g, ii, jj, ai);
   // inside D_MAX and outside the band: r2 <= cut_lo; in the band: cut_lo < r2 < cut_hi
   const bool cut = keep && r2 <= A.cut_lo;
   const unsigned long long bcut = __ballot(cut);
-  const unsigned long long bm = __ballot(keep && r2 < A.cut_hi) & ~bcut;
-  if (__builtin_expect(bm != 0ull, 0)) {
-    if ((bm >> (ii + CL * jj)) & 1ull) {
-      const int k = nrec + bits_below(bm);
-      if (k < kBandSlots) A.recs[(size_t)item * kBandSlots + k] = (int)L.sm[g + jj] | (ii << 9) | (KIND << 12);
-    }
-    nrec += __popcll(bm);
-  }
+  nrec += __popcll(__ballot(keep && r2 < A.cut_hi) & ~bcut);
   // a pair outside the cut gets sv = dfu = 0 from x = sw_far (fixed P) or by selection
   float sv, dfu;
   if constexpr (GENERIC) {
@@ -1217,7 +1225,8 @@ __device__ __forceinline__ void pair_term(const PairArgs& A, const P3& pj, const
     sv = cut ? sv : 0.f;
     dfu = cut ? dfu : 0.f;
   } else {
-    const float x = cut ? r2 * A.invr0_2 : sw_far<P>();
+    // positions are in units of R_0: x = r^2
+    const float x = cut ? r2 : sw_far<P>();
     if constexpr (VO) {
       sv = sw_fixed_value_x<P>(x);
       dfu = 0.f;
@@ -1266,69 +1275,115 @@ __device__ __forceinline__ void pair_term(const PairArgs& A, const P3& pj, const
 // area has spare slots past the last group for that read).
 template <int P, bool GENERIC, bool TWO, bool VO, int KIND>
 __device__ __forceinline__ void pair_groups(const PairArgs& A, int gb, int ge, const float4& pi, int ii, int jj,
-                                            int ai, const WaveLds& L, int item, int& nrec, int& npair,
-                                            PairAcc& acc) {
+                                            int ai, const WaveLds& L, int& nrec, int& npair, PairAcc& acc) {
   int g = gb;
   P3 pa = L.sj[g + jj];
   for (; g + CL < ge; g += 2 * CL) {
     const P3 pb = L.sj[g + CL + jj];
-    pair_term<P, GENERIC, TWO, VO, KIND>(A, pa, pi, ii, jj, g, ai, L, item, nrec, npair, acc);
+    pair_term<P, GENERIC, TWO, VO, KIND>(A, pa, pi, ii, jj, g, ai, L, nrec, npair, acc);
     pa = L.sj[g + 2 * CL + jj];
-    pair_term<P, GENERIC, TWO, VO, KIND>(A, pb, pi, ii, jj, g + CL, ai, L, item, nrec, npair, acc);
+    pair_term<P, GENERIC, TWO, VO, KIND>(A, pb, pi, ii, jj, g + CL, ai, L, nrec, npair, acc);
   }
-  if (g < ge) pair_term<P, GENERIC, TWO, VO, KIND>(A, pa, pi, ii, jj, g, ai, L, item, nrec, npair, acc);
+  if (g < ge) pair_term<P, GENERIC, TWO, VO, KIND>(A, pa, pi, ii, jj, g, ai, L, nrec, npair, acc);
 }
 
-// The band pairs of one group (the FP32 pass's test: cut_lo < r2 < cut_hi)
-// with the FP64 cut decision taken on the spot; every other pair adds
-// nothing (the item redo of the band kernel, for items whose band pairs
-// overflowed their record slots).
-template <int P, bool GENERIC, bool TWO, bool VO, int KIND>
-__device__ __forceinline__ void pair_term_exact(const PairArgs& A, const P3& pj, const float4& pi, int ii, int jj,
-                                                int g, int reqi, int ai, const WaveLds& L,
-                                                const float4* __restrict__ posJ, const PbcArg* __restrict__ pbc,
-                                                int& npair, PairAcc& acc) {
-  const float dx = pj.x - pi.x;
-  const float dy = pj.y - pi.y;
-  const float dz = pj.z - pi.z;
-  const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-  bool keep = true;
-  if constexpr (KIND == 0) keep = jj != ii;
-  if constexpr (TWO) keep = keep && (L.sa[g + jj] != ai);
-  if constexpr (VO && KIND == 0) keep = keep && jj > ii;
-  bool cut = false;
-  if (keep && r2 > A.cut_lo && r2 < A.cut_hi) {
-    const int meta = L.sm[g + jj];  // (entry << 3 | slot), -1 for padding
-    if (reqi >= 0 && meta >= 0) {
-      const int cj = L.sc[meta >> 3] & kCjMask;
-      const int reqj = __float_as_int(posJ[cj * CL + (meta & 7)].w);
-      if (reqj >= 0) cut = switch_inside(A.sw, exact_r2(pbc, A.pos64, reqi, reqj));
+// The band test of pair_term again for one group, numbering the band pairs in
+// program order: those numbered [r0, r0 + kBandSlots) become the wave's band
+// records (one round of band_terms).
+template <bool TWO, bool VO, int KIND>
+__device__ __forceinline__ void band_collect(const PairArgs& A, const P3& pj, const float4& pi, int ii, int jj, int g,
+                                             int ai, const WaveLds& L, int r0, int& cnt) {
+  float dx, dy, dz;
+  const float r2 = pair_d(pj, pi, dx, dy, dz);
+  const bool keep = pair_keep<TWO, VO, KIND>(L, g, ii, jj, ai);
+  const bool cut = keep && r2 <= A.cut_lo;
+  const unsigned long long bm = __ballot(keep && r2 < A.cut_hi) & ~__ballot(cut);
+  if (bm != 0ull) {
+    if ((bm >> (ii + CL * jj)) & 1ull) {
+      const int k = cnt + bits_below(bm) - r0;
+      if (k >= 0 && k < kBandSlots) L.sr[k] = band_record(g + jj, ii, KIND);
     }
+    cnt += __popcll(bm);
   }
-  float sv, dfu;
-  pair_switch<P, GENERIC>(A, r2, sv, dfu);
-  if constexpr (VO) {
-    npair += __popcll(__ballot(cut));
-    acc.sv += cut ? sv : 0.f;
-    return;
-  }
-  dfu = cut ? dfu : 0.f;
-  const float gx = dfu * dx, gy = dfu * dy, gz = dfu * dz;
-  acc.fx += gx;
-  acc.fy += gy;
-  acc.fz += gz;
-  const bool c2 = KIND == 1 ? cut : (KIND == 0 ? (cut && jj > ii) : false);
-  if constexpr (KIND != 2) {
-    const float wgt = c2 ? 1.f : 0.f;
-    npair += __popcll(__ballot(c2));
-    acc.sv = fmaf(wgt, sv, acc.sv);
-    const float hx = wgt * gx, hy = wgt * gy, hz = wgt * gz;
-    acc.vxx = fmaf(hx, dx, acc.vxx);
-    acc.vxy = fmaf(hx, dy, acc.vxy);
-    acc.vxz = fmaf(hx, dz, acc.vxz);
-    acc.vyy = fmaf(hy, dy, acc.vyy);
-    acc.vyz = fmaf(hy, dz, acc.vyz);
-    acc.vzz = fmaf(hz, dz, acc.vzz);
+}
+
+// The band pairs of the staged item, in rounds of kBandSlots (record r in
+// lane r, collected by band_collect; the first round also counts them): the
+// inclusive cut decided on the reference's Pbc::distance
+// (exact_r2), and for the pairs inside, their FP32 terms from the FP32
+// pass's r^2, added in record order in FP64: the i atom derivatives to
+// fadd[3] (lanes jj == 0, atom ii), the row entries to radd (lane q < 8:
+// value, six virial products, pair count).
+template <int P, bool GENERIC, bool TWO, bool VO>
+__device__ __forceinline__ void band_terms(const PairArgs& A, const WaveLds& L, int ci,
+                                           const float4* __restrict__ posI, const float4* __restrict__ posJ,
+                                           const float4& pi_lane, int ii, int jj, int lane, int ai, int n0, int b1,
+                                           int n1, int b2, int n2, double* fadd, double& radd) {
+  const PbcArg* pbc = A.pbc + nsys_of(A.cl_sys, ci);
+  int nrec = 0;
+  for (int r0 = 0; r0 == 0 || r0 < nrec; r0 += kBandSlots) {
+    {
+      int cnt = 0;
+      if (n0) band_collect<TWO, VO, 0>(A, L.sj[jj], pi_lane, ii, jj, 0, ai, L, r0, cnt);
+      for (int g = b1; g < b1 + n1; g += CL) band_collect<TWO, VO, 1>(A, L.sj[g + jj], pi_lane, ii, jj, g, ai, L, r0, cnt);
+      if constexpr (!VO)
+        for (int g = b2; g < b2 + n2; g += CL)
+          band_collect<TWO, VO, 2>(A, L.sj[g + jj], pi_lane, ii, jj, g, ai, L, r0, cnt);
+      nrec = cnt;
+      // the records are read by other lanes of this wave
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    const int nr = min(kBandSlots, nrec - r0);
+    int rii = 0, cnt = 0;
+    float sv = 0.f, g[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f};
+    if (lane < nr) {
+      const int rec = L.sr[lane];
+      const int st = rec & 1023, kind = (rec >> 13) & 3;
+      rii = (rec >> 10) & 7;
+      const float4 pi = posI[ci * CL + rii];
+      const float r2 = pair_d(L.sj[st], pi, d[0], d[1], d[2]);
+      const int meta = L.sm[st];  // (entry << 3 | slot), -1 for padding
+      const int reqi = __float_as_int(pi.w);
+      const int reqj = meta >= 0 ? __float_as_int(posJ[(L.sc[meta >> 3] & kCjMask) * CL + (meta & 7)].w) : -1;
+      if (reqi >= 0 && reqj >= 0 && switch_inside(A.sw, exact_r2(pbc, A.pos64, reqi, reqj))) {
+        float dfu;
+        pair_switch<P, GENERIC>(A, r2, sv, dfu);
+        if constexpr (!VO) {
+          g[0] = dfu * d[0];
+          g[1] = dfu * d[1];
+          g[2] = dfu * d[2];
+        }
+        // a self pair counts once: the staged index of a self entry is its slot
+        cnt = (kind == 1 || (kind == 0 && st > rii)) ? 1 : 0;
+      } else {
+        sv = 0.f;
+      }
+    }
+    for (int r = 0; r < nr; r++) {
+      const int ri = __builtin_amdgcn_readlane(rii, r);
+      const int rc = __builtin_amdgcn_readlane(cnt, r);
+      const float rs = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sv), r));
+      float rg[3], rd[3];
+      for (int c = 0; c < 3; c++) {
+        rg[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g[c]), r));
+        rd[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d[c]), r));
+      }
+      if (!VO && jj == 0 && ii == ri)
+        for (int c = 0; c < 3; c++) fadd[c] += -A.df_scale * (double)rg[c];
+      if (rc && lane < 8) {
+        if (lane == 0) radd += A.sv_scale * (double)rs + A.sv_shift;
+        else if (lane == 7) radd += 1.0;
+        else if (!VO) {
+          const int a = lane <= 3 ? 0 : (lane <= 5 ? 1 : 2);
+          const int b = lane <= 3 ? lane - 1 : (lane <= 5 ? lane - 3 : 2);
+          radd += A.vir_scale * (double)(rg[a] * rd[b]);
+        }
+      }
+    }
+    // the next round overwrites the records
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -1353,7 +1408,7 @@ __device__ __forceinline__ double item_row(const PairArgs& A, const PairAcc& acc
   for (int q = 1; q < 7; q++)
     if (lane == q) tq = tv[q];
   return lane == 0 ? A.sv_scale * (double)tq + A.sv_shift * (double)npair
-                   : (lane < 7 ? A.df_scale * (double)tq : (double)npair);
+                   : (lane < 7 ? A.vir_scale * (double)tq : (double)npair);
 }
 
 // Staging of one work item by one wave (i cluster ci, entries [e0, e1)):
@@ -1465,17 +1520,17 @@ constexpr int stage_cap() {
 // staged atoms: lane l evaluates the pair (i = l&7 of cluster ci, j = staged
 // atom 8g + (l>>3)).  VO (value only): no derivatives and no virial;
 // derivative-only entries are skipped, so each unordered pair is visited
-// once.  Pairs within the band around D_MAX^2 are left out and recorded: an
-// item with up to kBandSlots of them is queued in A.fix, an item with more
-// in A.redo, for the band kernel.
+// once.  The pairs in the band around D_MAX^2 are decided at the end of
+// their item on FP64 distances.
 // Outputs: fpart[item][8][3], the derivative of each i atom over the item,
 // and the value / virial / pair count rows described at PairArgs, all over
-// the pairs outside the band (the band kernel adds the band pairs).  For the
-// workgroup rows each lane keeps FP32 running sums over all of the wave's
-// items; their wave sums are taken once at the end, converted to FP64, and
-// the waves' rows are added in wave order.
+// the pairs outside the band plus the FP64 terms of the band pairs inside
+// the cut (band_terms).  For the workgroup rows each lane keeps FP32 running
+// sums over all of the wave's items; their wave sums are taken once at the
+// end, converted to FP64 and added to the wave's band sums, and the waves'
+// rows are added in wave order.
 template <int P, bool GENERIC, bool TWO, bool VO, int CH>
-__global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_pairs(
+__global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_pairs(
     PairArgs A, const int4* __restrict__ items, const Entry* __restrict__ ents, const float4* __restrict__ posI,
     const float4* __restrict__ posJ, const float4* __restrict__ bbI, const float4* __restrict__ bbJ,
     const int* __restrict__ absI, const int* __restrict__ absJ, double* __restrict__ fpart,
@@ -1486,14 +1541,17 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
   __shared__ short sm[NW][CAP];
   __shared__ int scode[NW][CH];
   __shared__ int sabs[TWO ? NW : 1][TWO ? CAP : 1];
+  __shared__ int srec[NW][kBandSlots];
   __shared__ double stot[NW][8];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int ii = lane & 7, jj = lane >> 3;
   const int nitems = A.nitems_dev ? min(*A.nitems_dev, A.items_cap) : A.nitems;
-  const WaveLds L = {sj[w], sm[w], scode[w], TWO ? sabs[w] : nullptr};
+  const WaveLds L = {sj[w], sm[w], scode[w], TWO ? sabs[w] : nullptr, srec[w]};
   PairAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   int ntot = 0;
+  // lanes < 8: the wave's FP64 sums of the band pairs' row entries
+  if (lane < 8) stot[w][lane] = 0.0;
 
   // the next item's descriptor and entries are loaded while the current item
   // computes (a wave handles several items when the grid is persistent)
@@ -1506,7 +1564,9 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
     const int e = it_n.y + lane;
     if (lane < CH && e < min(it_n.z, A.ent_cap)) my_n = ents[e];
   }
-  for (; k < nitems; k += kstride) {
+  // the wave's items with band pairs: bit j for its j-th item (j < 64)
+  unsigned long long bandbits = 0ull;
+  for (int j = 0; k < nitems; k += kstride, j++) {
     const int item = k;
     const int4 it = it_n;
     const Entry my_e = my_n;
@@ -1529,17 +1589,15 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
     b1 = __builtin_amdgcn_readfirstlane(b1);
     b2 = __builtin_amdgcn_readfirstlane(b2);
     int nrec = 0, npair = 0;
-    if (n0) pair_term<P, GENERIC, TWO, VO, 0>(A, L.sj[jj], pi, ii, jj, 0, ai, L, item, nrec, npair, acc);
-    pair_groups<P, GENERIC, TWO, VO, 1>(A, b1, b1 + n1, pi, ii, jj, ai, L, item, nrec, npair, acc);
+    if (n0) pair_term<P, GENERIC, TWO, VO, 0>(A, L.sj[jj], pi, ii, jj, 0, ai, L, nrec, npair, acc);
+    pair_groups<P, GENERIC, TWO, VO, 1>(A, b1, b1 + n1, pi, ii, jj, ai, L, nrec, npair, acc);
     if constexpr (!VO)
-      pair_groups<P, GENERIC, TWO, VO, 2>(A, b2, b2 + n2, pi, ii, jj, ai, L, item, nrec, npair, acc);
-    if (nrec > 0 && lane == 0) {
-      if (nrec > kBandSlots) A.redo[atomicAdd(A.counts, 1)] = item;
-      else A.fix[atomicAdd(A.counts + 1, 1)] = make_int2(item, nrec);
-      if (!A.item_partials) atomicOr(&A.bandmap[item >> 5], 1u << (item & 31));
+      pair_groups<P, GENERIC, TWO, VO, 2>(A, b2, b2 + n2, pi, ii, jj, ai, L, nrec, npair, acc);
+    if (j < 64) {
+      if (nrec > 0) bandbits |= 1ull << j;
+    } else if (lane == 0) {
+      A.bflag[item] = nrec;
     }
-    // the item's outputs over its pairs outside the band; the band kernel
-    // adds the band pairs
     if constexpr (!VO) {
       const float fx = sum_over_jj(acc.fx), fy = sum_over_jj(acc.fy), fz = sum_over_jj(acc.fz);
       if (jj == 0) {
@@ -1555,200 +1613,72 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(5)))
       if (lane < 8) A.ipart[(size_t)item * 8 + lane] = o;
       acc = PairAcc{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     } else {
-      // value and virial stay in the lanes' running sums over the wave's
-      // items; an item with band pairs gets a row for the band kernel's terms
+      // value and virial stay in the lanes' running sums over the wave's items
       ntot += npair;
-      if (nrec > 0 && lane < 8) A.ipart[(size_t)item * 8 + lane] = 0.0;
     }
     // the next item overwrites the staging read above
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
+  // the band pairs of the wave's items (rare), item by item after the pass so
+  // that their FP64 registers are not held during it: the item is staged
+  // again and its band terms are added to its outputs in FP64 (each output
+  // by the lane that wrote it above)
+  for (int jb = 0, kb = blockIdx.x * NW + w; kb < nitems; jb++, kb += kstride) {
+    if (jb < 64) {
+      // straight to the next item with band pairs
+      const unsigned long long rest = bandbits >> jb;
+      if (rest == 0ull) {
+        jb = 63;
+        kb = blockIdx.x * NW + w + 63 * kstride;
+        continue;
+      }
+      const int skip = __builtin_ctzll(rest);
+      jb += skip;
+      kb += skip * kstride;
+    } else if (__builtin_amdgcn_readfirstlane(A.bflag[kb]) == 0) {
+      continue;
+    }
+    const int4 it = items[kb];
+    const int ci = it.x;
+    const int e1 = min(it.z, A.ent_cap);
+    const float4 pi = posI[ci * CL + ii];
+    const int ai = TWO ? absI[ci * CL + ii] : 0;
+    const float4 ic = bbI[2 * ci], ih = bbI[2 * ci + 1];
+    const int e = it.y + lane;
+    const Entry my_e = (lane < CH && e < e1) ? ents[e] : Entry{0, 0.f, 0.f, 0.f};
+    int n0, b1, n1, b2, n2;
+    stage_item<TWO, VO, CH>(A, my_e, lane, it.y, e1, ic, ih, posJ, bbJ, absJ, L, n0, b1, n1, b2, n2);
+    n1 = __builtin_amdgcn_readfirstlane(n1);
+    n2 = __builtin_amdgcn_readfirstlane(n2);
+    b1 = __builtin_amdgcn_readfirstlane(b1);
+    b2 = __builtin_amdgcn_readfirstlane(b2);
+    double fadd[3] = {0.0, 0.0, 0.0}, radd = 0.0;
+    band_terms<P, GENERIC, TWO, VO>(A, L, ci, posI, posJ, pi, ii, jj, lane, ai, n0, b1, n1, b2, n2, fadd, radd);
+    if constexpr (!VO) {
+      if (jj == 0) {
+        double* fp = fpart + (size_t)kb * (CL * 3) + ii * 3;
+        fp[0] += fadd[0];
+        fp[1] += fadd[1];
+        fp[2] += fadd[2];
+      }
+    }
+    if (lane < 8) {
+      if (A.item_partials) A.ipart[(size_t)kb * 8 + lane] += radd;
+      else stot[w][lane] += radd;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
   if (!A.item_partials) {
     const double o = item_row<VO>(A, acc, ntot, lane);
-    if (lane < 8) stot[w][lane] = o;
+    if (lane < 8) stot[w][lane] += o;
     __syncthreads();
     if (w == 0 && lane < 8) {
       double s = 0.0;
       for (int v = 0; v < NW; v++) s += stot[v][lane];
       partial[(size_t)blockIdx.x * 8 + lane] = s;
     }
-  }
-}
-
-// Band kernel, after the FP32 pass.
-// (1) Fix-up: one wave per item of A.fix.  Lane r < nrec takes band record r
-//     (program order of the FP32 pass), rebuilds the pair's FP32 r^2 exactly
-//     as the staging did, decides the cut on the reference's FP64
-//     Pbc::distance (exact_r2) and forms the pair's FP32 terms; the records'
-//     terms are added to the item's outputs in record order.
-// (2) Redo: the items of A.redo (more than kBandSlots band pairs), each
-//     shared by the four waves of a workgroup: the FP32 pass's staging, then
-//     the band pairs with the FP64 cut decision inline; the waves' partials
-//     are summed in wave order and added to the item's outputs.  Its FP64 code needs ~50 more VGPRs than the FP32
-//     pass, which is why it is a separate kernel.
-template <int P, bool GENERIC, bool TWO, bool VO, int CH>
-__global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(1))) void k_pairs_band(
-    PairArgs A, const int4* __restrict__ items, const Entry* __restrict__ ents, const float4* __restrict__ posI,
-    const float4* __restrict__ posJ, const float4* __restrict__ bbI, const float4* __restrict__ bbJ,
-    const int* __restrict__ absI, const int* __restrict__ absJ, double* __restrict__ fpart) {
-  constexpr int NW = PAIR_BLOCK / WAVE;
-  constexpr int CAP = stage_cap<CH>();
-  __shared__ P3 sj[NW][CAP];
-  __shared__ short sm[NW][CAP];
-  __shared__ int scode[NW][CH];
-  __shared__ int sabs[TWO ? NW : 1][TWO ? CAP : 1];
-  __shared__ double sred[NW][3 * CL + 8];
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  const int ii = lane & 7, jj = lane >> 3;
-
-  // (1) fix-up
-  const int nfix = A.counts[1];
-  for (int q = blockIdx.x * NW + w; q < nfix; q += gridDim.x * NW) {
-    const int2 fx = A.fix[q];
-    const int item = fx.x, nr = min(fx.y, kBandSlots);
-    const int4 it = items[item];
-    const int ci = it.x;
-    const PbcArg* pbc = A.pbc + nsys_of(A.cl_sys, ci);
-    int rii = 0, cnt = 0;
-    float sv = 0.f, g[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f};
-    if (lane < nr) {
-      const int rec = A.recs[(size_t)item * kBandSlots + lane];
-      const int meta = rec & 511, kind = rec >> 12;
-      rii = (rec >> 9) & 7;
-      const int e = it.y + (meta >> 3), t = meta & 7;
-      const Entry en = ents[e];
-      const int cj = en.code & kCjMask;
-      const float4 pi = posI[ci * CL + rii];
-      const float4 v = posJ[cj * CL + t];
-      // the same FP32 operations as stage_item and pair_term
-      const float qx = v.x + en.dx, qy = v.y + en.dy, qz = v.z + en.dz;
-      d[0] = qx - pi.x;
-      d[1] = qy - pi.y;
-      d[2] = qz - pi.z;
-      const float r2 = fmaf(d[0], d[0], fmaf(d[1], d[1], d[2] * d[2]));
-      if (switch_inside(A.sw, exact_r2(pbc, A.pos64, __float_as_int(pi.w), __float_as_int(v.w)))) {
-        float dfu;
-        pair_switch<P, GENERIC>(A, r2, sv, dfu);
-        if (!VO) {
-          g[0] = dfu * d[0];
-          g[1] = dfu * d[1];
-          g[2] = dfu * d[2];
-        }
-        // the slot of a self pair is its staged index
-        cnt = (kind == 1 || (kind == 0 && t > rii)) ? 1 : 0;
-      } else {
-        sv = 0.f;
-      }
-    }
-    // the records' terms in record order, each output by one lane
-    double add = 0.0;
-    for (int r = 0; r < nr; r++) {
-      const int ri = __builtin_amdgcn_readlane(rii, r);
-      const int rc = __builtin_amdgcn_readlane(cnt, r);
-      const float rs = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sv), r));
-      float rg[3], rd[3];
-      for (int c = 0; c < 3; c++) {
-        rg[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g[c]), r));
-        rd[c] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d[c]), r));
-      }
-      if (!VO && lane < 3 * CL && lane / 3 == ri) add += -A.df_scale * (double)rg[lane % 3];
-      if (rc && lane >= 3 * CL && lane < 3 * CL + 8) {
-        const int q2 = lane - 3 * CL;
-        if (q2 == 0) add += A.sv_scale * (double)rs + A.sv_shift;
-        else if (q2 == 7) add += 1.0;
-        else if (!VO) {
-          const int a = q2 <= 3 ? 0 : (q2 <= 5 ? 1 : 2);
-          const int b = q2 <= 3 ? q2 - 1 : (q2 <= 5 ? q2 - 3 : 2);
-          add += A.df_scale * (double)(rg[a] * rd[b]);
-        }
-      }
-    }
-    if (!VO && lane < 3 * CL) fpart[(size_t)item * (CL * 3) + lane] += add;
-    if (lane >= 3 * CL && lane < 3 * CL + 8) A.ipart[(size_t)item * 8 + lane - 3 * CL] += add;
-  }
-
-  // (2) redo: the waves of a workgroup share one item (uniform over the workgroup)
-  const int nredo = A.counts[0];
-  const WaveLds L = {sj[w], sm[w], scode[w], TWO ? sabs[w] : nullptr};
-  for (int k = blockIdx.x; k < nredo; k += gridDim.x) {
-    const int item = A.redo[k];
-    const int4 it = items[item];
-    const int ci = it.x;
-    const int e1 = min(it.z, A.ent_cap);
-    const float4 pi = posI[ci * CL + ii];
-    const int reqi = __float_as_int(pi.w);
-    const int ai = TWO ? absI[ci * CL + ii] : 0;
-    const float4 ic = bbI[2 * ci], ih = bbI[2 * ci + 1];
-    const PbcArg* pbc = A.pbc + nsys_of(A.cl_sys, ci);
-    Entry my_e = {0, 0.f, 0.f, 0.f};
-    if (lane < CH && it.y + lane < e1) my_e = ents[it.y + lane];
-    int n0, b1, n1, b2, n2;
-    stage_item<TWO, VO, CH>(A, my_e, lane, it.y, e1, ic, ih, posJ, bbJ, absJ, L, n0, b1, n1, b2, n2);
-    PairAcc acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    int npair = 0;
-    // every wave staged the whole item; wave w evaluates groups w, w + NW, ...
-    if (n0 && w == 0)
-      pair_term_exact<P, GENERIC, TWO, VO, 0>(A, L.sj[jj], pi, ii, jj, 0, reqi, ai, L, posJ, pbc, npair, acc);
-    int gi = n0 ? 1 : 0;
-    for (int g = b1; g < b1 + n1; g += CL, gi++)
-      if (gi % NW == w)
-        pair_term_exact<P, GENERIC, TWO, VO, 1>(A, L.sj[g + jj], pi, ii, jj, g, reqi, ai, L, posJ, pbc, npair,
-                                                acc);
-    if constexpr (!VO)
-      for (int g = b2; g < b2 + n2; g += CL, gi++)
-        if (gi % NW == w)
-          pair_term_exact<P, GENERIC, TWO, VO, 2>(A, L.sj[g + jj], pi, ii, jj, g, reqi, ai, L, posJ, pbc, npair,
-                                                  acc);
-    double* red = sred[w];
-    if constexpr (!VO) {
-      const float fx = sum_over_jj(acc.fx), fy = sum_over_jj(acc.fy), fz = sum_over_jj(acc.fz);
-      if (jj == 0) {
-        red[ii * 3] = (double)fx;
-        red[ii * 3 + 1] = (double)fy;
-        red[ii * 3 + 2] = (double)fz;
-      }
-    }
-    const float r0 = wave_sum_f(acc.sv);
-    float r6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if constexpr (!VO) {
-      r6[0] = wave_sum_f(acc.vxx);
-      r6[1] = wave_sum_f(acc.vxy);
-      r6[2] = wave_sum_f(acc.vxz);
-      r6[3] = wave_sum_f(acc.vyy);
-      r6[4] = wave_sum_f(acc.vyz);
-      r6[5] = wave_sum_f(acc.vzz);
-    }
-    if (lane == 0) {
-      red[3 * CL] = (double)r0;
-      for (int q = 0; q < 6; q++) red[3 * CL + 1 + q] = (double)r6[q];
-      red[3 * CL + 7] = (double)npair;
-    }
-    __syncthreads();
-    // the waves' partials summed in wave order
-    if (w == 0) {
-      if (!VO && lane < 3 * CL) {
-        double s = 0.0;
-        for (int v = 0; v < NW; v++) s += sred[v][lane];
-        fpart[(size_t)item * (CL * 3) + lane] += -A.df_scale * s;
-      }
-      if (lane < 8) {
-        double s = 0.0;
-        for (int v = 0; v < NW; v++) s += sred[v][3 * CL + lane];
-        double o = s;
-        if (lane == 0) {
-          double np = 0.0;
-          for (int v = 0; v < NW; v++) np += sred[v][3 * CL + 7];
-          o = A.sv_scale * s + A.sv_shift * np;
-        } else if (lane < 7) {
-          o = A.df_scale * s;
-        }
-        A.ipart[(size_t)item * 8 + lane] += o;
-      }
-    }
-    // the next item overwrites the staging and sred
-    __syncthreads();
   }
 }
 
@@ -1761,15 +1691,10 @@ __device__ __forceinline__ void write_virial(double* virial, const double* s6) {
 }
 
 // Inputs of the final value / virial / pair-count sum: the workgroup rows of
-// the pair kernels, then, per list direction, the rows of the items with band
-// pairs (their bits in bandmap, in item order).
+// the pair kernels.
 struct RedArgs {
   int nred;
   const double* partial;
-  int ndir;
-  const unsigned* bandmap[2];
-  int nwords[2];
-  const double* ipart[2];
   double* value;
   double* virial;
   double* stats;
@@ -1812,18 +1737,6 @@ __global__ __launch_bounds__(256) void k_gather_reduce(int ngather, int ncl, con
   }
   __shared__ double s[8][4];
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  // the first band bitmap words go out together with the first rows, so that
-  // the workgroup waits on memory about twice in all (rows and words, then the
-  // rows of band items, which are few)
-  constexpr int kPre = 4;
-  unsigned wpre[2][kPre];
-#pragma unroll
-  for (int d = 0; d < 2; d++)
-#pragma unroll
-    for (int u = 0; u < kPre; u++) {
-      const int w = t + u * 256;
-      wpre[d][u] = (d < R.ndir && w < R.nwords[d]) ? R.bandmap[d][w] : 0u;
-    }
   const double4* p4 = reinterpret_cast<const double4*>(R.partial);
   // rows t, t + 256, t + 512, ... of each thread, eight in flight at once
   for (int i0 = t; i0 < R.nred; i0 += 256 * 8) {
@@ -1843,34 +1756,6 @@ __global__ __launch_bounds__(256) void k_gather_reduce(int ngather, int ncl, con
     for (int u = 0; u < 8; u++) {
       acc[0] += x[u].x; acc[1] += x[u].y; acc[2] += x[u].z; acc[3] += x[u].w;
       acc[4] += y[u].x; acc[5] += y[u].y; acc[6] += y[u].z; acc[7] += y[u].w;
-    }
-  }
-  // items with band pairs, in item order per thread (words t, t + 256, ...)
-#pragma unroll
-  for (int d = 0; d < 2; d++) {
-    if (d >= R.ndir) break;
-    const unsigned* bm = R.bandmap[d];
-    const double* ip = R.ipart[d];
-    const int nw = R.nwords[d];
-    for (int w0 = t; w0 < nw; w0 += 256 * kPre) {
-      unsigned b[kPre];
-#pragma unroll
-      for (int u = 0; u < kPre; u++) {
-        const int w = w0 + u * 256;
-        b[u] = w0 == t ? wpre[d][u] : (w < nw ? bm[w] : 0u);
-      }
-#pragma unroll
-      for (int u = 0; u < kPre; u++) {
-        unsigned bits = b[u];
-        while (bits) {
-          const int item = (w0 + u * 256) * 32 + __builtin_ctz(bits);
-          bits &= bits - 1;
-          const double4* row = reinterpret_cast<const double4*>(ip + (size_t)item * 8);
-          const double4 x = row[0], y = row[1];
-          acc[0] += x.x; acc[1] += x.y; acc[2] += x.z; acc[3] += x.w;
-          acc[4] += y.x; acc[5] += y.y; acc[6] += y.z; acc[7] += y.w;
-        }
-      }
     }
   }
   // wave sums on the VALU, then the four waves in order
@@ -2306,7 +2191,7 @@ struct PairTuning {
 // reference's FP64 Pbc::distance.
 // Outputs: rpart[item][8] (row sums of the i atoms), and with WT fpart
 // [item][8][3] and per-workgroup rows (0, six box-derivative products, 0).
-constexpr float kContactEps2 = 2.220446049250313e-16f;  // DBL_EPSILON
+constexpr double kContactEps2 = 2.220446049250313e-16;  // DBL_EPSILON
 
 template <int P, bool GENERIC, bool TWO, bool WT, int KIND>
 __device__ __forceinline__ void row_term(const PairArgs& A, const P3& pj, const float4& pi, int ii, int jj, int g,
@@ -2318,7 +2203,7 @@ __device__ __forceinline__ void row_term(const PairArgs& A, const P3& pj, const 
   const float dz = pj.z - pi.z;
   const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
   const float t = r2 - A.dmax2;
-  bool keep = r2 >= kContactEps2;
+  bool keep = r2 >= A.contact2;
   if constexpr (KIND == 0) keep = keep && jj != ii;
   if constexpr (TWO) keep = keep && (L.sa[g + jj] != ai);
   bool cut = keep && t <= 0.f;
@@ -2444,7 +2329,7 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_rows(PairArgs A, const float* __
         float tq = tv[0];
         for (int q = 1; q < 6; q++)
           if (lane == q + 1) tq = tv[q];
-        stot[w][lane] += A.df_scale * (double)tq;
+        stot[w][lane] += A.vir_scale * (double)tq;
       }
     }
     // the next item overwrites the staging read above
@@ -2485,6 +2370,8 @@ __global__ void k_row_weights(int nrows, int natoms, const double* __restrict__ 
   if (k < natoms) wreq[k] = k < nrows ? (float)w[k] : 0.f;
 }
 
+inline void pair_args_generic(PairArgs& A);
+
 inline void launch_rows(const PairArgs& A, bool weighted, int grid, hipStream_t s, const float* wreq,
                         const int4* items, const Entry* ents, const float4* posI, const float4* posJ,
                         const float4* bbI, const float4* bbJ, const int* absI, const int* absJ, double* rpart,
@@ -2505,29 +2392,25 @@ inline void launch_rows(const PairArgs& A, bool weighted, int grid, hipStream_t 
   } while (0)
   if (sw.type == P2A_SW_RATIONAL_FIXED && sw.nn / 2 == 3) P2A_RSW(3, false);
   else if (sw.type == P2A_SW_RATIONAL_FIXED && sw.nn / 2 == 4) P2A_RSW(4, false);
-  else P2A_RSW(1, true);
+  else {
+    // the other switches run the generic evaluation
+    PairArgs G = A;
+    pair_args_generic(G);
+    const PairArgs& A = G;
+    P2A_RSW(1, true);
+  }
 #undef P2A_RSW
 #undef P2A_R
 }
 
-// workgroups of the band kernel: it loops over the few items the FP32 pass
-// queues (one wave per fixed-up item, one workgroup per item redone whole)
-constexpr int kBandGrid = 64;
-
-// launches of the pair kernels for a switch, group mode and tuning: the FP32
-// pass over all items, then the band kernel over the items it queued
-// (A.counts / A.redo / A.fix / A.recs / A.ipart / A.bandmap must be set)
+// launch of the pair kernel for a switch, group mode and tuning
 inline void launch_pairs(const PairArgs& A, int grid, hipStream_t s, bool value_only, int chunk, const int4* items,
                          const Entry* ents, const float4* posI, const float4* posJ, const float4* bbI,
                          const float4* bbJ, const int* absI, const int* absJ, double* fpart, double* partial) {
   const p2a_switch& sw = A.sw;
 #define P2A_K(P, GEN, TWO, VO, CH)                                                                                \
-  do {                                                                                                            \
-    hipLaunchKernelGGL((k_pairs<P, GEN, TWO, VO, CH>), dim3(grid), dim3(PAIR_BLOCK), 0, s, A, items, ents, posI,    \
-                       posJ, bbI, bbJ, absI, absJ, fpart, partial);                                              \
-    hipLaunchKernelGGL((k_pairs_band<P, GEN, TWO, VO, CH>), dim3(kBandGrid), dim3(PAIR_BLOCK), 0, s, A, items,     \
-                       ents, posI, posJ, bbI, bbJ, absI, absJ, fpart);                                           \
-  } while (0)
+  hipLaunchKernelGGL((k_pairs<P, GEN, TWO, VO, CH>), dim3(grid), dim3(PAIR_BLOCK), 0, s, A, items, ents, posI,      \
+                     posJ, bbI, bbJ, absI, absJ, fpart, partial)
 #define P2A_KCH(P, GEN, TWO, VO)          \
   do {                                    \
     if (chunk == 32) P2A_K(P, GEN, TWO, VO, 32); \
@@ -2578,26 +2461,51 @@ inline void launch_list_scans(hipStream_t s, int ncap, const int* n_dev, const i
                      out1);
 }
 
+// Scale of the FP32 cluster geometry (packed positions, list offsets,
+// boxes): 1/R_0 for the fixedRational switches of the pair kernel (x = r^2
+// then needs no multiply), 1 otherwise.
+inline bool pair_fixed(const p2a_switch& sw) {
+  return sw.type == P2A_SW_RATIONAL_FIXED && sw.nn / 2 >= 1 && sw.nn / 2 <= 6;
+}
+inline double pack_scale(const p2a_switch& sw) { return pair_fixed(sw) ? std::sqrt(sw.invr0_2) : 1.0; }
+
 // FP32 constants of the pair kernels, the cut band, and the FP64 factors
 // that turn the per-item FP32 sums into outputs: for fixedRational<N> the
 // kernels sum sv = 1/(1+x^(N/2)) and dfu = x^(N/2-1) sv^2, so that
 // value = stretch sum(sv) + shift npair and dfunc = -N/r0^2 stretch dfu
 // (SwitchingFunction.cpp:203-215); the generic switches sum s and dfunc.
+// The FP32 separations are pack_scale times the real ones, hence the
+// 1/pack_scale of the derivative factor and 1/pack_scale^2 of the virial's.
 inline void pair_args_switch(const p2a_switch& sw, PairArgs& A) {
-  A.dmax2 = (float)sw.dmax_2;
-  double pr = sw.dmax + 1e-4;
+  const double k = pack_scale(sw), k2 = k * k;
+  A.dmax2 = (float)(sw.dmax_2 * k2);
+  const double pr = (sw.dmax + 1e-4) * k;
   A.prune2 = (float)(pr * pr);
   // 1.8x the worst-case FP32 error of r^2 from cluster-local coordinates
   A.band = 4.0e-6f * A.dmax2;
   // the band as two thresholds, each rounded outwards
   A.cut_lo = std::nextafter((float)((double)A.dmax2 - (double)A.band), 0.0f);
   A.cut_hi = std::nextafter((float)((double)A.dmax2 + (double)A.band), HUGE_VALF);
-  A.invr0_2 = (float)sw.invr0_2;
-  const bool fixed = sw.type == P2A_SW_RATIONAL_FIXED && sw.nn / 2 >= 1 && sw.nn / 2 <= 6;
+  A.contact2 = (float)(kContactEps2 * k2);
+  const bool fixed = pair_fixed(sw);
+  A.invr0_2 = fixed ? 1.0f : (float)sw.invr0_2;
+  A.r2_real = 1.0 / k2;
   A.sv_scale = fixed ? sw.stretch : 1.0;
   A.sv_shift = fixed ? sw.shift : 0.0;
-  A.df_scale = fixed ? -(double)sw.nn * sw.invr0_2 * sw.stretch : 1.0;
+  const double dfs = fixed ? -(double)sw.nn * sw.invr0_2 * sw.stretch : 1.0;
+  A.df_scale = dfs / k;
+  A.vir_scale = dfs / k2;
   A.sw = sw;
+}
+
+// the factors of pair_args_switch for a kernel that evaluates the switch
+// generically (switch_calc_sqr) on the same geometry
+inline void pair_args_generic(PairArgs& A) {
+  const double k2 = 1.0 / A.r2_real;
+  A.sv_scale = 1.0;
+  A.sv_shift = 0.0;
+  A.df_scale = 1.0 / std::sqrt(k2);
+  A.vir_scale = 1.0 / k2;
 }
 
 }  // namespace

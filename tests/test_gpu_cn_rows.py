@@ -48,6 +48,7 @@ def uniform(n, rho=33.4, seed=0):
 
 
 @pytest.mark.parametrize("defn", ["RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2", "RATIONAL R_0=0.35 NN=8 MM=16 D_MAX=1.0",
+                                  "RATIONAL R_0=0.3 NN=10 MM=20 D_MAX=1.2", "RATIONAL R_0=0.3 NN=4 MM=8 D_MAX=1.2",
                                   "RATIONAL R_0=0.25 D_0=0.05 NN=5 MM=9 D_MAX=1.1", "EXP R_0=0.2 D_MAX=1.2"])
 def test_species_rows_and_product_match_oracle(defn):
     import plumed2_amd as P

@@ -336,6 +336,34 @@ def test_lattice_pairs_exactly_at_dmax(chunk):
     act.close()
 
 
+@pytest.mark.parametrize("chunk", [32, 64])
+def test_lattice_shell_of_thirty_at_dmax(chunk):
+    """Simple cubic lattice with D_MAX = 3 a: the 30 partners of every atom
+    at (3,0,0) a and (2,2,1) a sit exactly at D_MAX, so the items hold far
+    more band pairs than one round of band records (64) and the band pairs
+    are collected again for the later rounds.  Exact pair count, CV, virial
+    against the oracle; the forces vanish by symmetry, so they are checked
+    against the derivative of one nearest-neighbour pair."""
+    P = _P()
+    a, m = 0.1, 15
+    g = np.arange(m) * a
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) + 0.0137
+    n = len(x)
+    box = np.diag([m * a] * 3)
+    defn = f"RATIONAL R_0=0.12 NN=6 MM=12 D_MAX={3 * a}"
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{n} SWITCH={{{defn}}}")
+    act.set_tuning(chunk, 0)
+    v, d, w = act.calculate(x, box, 0)
+    assert act.stats()["path"] == 0
+    ov, od, ow, npair = O.coordination(O.Switch(defn), box, x, n, 0)
+    assert act.stats()["contributing_pairs"] == npair
+    assert abs(v - ov) <= CV_TOL * abs(ov)
+    fpair = abs(O.Switch(defn).dist(a)[1][0]) * a
+    assert np.max(np.abs(d - od)) <= 2 * F_TOL * fpair, np.max(np.abs(d - od)) / fpair
+    assert np.max(np.abs(w - ow)) <= W_TOL * np.max(np.abs(ow))
+    act.close()
+
+
 def test_dense_long_list_radius():
     """100 atoms/nm^3 with NL_CUTOFF=1.6: most i clusters have more list
     candidates than the count pass keeps, so the fill pass searches again
