@@ -160,7 +160,7 @@ class Coordination:
         """Time the pair kernels of every `every`-th call with HIP events."""
         A.check(A.lib().p2a_coord_set_profiling(self._ctx, int(every) if on else 0))
 
-    def set_tuning(self, chunk: int = 64, blocks_per_cu: int = 8):
+    def set_tuning(self, chunk: int = 64, blocks_per_cu: int = 4):
         """Entries per wave (32/64) and persistent grid of the pair kernel."""
         A.check(A.lib().p2a_coord_set_tuning(self._ctx, int(chunk), int(blocks_per_cu)))
 

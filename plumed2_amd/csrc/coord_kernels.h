@@ -786,52 +786,67 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
     s_img[wv][lane] = img;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    for (int u0 = 0; u0 < total; u0 += WAVE) {
-      const int u = u0 + lane;
-      bool pass = false;
-      int cj = 0, mx = 0, my = 0, mz = 0;
-      if (u < total) {
-        int a = 0, b = WAVE - 1;  // last column lane whose offset is <= u
-        while (a < b) {
-          const int q = (a + b + 1) >> 1;
-          if (s_off[wv][q] <= u) a = q;
-          else b = q - 1;
+    // two candidates per lane and trip: their box loads are in flight together
+    for (int u0 = 0; u0 < total; u0 += 2 * WAVE) {
+      int cj[2] = {0, 0}, mx[2] = {0, 0}, my[2] = {0, 0}, mz[2] = {0, 0};
+      float4 jc[2], jh[2];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int u = u0 + h * WAVE + lane;
+        jc[h] = jh[h] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (u < total) {
+          int a = 0, b = WAVE - 1;  // last column lane whose offset is <= u
+          while (a < b) {
+            const int q = (a + b + 1) >> 1;
+            if (s_off[wv][q] <= u) a = q;
+            else b = q - 1;
+          }
+          int r = u - s_off[wv][a];
+          const int n0 = s_n[wv][0][a], n1 = s_n[wv][1][a];
+          int m = 0;
+          if (r >= n0) {
+            r -= n0;
+            m = 1;
+            if (r >= n1) {
+              r -= n1;
+              m = 2;
+            }
+          }
+          cj[h] = s_first[wv][m][a] + r;
+          const int im = s_img[wv][a];
+          mx[h] = (im & 3) - 1;
+          my[h] = ((im >> 2) & 3) - 1;
+          mz[h] = m - 1;
+          jc[h] = bbJ[2 * cj[h]];
+          jh[h] = bbJ[2 * cj[h] + 1];
         }
-        int r = u - s_off[wv][a];
-        const int n0 = s_n[wv][0][a], n1 = s_n[wv][1][a];
-        int m = 0;
-        if (r >= n0) {
-          r -= n0;
-          m = 1;
-          if (r >= n1) {
-            r -= n1;
-            m = 2;
+      }
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int u = u0 + h * WAVE + lane;
+        bool pass = false;
+        if (u < total) {
+          const float Tx = mx[h] * SJ.af[0] + my[h] * SJ.af[3] + mz[h] * SJ.af[6];
+          const float Ty = mx[h] * SJ.af[1] + my[h] * SJ.af[4] + mz[h] * SJ.af[7];
+          const float Tz = mx[h] * SJ.af[2] + my[h] * SJ.af[5] + mz[h] * SJ.af[8];
+          const float gx = fmaxf(fabsf(jc[h].x + Tx - ic.x) - ih.x - jh[h].x, 0.f);
+          const float gy = fmaxf(fabsf(jc[h].y + Ty - ic.y) - ih.y - jh[h].y, 0.f);
+          const float gz = fmaxf(fabsf(jc[h].z + Tz - ic.z) - ih.z - jh[h].z, 0.f);
+          pass = gx * gx + gy * gy + gz * gz < L.rlist2;
+        }
+        const unsigned long long bal = __ballot(pass);
+        if (pass) {
+          const int k = n + __popcll(bal & ((1ull << lane) - 1));
+          if (FILL) {
+            if (base + k < L.ent_cap)
+              out[base + k] = make_entry(L.mode, ci, cj[h], mx[h], my[h], mz[h], SJ, orgJ, oi, L.pscale);
+          } else if (k < kEll) {
+            ell[(size_t)ci * kEll + k] =
+                (unsigned)cj[h] | ((unsigned)((mx[h] + 1) + 3 * (my[h] + 1) + 9 * (mz[h] + 1)) << 26);
           }
         }
-        cj = s_first[wv][m][a] + r;
-        const int im = s_img[wv][a];
-        mx = (im & 3) - 1;
-        my = ((im >> 2) & 3) - 1;
-        mz = m - 1;
-        const float Tx = mx * SJ.af[0] + my * SJ.af[3] + mz * SJ.af[6];
-        const float Ty = mx * SJ.af[1] + my * SJ.af[4] + mz * SJ.af[7];
-        const float Tz = mx * SJ.af[2] + my * SJ.af[5] + mz * SJ.af[8];
-        const float4 jc = bbJ[2 * cj], jh = bbJ[2 * cj + 1];
-        const float gx = fmaxf(fabsf(jc.x + Tx - ic.x) - ih.x - jh.x, 0.f);
-        const float gy = fmaxf(fabsf(jc.y + Ty - ic.y) - ih.y - jh.y, 0.f);
-        const float gz = fmaxf(fabsf(jc.z + Tz - ic.z) - ih.z - jh.z, 0.f);
-        pass = gx * gx + gy * gy + gz * gz < L.rlist2;
+        n += __popcll(bal);
       }
-      const unsigned long long bal = __ballot(pass);
-      if (pass) {
-        const int k = n + __popcll(bal & ((1ull << lane) - 1));
-        if (FILL) {
-          if (base + k < L.ent_cap) out[base + k] = make_entry(L.mode, ci, cj, mx, my, mz, SJ, orgJ, oi, L.pscale);
-        } else if (k < kEll) {
-          ell[(size_t)ci * kEll + k] = (unsigned)cj | ((unsigned)((mx + 1) + 3 * (my + 1) + 9 * (mz + 1)) << 26);
-        }
-      }
-      n += __popcll(bal);
     }
     // the next column chunk overwrites the tables read above
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -2171,7 +2186,7 @@ struct PairTuning {
   int chunk = 64;        // 32 or 64
   // 0: one wave per item; > 0: persistent grid of that many blocks per CU.
   // The persistent grid also bounds the workgroup rows of the final sum.
-  int blocks_per_cu = 8;
+  int blocks_per_cu = 4;  // the workgroups of the pair kernel one CU holds at once
 };
 
 // ------------------------------------------------------------------------
