@@ -54,7 +54,7 @@ struct p2a_batch {
   p2a::DBuf<double4> origin;
   p2a::DBuf<float4> bb0, bb, p;
   p2a::DBuf<float2> kr;
-  p2a::DBuf<int> ecount, estart, istart, totals, scan_bsum, bflag;
+  p2a::DBuf<int> ecount, estart, istart, totals, scan_bsum, bflag, wave_start;
   p2a::DBuf<unsigned> ell;  // candidates the list count pass keeps for the fill pass
   p2a::DBuf<p2a::Entry> ents;
   p2a::DBuf<int4> items;
@@ -79,7 +79,9 @@ static int batch_reserve_lists(p2a_batch* b, int ent_cap, int items_cap) {
   if ((rc = b->ents.reserve(std::max(ent_cap, 1))) || (rc = b->items.reserve(std::max(items_cap, 1))) ||
       (rc = b->fpart.reserve((size_t)CL * 3 * std::max(items_cap, 1))) ||
       (rc = b->partial.reserve(8 * (size_t)std::max(items_cap, 1))) ||
-      (rc = b->bflag.reserve(std::max(items_cap, 1))))
+      (rc = b->bflag.reserve(std::max(items_cap, 1))) ||
+      // the pair kernel's schedule: at most one wave per item (grid1(items_cap, 4) workgroups)
+      (rc = b->wave_start.reserve((size_t)grid1(std::max(items_cap, 1), PAIR_BLOCK / WAVE) * (PAIR_BLOCK / WAVE) + 1)))
     return rc;
   b->ent_cap = ent_cap;
   b->items_cap = items_cap;
@@ -144,7 +146,16 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
   A.ipart = b->partial.p;
   A.bflag = b->bflag.p;
   int grid = grid1(std::max(b->items_cap, 1), PAIR_BLOCK / WAVE);
+  A.wave_start = nullptr;
   if (b->tune.blocks_per_cu > 0) grid = std::min(grid, b->tune.blocks_per_cu * b->cu_count);
+  const int nw = grid * (PAIR_BLOCK / WAVE);
+  // contiguous item ranges of balanced cost per wave of the persistent grid
+  // (the first step runs before the list arrays exist: no items, no schedule)
+  if (b->tune.blocks_per_cu > 0 && b->items_cap > 0 && b->wave_start.p && b->wave_start.n >= (size_t)nw + 1) {
+    hipLaunchKernelGGL(k_wave_sched, dim3(grid1(std::max(b->items_cap, 1), 256)), dim3(256), 0, s, 0,
+                       (const int*)A.nitems_dev, b->items_cap, (const int4*)b->items.p, nw, b->wave_start.p);
+    A.wave_start = b->wave_start.p;
+  }
   launch_pairs(A, grid, s, false, chunk, b->items.p, b->ents.p, b->p.p, b->p.p, b->bb.p, b->bb.p, (const int*)nullptr,
                (const int*)nullptr, b->fpart.p, b->partial.p);
   if ((rc = launch_check("batch: pair kernel"))) return rc;

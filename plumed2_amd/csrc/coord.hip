@@ -26,6 +26,9 @@ struct Dir {  // one cluster-pair list: i clusters of group I against group J
   DBuf<double> fpart;  // per item: derivative partials of the 8 i atoms
   DBuf<double> rpart;  // per item: per-atom coordination-number rows of the 8 i atoms
   DBuf<int> bflag;     // per item: band pairs found by the pair kernel
+  DBuf<int> wave_start;  // the pair kernel's balanced schedule (k_wave_sched)
+  int sched_waves = -1;  // the grid it was made for
+  int64_t sched_gen = -1;  // and the list build
   DBuf<unsigned> ell;  // candidates the list count pass keeps for the fill pass
   int nitems = 0, item_offset = 0;
   int64_t nentries = 0;
@@ -455,6 +458,19 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     A.items_cap = (int)D.items.n;
     A.ent_cap = (int)D.ents.n;
     A.bflag = D.bflag.p;
+    A.wave_start = nullptr;
+    if (c->tune.blocks_per_cu > 0) {
+      // a persistent grid takes contiguous item ranges of balanced cost
+      const int nw = grids[d] * (PAIR_BLOCK / WAVE);
+      if (D.sched_waves != nw || D.sched_gen != c->rebuilds) {
+        if ((rc = D.wave_start.reserve((size_t)nw + 1))) return rc;
+        hipLaunchKernelGGL(k_wave_sched, dim3(grid1(std::max(D.nitems, 1), 256)), dim3(256), 0, s, D.nitems,
+                           (const int*)nullptr, (int)D.items.n, (const int4*)D.items.p, nw, D.wave_start.p);
+        D.sched_waves = nw;
+        D.sched_gen = c->rebuilds;
+      }
+      A.wave_start = D.wave_start.p;
+    }
     launch_pairs(A, grids[d], s, value_only, c->tune.chunk, D.items.p, D.ents.p, GI.p.p, GJ.p.p, GI.bb.p, GJ.bb.p,
                  GI.abs_slot.p, GJ.abs_slot.p, D.fpart.p, c->partial.p + 8 * (size_t)off);
     off += grids[d];

@@ -871,6 +871,34 @@ __global__ void k_items(int ncl, const int* __restrict__ ncl_dev, const int* __r
   }
 }
 
+// Balanced static schedule of the pair kernel: the items split into nwaves
+// contiguous ranges of about equal estimated cost, an item costing its
+// entries plus kItemCost (the staging and sums each item pays).  Item i
+// (cost prefix P_i, total C) goes to wave floor(P_i * nwaves / C); one
+// thread per item writes the first item of each wave it starts.
+// wave_start has nwaves + 1 entries; nitems = *nitems_dev when given.
+constexpr int kItemCost = 32;
+__global__ void k_wave_sched(int nitems, const int* __restrict__ nitems_dev, int items_cap,
+                             const int4* __restrict__ items, int nwaves, int* __restrict__ wave_start) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nitems_dev) nitems = min(*nitems_dev, items_cap);
+  if (nitems == 0) {
+    for (int w = i; w <= nwaves; w += gridDim.x * blockDim.x) wave_start[w] = 0;
+    return;
+  }
+  if (i >= nitems) return;
+  const long long total = (long long)items[nitems - 1].z + (long long)kItemCost * nitems;
+  auto wave_of = [&](int q) {
+    const long long pre = (long long)items[q].y + (long long)kItemCost * q;
+    return (int)min((long long)nwaves - 1, pre * nwaves / total);
+  };
+  const int wi = wave_of(i);
+  const int wp = i == 0 ? -1 : wave_of(i - 1);
+  for (int w = wp + 1; w <= wi; w++) wave_start[w] = i;
+  if (i == nitems - 1)
+    for (int w = wi + 1; w <= nwaves; w++) wave_start[w] = nitems;
+}
+
 // ------------------------------------------------------------------------
 // per-step kernels
 // ------------------------------------------------------------------------
@@ -1142,6 +1170,9 @@ struct PairArgs {
   int item_partials;
   double* ipart;
   int* bflag;             // band pairs of each item from a wave's 65th item on
+  // Items of each wave of the grid: [wave_start[g], wave_start[g + 1]) for
+  // wave g = blockIdx * 4 + wave (k_wave_sched); null: items g, g + waves, ...
+  const int* wave_start;
 };
 
 // FP64 re-evaluation of the squared distance of one pair, used only when the
@@ -1570,23 +1601,31 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 
   // the next item's descriptor and entries are loaded while the current item
   // computes (a wave handles several items when the grid is persistent)
-  const int kstride = gridDim.x * NW;
-  int k = blockIdx.x * NW + w;
+  // the wave's items: a contiguous range of balanced cost, or every
+  // (grid waves)-th item
+  const int gw = blockIdx.x * NW + w;
+  int kfirst = gw, kstride = gridDim.x * NW, kend = nitems;
+  if (A.wave_start) {
+    kfirst = A.wave_start[gw];
+    kend = min(A.wave_start[gw + 1], nitems);
+    kstride = 1;
+  }
+  int k = kfirst;
   int4 it_n = make_int4(0, 0, 0, 0);
   Entry my_n = {0, 0.f, 0.f, 0.f};
-  if (k < nitems) {
+  if (k < kend) {
     it_n = items[k];
     const int e = it_n.y + lane;
     if (lane < CH && e < min(it_n.z, A.ent_cap)) my_n = ents[e];
   }
   // the wave's items with band pairs: bit j for its j-th item (j < 64)
   unsigned long long bandbits = 0ull;
-  for (int j = 0; k < nitems; k += kstride, j++) {
+  for (int j = 0; k < kend; k += kstride, j++) {
     const int item = k;
     const int4 it = it_n;
     const Entry my_e = my_n;
     const int kn = k + kstride;
-    if (kn < nitems) it_n = items[kn];
+    if (kn < kend) it_n = items[kn];
     const int ci = it.x;
     const int e1 = min(it.z, A.ent_cap);
     const float4 pi = posI[ci * CL + ii];
@@ -1594,7 +1633,7 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
     const float4 ic = bbI[2 * ci], ih = bbI[2 * ci + 1];
     int n0, b1, n1, b2, n2;
     stage_item<TWO, VO, CH>(A, my_e, lane, it.y, e1, ic, ih, posJ, bbJ, absJ, L, n0, b1, n1, b2, n2);
-    if (kn < nitems) {
+    if (kn < kend) {
       const int e = it_n.y + lane;
       my_n = (lane < CH && e < min(it_n.z, A.ent_cap)) ? ents[e] : Entry{0, 0.f, 0.f, 0.f};
     }
@@ -1639,13 +1678,13 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
   // that their FP64 registers are not held during it: the item is staged
   // again and its band terms are added to its outputs in FP64 (each output
   // by the lane that wrote it above)
-  for (int jb = 0, kb = blockIdx.x * NW + w; kb < nitems; jb++, kb += kstride) {
+  for (int jb = 0, kb = kfirst; kb < kend; jb++, kb += kstride) {
     if (jb < 64) {
       // straight to the next item with band pairs
       const unsigned long long rest = bandbits >> jb;
       if (rest == 0ull) {
         jb = 63;
-        kb = blockIdx.x * NW + w + 63 * kstride;
+        kb = kfirst + 63 * kstride;
         continue;
       }
       const int skip = __builtin_ctzll(rest);
