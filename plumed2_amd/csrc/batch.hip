@@ -103,12 +103,12 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
   hipLaunchKernelGGL(k_bin, dim3(grid1(N, 256)), dim3(256), 0, s, N, pos, 0, (const SysArg*)b->sys.p,
                      (const int*)b->atom_sys.p, b->info.p, b->key.p, b->col_count.p);
   hipLaunchKernelGGL(k_scan_cols, dim3(1), dim3(1024), 0, s, ncol, b->col_count.p, b->col_start.p, b->col_cl.p,
-                     b->totals.p, b->cl_col.p);
+                     b->totals.p);
   hipLaunchKernelGGL(k_scatter, dim3(grid1(N, 256)), dim3(256), 0, s, N, b->info.p, b->col_start.p, b->fill.p,
                      b->tmp.p);
   hipLaunchKernelGGL(k_rank, dim3(ncol), dim3(256), 0, s, ncol, b->key.p, b->col_start.p, b->col_cl.p, b->tmp.p,
                      b->slot_loc.p, b->info.p, pos, 0, (const SysArg*)b->sys.p, (const int*)b->atom_sys.p,
-                     b->origin.p, b->bb0.p, b->kr.p, b->cl_sys.p);
+                     b->origin.p, b->bb0.p, b->kr.p, b->cl_sys.p, b->cl_col.p);
   ListArgs L;
   L.nclI = clc;
   L.nclI_dev = ncl_dev;

@@ -196,13 +196,13 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
                          (const int*)nullptr, G.info.p, G.key.p, G.col_count.p, c->totals.p + wslot,
                          c->totals.p + (25 - wslot));
     hipLaunchKernelGGL(k_scan_cols, dim3(1), dim3(1024), 0, s, G.ncol, G.col_count.p, G.col_start.p, G.col_cl.p,
-                       c->totals.p + 4 + 2 * gi, G.cl_col.p);
+                       c->totals.p + 4 + 2 * gi);
     if (G.n > 0) {
       hipLaunchKernelGGL(k_scatter, dim3(grid1(G.n, 256)), dim3(256), 0, s, G.n, G.info.p, G.col_start.p, G.fill.p,
                          G.tmp.p);
       hipLaunchKernelGGL(k_rank, dim3(G.ncol), dim3(256), 0, s, G.ncol, G.key.p, G.col_start.p, G.col_cl.p,
                          G.tmp.p, G.slot_loc.p, G.info.p, pos_dev, G.off, (const SysArg*)G.sys.p,
-                         (const int*)nullptr, G.origin.p, G.bb0.p, G.kr.p, (int*)nullptr);
+                         (const int*)nullptr, G.origin.p, G.bb0.p, G.kr.p, (int*)nullptr, G.cl_col.p);
     }
     if ((rc = launch_check("rebuild: binning"))) return rc;
   }

@@ -149,6 +149,36 @@ __device__ __forceinline__ int wave_excl_scan(int v, int lane, int& total) {
   return x - v;
 }
 
+// exclusive prefix sums of a and c over the workgroup (at most 1024 threads):
+// wave scans, then the wave totals from LDS (sw: 32 ints); ta, tc receive
+// the workgroup totals
+__device__ __forceinline__ void block_excl_scan2(int& a, int& c, int& ta, int& tc, int* sw) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  int wa, wc;
+  const int ea = wave_excl_scan(a, lane, wa);
+  const int ec = wave_excl_scan(c, lane, wc);
+  if (lane == 0) {
+    sw[w] = wa;
+    sw[16 + w] = wc;
+  }
+  __syncthreads();
+  int pa = 0, pc = 0;
+  ta = 0;
+  tc = 0;
+  for (int q = 0; q < nw; q++) {
+    const int xa = sw[q], xc = sw[16 + q];
+    if (q < w) {
+      pa += xa;
+      pc += xc;
+    }
+    ta += xa;
+    tc += xc;
+  }
+  a = pa + ea;
+  c = pc + ec;
+  __syncthreads();
+}
+
 __device__ __forceinline__ int floordiv(int a, int n) { return (a >= 0) ? a / n : -((-a + n - 1) / n); }
 
 __device__ __forceinline__ int nsys_of(const int* idx, int i) { return idx ? idx[i] : 0; }
@@ -237,44 +267,39 @@ __global__ __launch_bounds__(256) void k_bin(int n, const double* __restrict__ p
 }
 
 // exclusive scans of column counts (atoms) and of ceil(count/8) (clusters),
-// one workgroup
-// (with cl_col, also the column of every cluster)
-__global__ void k_scan_cols(int ncol, const int* __restrict__ cnt, int* __restrict__ col_start,
-                            int* __restrict__ col_cl, int* __restrict__ totals, int* __restrict__ cl_col = nullptr) {
-  __shared__ int sa[1024], sc[1024];
+// one workgroup of 1024 threads, a contiguous run of columns per thread
+__global__ __launch_bounds__(1024) void k_scan_cols(int ncol, const int* __restrict__ cnt,
+                                                    int* __restrict__ col_start, int* __restrict__ col_cl,
+                                                    int* __restrict__ totals) {
+  __shared__ int sw[32];
   const int t = threadIdx.x, T = blockDim.x;
-  int chunk = (ncol + T - 1) / T;
-  int b = t * chunk, e = min(ncol, b + chunk);
+  const int per = (ncol + T - 1) / T;
+  const int b = min(ncol, t * per), e = min(ncol, b + per);
   int la = 0, lc = 0;
-  for (int i = b; i < e; i++) {
-    la += cnt[i];
-    lc += (cnt[i] + CL - 1) / CL;
+  for (int i0 = b; i0 < e; i0 += 8) {
+    int v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = (i0 + k < e) ? cnt[i0 + k] : 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      la += v[k];
+      lc += (v[k] + CL - 1) / CL;
+    }
   }
-  sa[t] = la;
-  sc[t] = lc;
-  __syncthreads();
-  for (int o = 1; o < T; o <<= 1) {
-    int va = (t >= o) ? sa[t - o] : 0, vc = (t >= o) ? sc[t - o] : 0;
-    __syncthreads();
-    sa[t] += va;
-    sc[t] += vc;
-    __syncthreads();
-  }
-  int ra = sa[t] - la, rc = sc[t] - lc;
+  int ra = la, rc = lc, ta, tc;
+  block_excl_scan2(ra, rc, ta, tc, sw);
   for (int i = b; i < e; i++) {
+    const int v = cnt[i];
     col_start[i] = ra;
     col_cl[i] = rc;
-    const int ncl = (cnt[i] + CL - 1) / CL;
-    if (cl_col)
-      for (int k = 0; k < ncl; k++) cl_col[rc + k] = i;
-    ra += cnt[i];
-    rc += ncl;
+    ra += v;
+    rc += (v + CL - 1) / CL;
   }
-  if (t == T - 1) {
-    col_start[ncol] = sa[t];
-    col_cl[ncol] = sc[t];
-    totals[0] = sa[t];
-    totals[1] = sc[t];
+  if (t == 0) {
+    col_start[ncol] = ta;
+    col_cl[ncol] = tc;
+    totals[0] = ta;
+    totals[1] = tc;
   }
 }
 
@@ -301,46 +326,43 @@ __device__ __forceinline__ int block_excl_scan256(int v, int* sh, int& total) {
   return r;
 }
 
-// the same two scans in one workgroup of 1024 threads (contiguous chunks per
-// thread), for counts that one workgroup scans faster than three launches
-constexpr int kScanOneMax = 1 << 16;
+// the same two scans in one workgroup of 1024 threads (a contiguous run of at
+// most 8 counts per thread), for counts that one workgroup scans faster than
+// three launches
+constexpr int kScanOneMax = 8192;
 __global__ __launch_bounds__(1024) void k_scan2_one(int n, const int* __restrict__ n_dev, const int* __restrict__ in,
                                                     int chunk, int* __restrict__ out0, int* __restrict__ out1,
                                                     int* __restrict__ totals, int slot0, int slot1) {
-  __shared__ int sa[1024], sc[1024];
+  __shared__ int sw[32];
   if (n_dev) n = *n_dev;
   const int t = threadIdx.x, T = blockDim.x;
-  const int per = (n + T - 1) / T;
+  const int per = (n + T - 1) / T;  // <= 8: n <= kScanOneMax
   const int b = min(n, t * per), e = min(n, b + per);
+  int v[8];
   int la = 0, lc = 0;
-  for (int i = b; i < e; i++) {
-    const int v = in[i];
-    la += v;
-    lc += (v + chunk - 1) / chunk;
+#pragma unroll
+  for (int k = 0; k < 8; k++) v[k] = (b + k < e) ? in[b + k] : 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    la += v[k];
+    lc += (v[k] + chunk - 1) / chunk;
   }
-  sa[t] = la;
-  sc[t] = lc;
-  __syncthreads();
-  for (int o = 1; o < T; o <<= 1) {
-    const int va = (t >= o) ? sa[t - o] : 0, vc = (t >= o) ? sc[t - o] : 0;
-    __syncthreads();
-    sa[t] += va;
-    sc[t] += vc;
-    __syncthreads();
+  int ra = la, rc = lc, ta, tc;
+  block_excl_scan2(ra, rc, ta, tc, sw);
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    if (b + k < e) {
+      out0[b + k] = ra;
+      out1[b + k] = rc;
+      ra += v[k];
+      rc += (v[k] + chunk - 1) / chunk;
+    }
   }
-  int ra = sa[t] - la, rc = sc[t] - lc;
-  for (int i = b; i < e; i++) {
-    const int v = in[i];
-    out0[i] = ra;
-    out1[i] = rc;
-    ra += v;
-    rc += (v + chunk - 1) / chunk;
-  }
-  if (t == T - 1) {
-    out0[n] = sa[t];
-    out1[n] = sc[t];
-    totals[slot0] = sa[t];
-    totals[slot1] = sc[t];
+  if (t == 0) {
+    out0[n] = ta;
+    out1[n] = tc;
+    totals[slot0] = ta;
+    totals[slot1] = tc;
   }
 }
 
@@ -518,7 +540,8 @@ __global__ __launch_bounds__(256) void k_rank(int ncol, const double* __restrict
                                               const int4* __restrict__ info, const double* __restrict__ pos, int off,
                                               const SysArg* __restrict__ sys, const int* __restrict__ atom_sys,
                                               double4* __restrict__ origin, float4* __restrict__ bb0,
-                                              float2* __restrict__ kr, int* __restrict__ cl_sys) {
+                                              float2* __restrict__ kr, int* __restrict__ cl_sys,
+                                              int* __restrict__ cl_col) {
   __shared__ __align__(16) double sk[kRankLds];
   __shared__ __align__(16) int si[kRankLds];
   __shared__ int ss[kRankLds];
@@ -527,6 +550,7 @@ __global__ __launch_bounds__(256) void k_rank(int ncol, const double* __restrict
   const int b = col_start[col], n = col_start[col + 1] - b;
   const int c0 = col_cl[col], ncl = col_cl[col + 1] - c0;
   const int base = c0 * CL;
+  for (int q = threadIdx.x; q < ncl; q += blockDim.x) cl_col[c0 + q] = col;
   const int n4 = (n + 3) & ~3;
   const bool lds = n4 <= kRankLds;
   if (lds) {
