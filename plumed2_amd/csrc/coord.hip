@@ -42,6 +42,10 @@ struct Group {
   int maxcl = 0;   // cluster capacity: n/8 + one partial cluster per column
   int ncol = 1;
   int ncl = 0;  // clusters (device total copied at rebuild)
+  // slot_loc of the last build (order_nslot slots at order_ptr) orders the
+  // atoms of the next one; 0 = no such order
+  int order_nslot = 0;
+  const int* order_ptr = nullptr;
   DBuf<SysArg> sys;
   DBuf<int4> info;
   DBuf<double> key;
@@ -189,17 +193,27 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
   const int wslot = 12 + (int)(c->rebuilds & 1);
   for (int gi = 0; gi < ngroups; gi++) {
     Group& G = c->g[gi];
+    // the atoms are binned in the slot order of the last build when it is
+    // still in slot_loc (atoms move little between builds, so a workgroup's
+    // column counts stay in LDS); k_rank then writes every slot of the new
+    // clusters, padding included
+    const bool use_order = G.order_nslot > 0 && G.order_ptr == G.slot_loc.p;
+    const int norder = use_order ? G.order_nslot : 0;
+    const int* order = use_order ? G.slot_loc.p : nullptr;
+    const int nthr = use_order ? norder : G.n;
+    G.order_nslot = 0;
     hipLaunchKernelGGL(k_fill_int3, dim3(std::max(1, std::min(grid1(G.slot_loc.n, 256), 1024))), dim3(256), 0, s,
-                       G.col_count.p, G.ncol + 1, 0, G.fill.p, G.ncol + 1, 0, G.slot_loc.p, (int)G.slot_loc.n, -1);
+                       G.col_count.p, G.ncol + 1, 0, G.fill.p, G.ncol + 1, 0, G.slot_loc.p,
+                       use_order ? 0 : (int)G.slot_loc.n, -1);
     if (G.n > 0)
-      hipLaunchKernelGGL(k_bin, dim3(grid1(G.n, 256)), dim3(256), 0, s, G.n, pos_dev, G.off, (const SysArg*)G.sys.p,
+      hipLaunchKernelGGL(k_bin, dim3(grid1(nthr, 256)), dim3(256), 0, s, G.n, pos_dev, G.off, (const SysArg*)G.sys.p,
                          (const int*)nullptr, G.info.p, G.key.p, G.col_count.p, c->totals.p + wslot,
-                         c->totals.p + (25 - wslot));
+                         c->totals.p + (25 - wslot), order, norder);
     hipLaunchKernelGGL(k_scan_cols, dim3(1), dim3(1024), 0, s, G.ncol, G.col_count.p, G.col_start.p, G.col_cl.p,
                        c->totals.p + 4 + 2 * gi);
     if (G.n > 0) {
-      hipLaunchKernelGGL(k_scatter, dim3(grid1(G.n, 256)), dim3(256), 0, s, G.n, G.info.p, G.col_start.p, G.fill.p,
-                         G.tmp.p);
+      hipLaunchKernelGGL(k_scatter, dim3(grid1(nthr, 256)), dim3(256), 0, s, G.n, G.info.p, G.col_start.p, G.fill.p,
+                         G.tmp.p, order, norder);
       hipLaunchKernelGGL(k_rank, dim3(G.ncol), dim3(256), 0, s, G.ncol, G.key.p, G.col_start.p, G.col_cl.p,
                          G.tmp.p, G.slot_loc.p, G.info.p, pos_dev, G.off, (const SysArg*)G.sys.p,
                          (const int*)nullptr, G.origin.p, G.bb0.p, G.kr.p, (int*)nullptr, G.cl_col.p);
@@ -243,7 +257,12 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
   int* hostT = c->host_ro;
   P2A_HIP(hipMemcpyAsync(hostT, c->totals.p, 16 * sizeof(int), hipMemcpyDeviceToHost, s));
   P2A_HIP(hipStreamSynchronize(s));
-  for (int gi = 0; gi < ngroups; gi++) c->g[gi].ncl = hostT[5 + 2 * gi];
+  for (int gi = 0; gi < ngroups; gi++) {
+    Group& G = c->g[gi];
+    G.ncl = hostT[5 + 2 * gi];
+    G.order_nslot = G.ncl * CL;
+    G.order_ptr = G.slot_loc.p;
+  }
   c->max_wrap = hostT[wslot];
   c->nitems = 0;
   for (int d = 0; d < c->ndir; d++) {

@@ -214,18 +214,23 @@ __device__ __forceinline__ void block_col_range(int col, int* sh, int& lo, int& 
 // fractional coordinates, image and column of every atom (atom i of the
 // arrays is requested atom off + i); column counts
 // mmax (optional): largest |lattice translation| of a wrapped atom, as an
-// atomic maximum; mclear is zeroed for the next rebuild (two alternating slots)
+// atomic maximum; mclear is zeroed for the next rebuild (two alternating slots).
+// order (optional): the atoms in the slot order of the previous build (norder
+// slots, -1 = padding), so that a workgroup's atoms share few columns and its
+// counts stay in LDS; otherwise thread i takes atom i.
 __global__ __launch_bounds__(256) void k_bin(int n, const double* __restrict__ pos, int off,
                                              const SysArg* __restrict__ sys, const int* __restrict__ atom_sys,
                                              int4* __restrict__ info, double* __restrict__ key,
                                              int* __restrict__ col_count, int* __restrict__ mmax = nullptr,
-                                             int* __restrict__ mclear = nullptr) {
+                                             int* __restrict__ mclear = nullptr, const int* __restrict__ order = nullptr,
+                                             int norder = 0) {
   __shared__ int sh[512];
   __shared__ int hist[kBinLds];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = order ? (s < norder ? order[s] : -1) : (s < n ? s : -1);
   int col = -1;
   int am = 0;
-  if (i < n) {
+  if (i >= 0) {
     const SysArg& S = sys[nsys_of(atom_sys, i)];
     const double* x = pos + 3 * (size_t)(off + i);
     double sf[3];
@@ -248,7 +253,7 @@ __global__ __launch_bounds__(256) void k_bin(int n, const double* __restrict__ p
   if (mmax) {
     for (int o = 32; o > 0; o >>= 1) am = max(am, __shfl_xor(am, o, 64));
     if ((threadIdx.x & 63) == 0 && am > 0) atomicMax(mmax, am);
-    if (i == 0) *mclear = 0;
+    if (s == 0) *mclear = 0;
   }
   int lo, hi;
   block_col_range(col, sh, lo, hi);
@@ -493,14 +498,17 @@ __global__ void k_fill_double(double* p, size_t n) {
 }
 
 // atoms listed by column (any order inside a column; k_rank orders them);
-// slots within a column are reserved once per (workgroup, column) as in k_bin
+// slots within a column are reserved once per (workgroup, column) as in
+// k_bin, over the same atom order
 __global__ __launch_bounds__(256) void k_scatter(int n, const int4* __restrict__ info,
                                                  const int* __restrict__ col_start, int* __restrict__ fill,
-                                                 int* __restrict__ tmp) {
+                                                 int* __restrict__ tmp, const int* __restrict__ order = nullptr,
+                                                 int norder = 0) {
   __shared__ int sh[512];
   __shared__ int hist[kBinLds];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int col = (i < n) ? info[i].w : -1;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = order ? (s < norder ? order[s] : -1) : (s < n ? s : -1);
+  const int col = (i >= 0) ? info[i].w : -1;
   int lo, hi;
   block_col_range(col, sh, lo, hi);
   if (hi < 0) return;  // uniform
@@ -525,15 +533,20 @@ __global__ __launch_bounds__(256) void k_scatter(int n, const int4* __restrict__
 
 // Slots of one column and the bounds of its clusters; one workgroup per column.
 // (1) Rank of each atom inside its column by (key, index): a deterministic
-//     order.  The column's keys and indices are staged in LDS (padded to a
-//     multiple of 4 with keys that rank last) when they fit; each thread
-//     counts the keys below its own, four at a time.
+//     order.  When the column fits in LDS its atoms are grouped into buckets
+//     of key (a counting pass, a scan, a scatter), and each atom is ranked
+//     against its own bucket only; larger columns count over the whole
+//     column in global memory.
 // (2) Per cluster (8 lanes, one per slot): FP64 origin (bounding-box centre
 //     of the wrapped positions at the rebuild), FP32 bounding box, range of
 //     fractional c (widened outwards so that the FP32 range contains the
 //     FP64 keys) and system.  Padding slots come last in a cluster, so its
 //     first slot always holds an atom.
 constexpr int kRankLds = 2048;
+constexpr int kRankBuckets = 256;  // the workgroup size: one scan
+__device__ __forceinline__ int rank_bucket(double k) {
+  return max(0, min((int)(k * kRankBuckets), kRankBuckets - 1));
+}
 __global__ __launch_bounds__(256) void k_rank(int ncol, const double* __restrict__ key,
                                               const int* __restrict__ col_start, const int* __restrict__ col_cl,
                                               const int* __restrict__ tmp, int* __restrict__ slot_loc,
@@ -542,36 +555,58 @@ __global__ __launch_bounds__(256) void k_rank(int ncol, const double* __restrict
                                               double4* __restrict__ origin, float4* __restrict__ bb0,
                                               float2* __restrict__ kr, int* __restrict__ cl_sys,
                                               int* __restrict__ cl_col) {
-  __shared__ __align__(16) double sk[kRankLds];
-  __shared__ __align__(16) int si[kRankLds];
+  __shared__ double sk[kRankLds];
+  __shared__ int si[kRankLds];
   __shared__ int ss[kRankLds];
+  __shared__ int sbk[kRankLds];  // local indices grouped by key bucket
+  __shared__ int bcnt[kRankBuckets], bstart[kRankBuckets + 1];
+  __shared__ int sw[32];
   const int col = blockIdx.x;
   if (col >= ncol) return;
   const int b = col_start[col], n = col_start[col + 1] - b;
   const int c0 = col_cl[col], ncl = col_cl[col + 1] - c0;
   const int base = c0 * CL;
   for (int q = threadIdx.x; q < ncl; q += blockDim.x) cl_col[c0 + q] = col;
-  const int n4 = (n + 3) & ~3;
-  const bool lds = n4 <= kRankLds;
+  // padding slots at the end of the column's last cluster
+  for (int t = n + (int)threadIdx.x; t < ncl * CL; t += blockDim.x) slot_loc[base + t] = -1;
+  const bool lds = n <= kRankLds;
   if (lds) {
-    for (int t = threadIdx.x; t < n4; t += blockDim.x) {
-      const int o = t < n ? tmp[b + t] : INT_MAX;
+    // the keys lie in [0, 1): bucket floor(key * kRankBuckets) keeps their
+    // order, so an atom's rank is the atoms of the lower buckets plus those of
+    // its own bucket below it by (key, index)
+    for (int q = threadIdx.x; q < kRankBuckets; q += blockDim.x) bcnt[q] = 0;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      const int o = tmp[b + t];
       si[t] = o;
-      sk[t] = t < n ? key[o] : __builtin_huge_val();
+      sk[t] = key[o];
     }
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += blockDim.x)
+      atomicAdd(&bcnt[rank_bucket(sk[t])], 1);
+    __syncthreads();
+    {
+      int v = threadIdx.x < kRankBuckets ? bcnt[threadIdx.x] : 0, dummy = 0, tot, tot2;
+      block_excl_scan2(v, dummy, tot, tot2, sw);
+      if (threadIdx.x < kRankBuckets) {
+        bstart[threadIdx.x] = v;
+        bcnt[threadIdx.x] = v;  // fill pointer of the bucket
+      }
+      if (threadIdx.x == 0) bstart[kRankBuckets] = tot;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += blockDim.x)
+      sbk[atomicAdd(&bcnt[rank_bucket(sk[t])], 1)] = t;
     __syncthreads();
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
       const double ka = sk[t];
       const int ia = si[t];
-      int rank = 0;
-      for (int u = 0; u < n4; u += 4) {
-        const double2 k01 = *reinterpret_cast<const double2*>(sk + u);
-        const double2 k23 = *reinterpret_cast<const double2*>(sk + u + 2);
-        const int4 iv = *reinterpret_cast<const int4*>(si + u);
-        rank += (k01.x < ka) || (k01.x == ka && iv.x < ia);
-        rank += (k01.y < ka) || (k01.y == ka && iv.y < ia);
-        rank += (k23.x < ka) || (k23.x == ka && iv.z < ia);
-        rank += (k23.y < ka) || (k23.y == ka && iv.w < ia);
+      const int bk = rank_bucket(ka);
+      const int p1 = bstart[bk + 1];
+      int rank = bstart[bk];
+      for (int p = bstart[bk]; p < p1; p++) {
+        const int u = sbk[p];
+        const double kb = sk[u];
+        rank += (kb < ka) || (kb == ka && si[u] < ia);
       }
       slot_loc[base + rank] = ia;
       ss[rank] = ia;
