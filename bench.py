@@ -156,9 +156,10 @@ def bench_autograd(P, torch, steps=20, warm=2, skin=0.1):
     """Differentiable COORDINATION: GROUPA=10,000, GROUPB=490,000 at 100 /nm^3,
     RATIONAL R_0=0.35 NN=8 MM=16 D_MAX=1.0, float32 positions on a random walk
     (sigma 0.005 nm per step, frames generated on the GPU before timing); one
-    step = the value-only forward plus the recomputing backward on the next
-    frame.  Timed twice: with the pair list kept while no atom has moved more
-    than skin/2, and rebuilt at every call."""
+    step = the forward on the next frame (value and derivatives in one pass)
+    and its backward (grad x the kept derivatives).  Timed twice: with the
+    pair list kept while no atom has moved more than skin/2, and rebuilt at
+    every call."""
     from plumed2_amd.autograd import coordination
 
     na, nb = 10_000, 490_000
@@ -542,7 +543,7 @@ def main():
                         f"random walk sigma={SIGMA} nm/frame",
             "atoms": N_ATOMS,
             "contributing_pairs_per_step": pairs_per_step,
-            "accumulation": "f64",
+            "accumulation": "f32 per lane and work item, f64 across lanes, items and workgroups",
             "wall_s": wall,
             "autotuned": {"chunk": tuned[0], "blocks_per_cu": tuned[1]},
         },
@@ -553,7 +554,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": measured_traffic(),
-            "kernel": f"k_pairs<3,false,false,false,{tuned[0]}> + k_pairs_band (FP32 pass + FP64 cut decisions of the band pairs, value/derivative/virial)",
+            "kernel": f"k_pairs<3,false,false,false,{tuned[0]}> (FP32 pass with the FP64 cut decisions of its band pairs; value, derivatives, virial)",
             "kernel_ms": kern_ms,
             "flop_per_pair": FLOP_PER_PAIR,
         },

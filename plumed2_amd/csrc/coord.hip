@@ -103,6 +103,9 @@ struct p2a_coord {
   // atom has moved more than skin/2 (skin_active: 0 when the cell is too
   // narrow for the larger radius)
   double skin = 0.0, skin_active = 0.0;
+  // positions packed ahead by the skin test, for the list of build packed_gen
+  const double* packed_pos = nullptr;
+  int64_t packed_gen = -1;
   p2a::DBuf<double> pos_ref;  // positions at the last build
   p2a::DBuf<int> moved;
   int* host_ro = nullptr;  // pinned host words for the few device-to-host reads
@@ -417,18 +420,28 @@ static int reduce_partials(p2a_coord* c, int nred, double* value_dev, double* vi
   return P2A_OK;
 }
 
+// FP32 cluster positions and boxes of this step's positions, unless the
+// skin test already packed them for the list in use
+static void pack_positions(p2a_coord* c, const double* pos_dev) {
+  const bool ahead = c->packed_pos == pos_dev && c->packed_gen == c->rebuilds;
+  c->packed_pos = nullptr;
+  if (ahead) return;
+  const int ngroups = c->spec.nb > 0 ? 2 : 1;
+  for (int gi = 0; gi < ngroups; gi++) {
+    Group& G = c->g[gi];
+    if (G.ncl == 0) continue;
+    hipLaunchKernelGGL(k_pack_bbox, dim3(grid1((size_t)G.ncl * CL, 256)), dim3(256), 0, c->stream, G.ncl * CL,
+                       (const int*)nullptr, G.slot_loc.p, G.info.p, pos_dev, G.off, (const SysArg*)G.sys.p,
+                       (const int*)nullptr, G.origin.p, G.p.p, G.bb.p, pack_scale(c->spec.sw));
+  }
+}
+
 static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, double* value_dev, double* deriv_dev,
                      double* virial_dev, bool value_only) {
   int rc;
   hipStream_t s = c->stream;
   const int ngroups = c->spec.nb > 0 ? 2 : 1;
-  for (int gi = 0; gi < ngroups; gi++) {
-    Group& G = c->g[gi];
-    if (G.ncl == 0) continue;
-    hipLaunchKernelGGL(k_pack_bbox, dim3(grid1((size_t)G.ncl * CL, 256)), dim3(256), 0, s, G.ncl * CL,
-                       (const int*)nullptr, G.slot_loc.p, G.info.p, pos_dev, G.off, (const SysArg*)G.sys.p,
-                       (const int*)nullptr, G.origin.p, G.p.p, G.bb.p, pack_scale(c->spec.sw));
-  }
+  pack_positions(c, pos_dev);
   PairArgs A;
   std::memset(&A, 0, sizeof(A));
   A.twogroup = ngroups == 2;
@@ -662,13 +675,7 @@ static int run_rows(p2a_coord* c, const double* box9, const double* pos_dev, dou
   int rc;
   hipStream_t s = c->stream;
   const int ngroups = c->spec.nb > 0 ? 2 : 1;
-  for (int gi = 0; gi < ngroups; gi++) {
-    Group& G = c->g[gi];
-    if (G.ncl == 0) continue;
-    hipLaunchKernelGGL(k_pack_bbox, dim3(grid1((size_t)G.ncl * CL, 256)), dim3(256), 0, s, G.ncl * CL,
-                       (const int*)nullptr, G.slot_loc.p, G.info.p, pos_dev, G.off, (const SysArg*)G.sys.p,
-                       (const int*)nullptr, G.origin.p, G.p.p, G.bb.p, pack_scale(c->spec.sw));
-  }
+  pack_positions(c, pos_dev);
   PairArgs A;
   std::memset(&A, 0, sizeof(A));
   A.twogroup = ngroups == 2;
@@ -758,6 +765,11 @@ static int prepare_cluster_list(p2a_coord* c, int64_t step, const double* box9, 
       hipLaunchKernelGGL(k_moved, dim3(grid1((size_t)c->natoms, 256)), dim3(256), 0, c->stream, c->natoms, pos_dev,
                          (const double*)c->pos_ref.p, half * half, c->moved.p);
       P2A_HIP(hipMemcpyAsync(c->host_ro + 32, c->moved.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      // the positions are packed for the kept list while the host waits for
+      // the answer (a rebuild packs them again)
+      pack_positions(c, pos_dev);
+      c->packed_pos = pos_dev;
+      c->packed_gen = c->rebuilds;
       P2A_HIP(hipStreamSynchronize(c->stream));
       need = c->host_ro[32] != 0;
     } else {
