@@ -32,6 +32,13 @@
 
 #include "p2a_internal.h"
 
+// The grid arithmetic follows the reference's FP64 operations one rounding at
+// a time: no fused multiply-add contraction.  It also makes every evaluation
+// of a hill's Gaussian round the same way wherever it is inlined, which the
+// run-wise deposition relies on (a corner's value for a height and the node's
+// value on the grid receive bit-identical additions).
+#pragma clang fp contract(off)
+
 namespace p2a {
 
 #define P2A_HIPM(x)                                                                                  \
@@ -367,24 +374,44 @@ __global__ __launch_bounds__(kSeqThreads) void k_deposit_seq(GridArg G, double* 
 
 // Two non-periodic CVs: the same sequential deposition with the grid around
 // the walk held in LDS (a tile of tw x th <= kTileNodes nodes, shaped like
-// the hill support: value and the two gradient components), in runs of up to kRun consecutive hills that the
-// tile holds (support and upper spline corner):
-//   1. wave 0 decides the run: the current tile when it holds the first
-//      hill, else a tile centred on that hill (the old one is written back);
-//      the run ends before the first hill outside the grid or the tile;
-//   2. the workgroup evaluates the shape (scaled distances, exponential) of
-//      every hill of the run at the spline corners of every later hill;
-//      then wave 0 finds the heights in order, lane 4r+q holding corner q of
-//      hill r: hill k's height comes from its four corners, and every later
-//      corner in hill k's support adds hill k's Gaussian for that height;
-//   3. the whole workgroup adds the run to the tile, one node per thread,
-//      the hills of the run in order.
-// Every node, and every corner read for a height, receives the same FP64
-// additions in the same order as hill-by-hill deposition, so grids and
-// heights are identical to it; the workgroup meets two barriers per run
-// instead of two per hill.
+// the hill support: value and the two gradient components), in runs of up
+// to kRun consecutive hills that the tile holds (support and upper spline
+// corner).  A run's heights are found by wave 0 alone (lane 4r+q holds
+// corner q of hill r: hill k's height comes from its four corners, then
+// every later corner in hill k's support adds hill k's Gaussian), from the
+// shapes of each hill at the corners of the later hills of the run, which
+// the whole workgroup evaluates beforehand.  Runs are pipelined: while waves
+// 1-15 add the current run to the tile (one node per thread, the run's hills
+// in order), wave 0 finds the heights of the next run, whose corners start
+// from the tile as it was before the current run plus the current run's
+// hills added in order; only a run that needs another tile (the old one is
+// written back first) waits for the current run.  Every node, and every
+// corner read for a height, receives the same FP64 additions in the same
+// order as hill-by-hill deposition, so grids and heights are identical to it.
 constexpr int kTileNodes = 5120;
 constexpr int kRun = 16;  // hills per run: 4 corners each on one wave
+constexpr int kFacLen = 160;  // tile width + height at most: factor tables
+
+// The 2-CV kernel takes the Gaussian as a product of one factor per CV,
+// exp(-dp0^2/2) exp(-dp1^2/2) (the same value as exp(-dp2) up to rounding),
+// so that a run's deposition evaluates one exponential per hill and grid
+// column or row; the support and the cutoff test are those of gauss_shape.
+__device__ __forceinline__ double gfac(double x) { return exp(-0.5 * (x * x)); }
+__device__ __forceinline__ double gscaled(const GridArg& G, int i, double c, double x) {
+  return vdiff(G, i, c, x) * G.invsigma[i];
+}
+__device__ __forceinline__ bool gauss_shape_f2(const GridArg& G, const double* xx, const double* c, double* dp,
+                                               double& e) {
+  dp[0] = gscaled(G, 0, c[0], xx[0]);
+  dp[1] = gscaled(G, 1, c[1], xx[1]);
+  double dp2 = dp[0] * dp[0];
+  dp2 += dp[1] * dp[1];
+  dp2 *= 0.5;
+  e = 0.0;
+  if (!(dp2 < 6.25)) return false;
+  e = gfac(dp[0]) * gfac(dp[1]);
+  return true;
+}
 
 __device__ __forceinline__ double spline_corner_vals(const GridArg& G, const double* x, const unsigned* ind,
                                                      const unsigned* ni, double grid, const double* der) {
@@ -416,12 +443,22 @@ __global__ __launch_bounds__(kSeqThreads) void k_deposit_seq_tile2(GridArg G, do
                                                                    double* __restrict__ heights,
                                                                    int* __restrict__ err, int tw, int th) {
   __shared__ double tv[kTileNodes], tdx[kTileNodes], tdy[kTileNodes];
-  __shared__ double sh_c[kRun][2], sh_h[kRun];
-  __shared__ int sh_ind[kRun][2];
-  // shape of hill j at corner lane l (4r+q, r > j): e (negative: no
-  // contribution) and the scaled distances
-  __shared__ double sh_e[kRun][4 * kRun], sh_dp[kRun][4 * kRun][2];
-  __shared__ int sh_nb, sh_reload, sh_nox, sh_noy, sh_bx[4];
+  // two runs: the current one (heights known, not yet on the tile) and the next
+  __shared__ double sh_c[2][kRun][2], sh_h[2][kRun];
+  __shared__ int sh_ind[2][kRun][2];
+  // shape of hill j at corner lane l (e, negative: no contribution; scaled
+  // distances): of the next run's hills (j < r), or of the current run's
+  // hills at the next run's corners
+  __shared__ double sh_e[kRun][4 * kRun];
+  // the same for the current run's hills at the next run's corners
+  __shared__ double sh_ex[kRun][4 * kRun];
+  // per hill of the current run: its factor at each column of the run's
+  // box, then at each row (sh_box: the box)
+  __shared__ double sh_f[kRun][kFacLen];
+  __shared__ int sh_box[4];
+  // value and gradient at each corner of the next run before its own hills
+  __shared__ double sh_y[4 * kRun][3];
+  __shared__ int sh_nb, sh_fit, sh_nox, sh_noy;
   const int t = threadIdx.x;
   const int lane = t & (WAVE_M - 1);
   const bool wave0 = t < WAVE_M;
@@ -439,209 +476,341 @@ __global__ __launch_bounds__(kSeqThreads) void k_deposit_seq_tile2(GridArg G, do
       der[2 * node + 1] = tdy[l];
     }
   };
-  long long k0 = 0;
-  while (k0 < nhills) {
-    // 1. the run
-    if (wave0) {
-      const long long k = k0 + lane;
-      const bool valid = k < nhills;
-      double c[2] = {0.0, 0.0};
-      unsigned ind[2] = {0u, 0u};
-      int ok = 0;
-      if (valid) {
+  // wave 0: the run of hills from k0 into buffer b; sh_fit = it lies in the
+  // current tile, else the tile centred on its first hill is sh_nox/noy.
+  // Lane l's centre (hill k0 + l) comes in c0, c1 when `pre`.
+  auto decide = [&](long long k0, int b, bool pre, double c0, double c1) {
+    const long long k = k0 + lane;
+    const bool valid = k < nhills;
+    double c[2] = {0.0, 0.0};
+    unsigned ind[2] = {0u, 0u};
+    int ok = 0;
+    if (valid) {
+      if (pre) {
+        c[0] = c0;
+        c[1] = c1;
+      } else {
         c[0] = centers[2 * k];
         c[1] = centers[2 * k + 1];
-        ok = grid_index_of(G, c, ind) >= 0;
       }
-      // nodes the hill touches: support, and the cell's upper spline corner
-      const int xl = max(0, (int)ind[0] - nn0), xh = min(nx - 1, (int)ind[0] + nn0 + 1);
-      const int yl = max(0, (int)ind[1] - nn1), yh = min(ny - 1, (int)ind[1] + nn1 + 1);
-      const int xl0 = __builtin_amdgcn_readfirstlane(xl), xh0 = __builtin_amdgcn_readfirstlane(xh);
-      const int yl0 = __builtin_amdgcn_readfirstlane(yl), yh0 = __builtin_amdgcn_readfirstlane(yh);
-      const int i00 = __builtin_amdgcn_readfirstlane((int)ind[0]);
-      const int i01 = __builtin_amdgcn_readfirstlane((int)ind[1]);
-      const bool fit0 = ox >= 0 && xl0 >= ox && xh0 < ox + tw && yl0 >= oy && yh0 < oy + th;
-      // a tile centred on the hill holds it whenever nneigh + 2 <= tile / 2,
-      // which the launch guarantees; checked anyway before any LDS index
-      const int nox = max(0, min(nx - tw, i00 - tw / 2));
-      const int noy = max(0, min(ny - th, i01 - th / 2));
-      const int bx = fit0 ? ox : nox, by = fit0 ? oy : noy;
-      const bool fits = lane < kRun && valid && ok && xl >= bx && xh < bx + tw && yl >= by && yh < by + th;
-      const unsigned long long bad = __ballot(!fits);
-      const int nb = min(kRun, __ffsll((long long)bad) - 1);
-      if (lane < nb) {
-        sh_c[lane][0] = c[0];
-        sh_c[lane][1] = c[1];
-        sh_ind[lane][0] = (int)ind[0];
-        sh_ind[lane][1] = (int)ind[1];
-      }
-      // nodes the run deposits on: ind -/+ nneigh, inside the grid
-      int dxl = lane < nb ? max(0, (int)ind[0] - nn0) : INT_MAX;
-      int dxh = lane < nb ? min(nx - 1, (int)ind[0] + nn0) : -1;
-      int dyl = lane < nb ? max(0, (int)ind[1] - nn1) : INT_MAX;
-      int dyh = lane < nb ? min(ny - 1, (int)ind[1] + nn1) : -1;
-      for (int o = 1; o < kRun; o++) {
-        dxl = min(dxl, __builtin_amdgcn_readlane(dxl, o));
-        dxh = max(dxh, __builtin_amdgcn_readlane(dxh, o));
-        dyl = min(dyl, __builtin_amdgcn_readlane(dyl, o));
-        dyh = max(dyh, __builtin_amdgcn_readlane(dyh, o));
-      }
-      if (lane == 0) {
-        sh_nb = nb;
-        sh_reload = !fit0;
-        sh_nox = nox;
-        sh_noy = noy;
-        sh_bx[0] = dxl;
-        sh_bx[1] = dxh;
-        sh_bx[2] = dyl;
-        sh_bx[3] = dyh;
-        // the first hill can be placed in no tile: outside the grid
-        if (nb == 0) atomicCAS(err, 0, (int)(1 + (k0 < 0x3fffffff ? k0 : 0x3fffffff)));
-      }
+      ok = grid_index_of(G, c, ind) >= 0;
     }
-    __syncthreads();
-    const int nb = sh_nb;  // uniform
-    if (nb == 0) {
-      flush();
-      return;
+    // nodes the hill touches: support, and the cell's upper spline corner
+    const int xl = max(0, (int)ind[0] - nn0), xh = min(nx - 1, (int)ind[0] + nn0 + 1);
+    const int yl = max(0, (int)ind[1] - nn1), yh = min(ny - 1, (int)ind[1] + nn1 + 1);
+    const int xl0 = __builtin_amdgcn_readfirstlane(xl), xh0 = __builtin_amdgcn_readfirstlane(xh);
+    const int yl0 = __builtin_amdgcn_readfirstlane(yl), yh0 = __builtin_amdgcn_readfirstlane(yh);
+    const int i00 = __builtin_amdgcn_readfirstlane((int)ind[0]);
+    const int i01 = __builtin_amdgcn_readfirstlane((int)ind[1]);
+    const bool fit0 = ox >= 0 && xl0 >= ox && xh0 < ox + tw && yl0 >= oy && yh0 < oy + th;
+    // a tile centred on the hill holds it whenever nneigh + 2 <= tile / 2,
+    // which the launch guarantees; checked anyway before any LDS index
+    const int nox = max(0, min(nx - tw, i00 - tw / 2));
+    const int noy = max(0, min(ny - th, i01 - th / 2));
+    const int bx = fit0 ? ox : nox, by = fit0 ? oy : noy;
+    const bool fits = lane < kRun && valid && ok && xl >= bx && xh < bx + tw && yl >= by && yh < by + th;
+    const unsigned long long bad = __ballot(!fits);
+    const int nb = min(kRun, __ffsll((long long)bad) - 1);
+    if (lane < nb) {
+      sh_c[b][lane][0] = c[0];
+      sh_c[b][lane][1] = c[1];
+      sh_ind[b][lane][0] = (int)ind[0];
+      sh_ind[b][lane][1] = (int)ind[1];
     }
-    if (sh_reload) {
-      flush();
-      // the tile written above may overlap the one read below
-      __threadfence();
-      __syncthreads();
-      ox = sh_nox;
-      oy = sh_noy;
-      for (int j = t; j < tw * th; j += blockDim.x) {
-        const int lx = j % tw, ly = j / tw;
-        const long long node = (long long)(oy + ly) * nx + (ox + lx);
-        const int l = lx + tw * ly;
-        tv[l] = val[node];
-        tdx[l] = der[2 * node];
-        tdy[l] = der[2 * node + 1];
-      }
-      __syncthreads();
+    if (lane == 0) {
+      sh_nb = nb;
+      sh_fit = fit0;
+      sh_nox = nox;
+      sh_noy = noy;
+      // a first hill that no tile can hold lies outside the grid
+      if (nb == 0 && k0 < nhills) atomicCAS(err, 0, (int)(1 + (k0 < 0x3fffffff ? k0 : 0x3fffffff)));
     }
-    // 2a. shapes of each hill j at the corners of the later hills
+  };
+  // the shapes of the hills j < nsrc of run bs at the corners of the hills
+  // r < ndst of run bd (only j < r when both are the same run)
+  auto shapes = [&](int bs, int nsrc, int bd, int ndst, double (*out)[4 * kRun]) {
     for (int p = t; p < kRun * 4 * kRun; p += blockDim.x) {
       const int j = p / (4 * kRun), l = p % (4 * kRun);
       const int r = l >> 2, q = l & 3;
       double e = -1.0, dp[2] = {0.0, 0.0};
-      if (j < r && r < nb) {
-        const unsigned n0 = (unsigned)sh_ind[r][0] + (q & 1), n1 = (unsigned)sh_ind[r][1] + (q >> 1);
-        if (n0 < G.nbin[0] && n1 < G.nbin[1] && abs((int)n0 - sh_ind[j][0]) <= nn0 &&
-            abs((int)n1 - sh_ind[j][1]) <= nn1) {
+      if (j < nsrc && r < ndst && (bs != bd || j < r)) {
+        const unsigned n0 = (unsigned)sh_ind[bd][r][0] + (q & 1), n1 = (unsigned)sh_ind[bd][r][1] + (q >> 1);
+        if (n0 < G.nbin[0] && n1 < G.nbin[1] && abs((int)n0 - sh_ind[bs][j][0]) <= nn0 &&
+            abs((int)n1 - sh_ind[bs][j][1]) <= nn1) {
           const double xx[2] = {G.min[0] + (double)n0 * G.dx[0], G.min[1] + (double)n1 * G.dx[1]};
-          const double cj[2] = {sh_c[j][0], sh_c[j][1]};
+          const double cj[2] = {sh_c[bs][j][0], sh_c[bs][j][1]};
           double ej;
-          if (gauss_shape<2>(G, xx, cj, dp, ej)) e = ej;
+          if (gauss_shape_f2(G, xx, cj, dp, ej)) e = ej;
         }
       }
-      sh_e[j][l] = e;
-      sh_dp[j][l][0] = dp[0];
-      sh_dp[j][l][1] = dp[1];
+      out[j][l] = e;
     }
-    __syncthreads();
-    // 2b. heights, in order (wave 0)
-    if (wave0) {
-      const int r = lane >> 2, q = lane & 3;
-      const bool mine = r < nb;
-      double c[2] = {0.0, 0.0};
-      unsigned ind[2] = {0u, 0u}, ni[2] = {0u, 0u};
-      bool has = false;
-      double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-      if (mine) {
-        c[0] = sh_c[r][0];
-        c[1] = sh_c[r][1];
-        ind[0] = (unsigned)sh_ind[r][0];
-        ind[1] = (unsigned)sh_ind[r][1];
-        // corner q of the cell (Grid.cpp:442-505): absent past the last node
-        ni[0] = ind[0] + (q & 1);
-        ni[1] = ind[1] + (q >> 1);
-        has = ni[0] < G.nbin[0] && ni[1] < G.nbin[1];
-        if (has) {
-          const int l = (int)(ni[0] - ox) + tw * (int)(ni[1] - oy);
-          acc0 = tv[l];
-          acc1 = tdx[l];
-          acc2 = tdy[l];
+  };
+  // threads t < 4 kRun: value and gradient at corner t of run b, from the
+  // tile, or from the grid in memory for a node outside the tile
+  auto snapshot = [&](int b, int nb) {
+    if (t < 4 * kRun) {
+      const int r = t >> 2, q = t & 3;
+      double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+      if (r < nb) {
+        const unsigned n0 = (unsigned)sh_ind[b][r][0] + (q & 1), n1 = (unsigned)sh_ind[b][r][1] + (q >> 1);
+        if (n0 < G.nbin[0] && n1 < G.nbin[1]) {
+          if ((int)n0 >= ox && (int)n0 < ox + tw && (int)n1 >= oy && (int)n1 < oy + th) {
+            const int l = (int)(n0 - ox) + tw * (int)(n1 - oy);
+            y0 = tv[l];
+            y1 = tdx[l];
+            y2 = tdy[l];
+          } else {
+            const long long node = (long long)n1 * nx + n0;
+            y0 = val[node];
+            y1 = der[2 * node];
+            y2 = der[2 * node + 1];
+          }
         }
       }
-      // the terms of spline_corner_vals that depend on the position only
-      double X[2] = {0.0, 0.0}, X2[2] = {0.0, 0.0}, X3[2] = {0.0, 0.0}, sg[2] = {1.0, 1.0};
+      sh_y[t][0] = y0;
+      sh_y[t][1] = y1;
+      sh_y[t][2] = y2;
+    }
+  };
+  // wave 0: the heights of run b (nb hills from hill kfirst), from sh_y plus
+  // the terms of the nprev hills of run 1 - b (sh_ex, added in order), and
+  // the run's own shapes
+  auto chain = [&](int b, int nb, long long kfirst, int nprev) {
+    const int r = lane >> 2, q = lane & 3;
+    const bool mine = r < nb;
+    double c[2] = {0.0, 0.0};
+    unsigned ind[2] = {0u, 0u}, ni[2] = {0u, 0u};
+    bool has = false;
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    if (mine) {
+      c[0] = sh_c[b][r][0];
+      c[1] = sh_c[b][r][1];
+      ind[0] = (unsigned)sh_ind[b][r][0];
+      ind[1] = (unsigned)sh_ind[b][r][1];
+      // corner q of the cell (Grid.cpp:442-505): absent past the last node
+      ni[0] = ind[0] + (q & 1);
+      ni[1] = ind[1] + (q >> 1);
+      has = ni[0] < G.nbin[0] && ni[1] < G.nbin[1];
+      if (has) {
+        acc0 = sh_y[lane][0];
+        acc1 = sh_y[lane][1];
+        acc2 = sh_y[lane][2];
+        const int pb = 1 - b;
+        for (int j0 = 0; j0 < nprev; j0 += 4) {
+          double tb[4], td0[4], td1[4];
+          bool add[4];
 #pragma unroll
-      for (int j = 0; j < 2; j++) {
-        const int x0 = (ni[j] == ind[j]) ? 0 : 1;
-        const double dx = G.dx[j];
-        const double xfloor = G.min[j] + (double)ind[j] * dx;
-        X[j] = fabs((c[j] - xfloor) / dx - (double)x0);
-        X2[j] = X[j] * X[j];
-        X3[j] = X2[j] * X[j];
-        sg[j] = x0 ? -1.0 : 1.0;
+          for (int u = 0; u < 4; u++) {
+            const int j = min(j0 + u, nprev - 1);
+            const double e = sh_ex[j][lane];
+            add[u] = j0 + u < nprev && e >= 0.0;
+            const double dp[2] = {gscaled(G, 0, sh_c[pb][j][0], G.min[0] + (double)ni[0] * G.dx[0]),
+                                  gscaled(G, 1, sh_c[pb][j][1], G.min[1] + (double)ni[1] * G.dx[1])};
+            double d[2];
+            tb[u] = gauss_apply<2>(G, dp, e, sh_h[pb][j], d);
+            td0[u] = d[0];
+            td1[u] = d[1];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            if (add[u]) {
+              acc0 += tb[u];
+              acc1 += td0[u];
+              acc2 += td1[u];
+            }
+          }
+        }
       }
+    }
+    // the terms of spline_corner_vals that depend on the position only
+    double X[2] = {0.0, 0.0}, X2[2] = {0.0, 0.0}, X3[2] = {0.0, 0.0}, sg[2] = {1.0, 1.0};
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int x0 = (ni[j] == ind[j]) ? 0 : 1;
+      const double dx = G.dx[j];
+      const double xfloor = G.min[j] + (double)ind[j] * dx;
+      X[j] = fabs((c[j] - xfloor) / dx - (double)x0);
+      X2[j] = X[j] * X[j];
+      X3[j] = X2[j] * X[j];
+      sg[j] = x0 ? -1.0 : 1.0;
+    }
+    for (int k = 0; k < nb; k++) {
+      // corner term of this lane (used by the lanes of hill k only): the
+      // arithmetic of spline_corner_vals
+      double v = 0.0;
+      if (G.welltemp && has && r == k) {
+        const double grid = acc0;
+        const double dg[2] = {acc1, acc2};
+        double ff = 1.0;
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          const double yy = fabs(grid) < 0.0000001 ? 0.0 : -dg[j] / grid;
+          ff *= (1.0 - 3.0 * X2[j] + 2.0 * X3[j]) - sg[j] * yy * (X[j] - 2.0 * X2[j] + X3[j]) * G.dx[j];
+        }
+        v = grid * ff;
+      }
+      // the four corners of hill k in corner order, on every lane
+      double tot = 0.0;
+#pragma unroll
+      for (int qq = 0; qq < 4; qq++) tot += readlane_d(v, 4 * k + qq);
+      const double h = G.welltemp ? wt_height(G, tot) : G.height0;
+      if (lane == 4 * k) {
+        sh_h[b][k] = h;
+        if (heights) heights[kfirst + k] = h;
+      }
+      if (r > k && mine) {
+        const double e = sh_e[k][lane];
+        if (e >= 0.0) {
+          const double dp[2] = {gscaled(G, 0, sh_c[b][k][0], G.min[0] + (double)ni[0] * G.dx[0]),
+                                gscaled(G, 1, sh_c[b][k][1], G.min[1] + (double)ni[1] * G.dx[1])};
+          double d[2];
+          const double bb = gauss_apply<2>(G, dp, e, h, d);
+          acc0 += bb;
+          acc1 += d[0];
+          acc2 += d[1];
+        }
+      }
+    }
+  };
+  // run b (nb hills) onto the tile: nodes t0, t0 + step, ..., hills in order
+  // the box and the factor tables of run b (all threads; a barrier follows)
+  auto factors = [&](int b, int nb) {
+    // nodes the run deposits on: ind -/+ nneigh, inside the grid
+    int bx0 = INT_MAX, bx1 = -1, by0 = INT_MAX, by1 = -1;
+    for (int k = 0; k < nb; k++) {
+      bx0 = min(bx0, max(0, sh_ind[b][k][0] - nn0));
+      bx1 = max(bx1, min(nx - 1, sh_ind[b][k][0] + nn0));
+      by0 = min(by0, max(0, sh_ind[b][k][1] - nn1));
+      by1 = max(by1, min(ny - 1, sh_ind[b][k][1] + nn1));
+    }
+    const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1;  // bw + bh <= tw + th <= kFacLen
+    if (t == 0) {
+      sh_box[0] = bx0;
+      sh_box[1] = by0;
+      sh_box[2] = bw;
+      sh_box[3] = bh;
+    }
+    for (int p = t; p < nb * (bw + bh); p += blockDim.x) {
+      const int k = p / (bw + bh), m = p % (bw + bh);
+      const int i = m < bw ? 0 : 1;
+      const int n = i == 0 ? bx0 + m : by0 + (m - bw);
+      sh_f[k][m] = gfac(gscaled(G, i, sh_c[b][k][i], G.min[i] + (double)n * G.dx[i]));
+    }
+  };
+  // run b (nb hills) onto the tile: nodes t0, t0 + step, ..., hills in order
+  // (factors() of the run first)
+  auto deposit = [&](int b, int nb, int t0, int step) {
+    const int bx0 = sh_box[0], by0 = sh_box[1], bw = sh_box[2], bh = sh_box[3];
+    for (int j = t0; j < bw * bh; j += step) {
+      const int cx = j % bw, cy = j / bw;
+      const int n0 = bx0 + cx, n1 = by0 + cy;
+      const int l = (n0 - ox) + tw * (n1 - oy);
+      const double xx[2] = {G.min[0] + (double)n0 * G.dx[0], G.min[1] + (double)n1 * G.dx[1]};
+      double v = tv[l], gx = tdx[l], gy = tdy[l];
       for (int k = 0; k < nb; k++) {
-        // corner term of this lane (used by the lanes of hill k only): the
-        // arithmetic of spline_corner_vals
-        double v = 0.0;
-        if (G.welltemp && has && r == k) {
-          const double grid = acc0;
-          const double der[2] = {acc1, acc2};
-          double ff = 1.0;
-#pragma unroll
-          for (int j = 0; j < 2; j++) {
-            const double yy = fabs(grid) < 0.0000001 ? 0.0 : -der[j] / grid;
-            ff *= (1.0 - 3.0 * X2[j] + 2.0 * X3[j]) - sg[j] * yy * (X[j] - 2.0 * X2[j] + X3[j]) * G.dx[j];
-          }
-          v = grid * ff;
-        }
-        // the four corners of hill k in corner order, on every lane
-        double tot = 0.0;
-#pragma unroll
-        for (int qq = 0; qq < 4; qq++) tot += readlane_d(v, 4 * k + qq);
-        const double h = G.welltemp ? wt_height(G, tot) : G.height0;
-        if (lane == 4 * k) {
-          sh_h[k] = h;
-          if (heights) heights[k0 + k] = h;
-        }
-        if (r > k && mine) {
-          const double e = sh_e[k][lane];
-          if (e >= 0.0) {
-            const double dp[2] = {sh_dp[k][lane][0], sh_dp[k][lane][1]};
-            double d[2];
-            const double b = gauss_apply<2>(G, dp, e, h, d);
-            acc0 += b;
-            acc1 += d[0];
-            acc2 += d[1];
-          }
+        if (abs(n0 - sh_ind[b][k][0]) <= nn0 && abs(n1 - sh_ind[b][k][1]) <= nn1) {
+          // gauss_shape_f2 with the factors from the tables, then gauss_apply
+          double dp[2];
+          dp[0] = gscaled(G, 0, sh_c[b][k][0], xx[0]);
+          dp[1] = gscaled(G, 1, sh_c[b][k][1], xx[1]);
+          double dp2 = dp[0] * dp[0];
+          dp2 += dp[1] * dp[1];
+          dp2 *= 0.5;
+          double bb = 0.0, d[2] = {0.0, 0.0};
+          if (dp2 < 6.25) bb = gauss_apply<2>(G, dp, sh_f[k][cx] * sh_f[k][bw + cy], sh_h[b][k], d);
+          v += bb;
+          gx += d[0];
+          gy += d[1];
         }
       }
+      tv[l] = v;
+      tdx[l] = gx;
+      tdy[l] = gy;
+    }
+  };
+
+  // the old tile written back, the tile at sh_nox/noy read
+  auto reload = [&]() {
+    flush();
+    // the tile written above may overlap the one read below
+    __threadfence();
+    __syncthreads();
+    ox = sh_nox;
+    oy = sh_noy;
+    for (int j = t; j < tw * th; j += blockDim.x) {
+      const int lx = j % tw, ly = j / tw;
+      const long long node = (long long)(oy + ly) * nx + (ox + lx);
+      const int l = lx + tw * ly;
+      tv[l] = val[node];
+      tdx[l] = der[2 * node];
+      tdy[l] = der[2 * node + 1];
     }
     __syncthreads();
-    // 3. the run on the tile: one node per thread, hills in order
-    {
-      const int bx0 = sh_bx[0], bx1 = sh_bx[1], by0 = sh_bx[2], by1 = sh_bx[3];
-      const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1;
-      for (int j = t; j < bw * bh; j += blockDim.x) {
-        const int n0 = bx0 + j % bw, n1 = by0 + j / bw;
-        const int l = (n0 - ox) + tw * (n1 - oy);
-        const double xx[2] = {G.min[0] + (double)n0 * G.dx[0], G.min[1] + (double)n1 * G.dx[1]};
-        double v = tv[l], gx = tdx[l], gy = tdy[l];
-        for (int k = 0; k < nb; k++) {
-          if (abs(n0 - sh_ind[k][0]) <= nn0 && abs(n1 - sh_ind[k][1]) <= nn1) {
-            const double ck[2] = {sh_c[k][0], sh_c[k][1]};
-            double d[2];
-            const double b = gaussian<2>(G, xx, ck, sh_h[k], d);
-            v += b;
-            gx += d[0];
-            gy += d[1];
-          }
-        }
-        tv[l] = v;
-        tdx[l] = gx;
-        tdy[l] = gy;
+  };
+
+  long long k0 = 0;  // first hill not yet in a run
+  int cb = 0;        // buffer of the current run
+  int nbc = 0;       // hills of the current run (heights known, not on the tile)
+  // wave 0: the centres of the hills the next run decision starts with,
+  // loaded while the heights of the next run are found
+  bool pre = false;
+  double pc0 = 0.0, pc1 = 0.0;
+  auto prefetch = [&](long long kn) {
+    pre = true;
+    const long long k = kn + lane;
+    if (lane < kRun && k < nhills) {
+      pc0 = centers[2 * k];
+      pc1 = centers[2 * k + 1];
+    }
+  };
+  while (true) {
+    if (wave0) decide(k0, 1 - cb, pre, pc0, pc1);
+    __syncthreads();
+    const int nbn = sh_nb;  // uniform; 0: no hill left, or the next is outside the grid
+    const bool fit = sh_fit;
+    if (nbc > 0 && nbn > 0) {
+      // the next run's corners: the grid before the current run (tile, or
+      // memory outside it, which the current run does not touch), then the
+      // current run's hills in order (at the start of the chain); the next
+      // run's own shapes; the current run's factors
+      shapes(cb, nbc, 1 - cb, nbn, sh_ex);
+      shapes(1 - cb, nbn, 1 - cb, nbn, sh_e);
+      snapshot(1 - cb, nbn);
+      factors(cb, nbc);
+      __syncthreads();
+      // wave 0: the next run's heights; the other waves: the current run
+      if (wave0) {
+        prefetch(k0 + nbn);
+        chain(1 - cb, nbn, k0, nbc);
+      } else {
+        deposit(cb, nbc, t - WAVE_M, (int)blockDim.x - WAVE_M);
       }
+      __syncthreads();
+      // the next run lies in another tile
+      if (!fit) reload();
+    } else {
+      // the first run, the end, or a hill outside the grid: in sequence
+      if (nbc > 0) {
+        factors(cb, nbc);
+        __syncthreads();
+        deposit(cb, nbc, t, blockDim.x);
+        __syncthreads();
+      }
+      if (nbn == 0) break;
+      if (!fit) reload();
+      shapes(1 - cb, nbn, 1 - cb, nbn, sh_e);
+      snapshot(1 - cb, nbn);
+      __syncthreads();
+      if (wave0) {
+        prefetch(k0 + nbn);
+        chain(1 - cb, nbn, k0, 0);
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    k0 += nb;
+    cb = 1 - cb;
+    nbc = nbn;
+    k0 += nbn;
   }
   flush();
 }
@@ -921,7 +1090,7 @@ int p2a_metad_deposit_device(p2a_metad* m, int64_t nhills, const double* centers
       th = std::min(ny, need1);
       tw = std::min(nx, kTileNodes / th);
     }
-    tile2 = tw >= std::min(nx, need0) && th >= std::min(ny, need1) && tw * th <= kTileNodes &&
+    tile2 = tw >= std::min(nx, need0) && th >= std::min(ny, need1) && tw * th <= kTileNodes && tw + th <= kFacLen &&
             need0 <= nx && need1 <= ny;
   }
   if (tile2)

@@ -189,6 +189,28 @@ def test_deposition_at_grid_edges_matches_oracle():
     assert np.max(np.abs(d - g.derivs())) <= TOL * np.max(np.abs(g.derivs()))
 
 
+def test_bulk_deposition_equals_hill_by_hill_bitwise():
+    """Thousands of well-tempered hills in one call (runs of hills, the next
+    run's heights found while the current run is added to the grid) against
+    the same hills deposited one call per hill: heights and grid identical
+    bit for bit, since every node and corner receives the same FP64
+    additions in the same order.  The walk leaves the LDS tile now and then
+    (reloads) and runs into the grid edge."""
+    P = _P()
+    line = ("m: METAD ARG=c,d SIGMA=20,0.05 HEIGHT=1.2 PACE=500 BIASFACTOR=10 TEMP=300 "
+            "GRID_MIN=0,0 GRID_MAX=5000,6 GRID_BIN=512,512")
+    centers = random_walk(1500, [2500.0, 3.0], [25.0, 0.06], [0, 0], [4999.9, 5.999], 21)
+    centers = np.concatenate([centers, random_walk(500, [4990.0, 0.01], [25.0, 0.06], [0, 0], [4999.9, 5.999], 22)])
+    m1, m2 = P.MetaD(line, periodic={}), P.MetaD(line, periodic={})
+    h1 = m1.deposit(centers)
+    h2 = np.concatenate([m2.deposit(c[None, :]) for c in centers])
+    assert np.array_equal(h1, h2)
+    v1, d1 = m1.grid()
+    v2, d2 = m2.grid()
+    assert np.array_equal(v1, v2)
+    assert np.array_equal(d1, d2)
+
+
 def test_rt_backup_grid_files_through_gpu(tmp_path):
     """regtest/basic/rt-backup-grid: the GRID_WFILE dumps of the GPU grid, the
     last one and the one it replaced, equal the reference files."""
