@@ -108,6 +108,7 @@ struct p2a_coord {
   int64_t packed_gen = -1;
   p2a::DBuf<double> pos_ref;  // positions at the last build
   p2a::DBuf<int> moved;
+  hipEvent_t moved_ev = nullptr;  // the moved flag has reached host_ro[32]
   int* host_ro = nullptr;  // pinned host words for the few device-to-host reads
   // neighbour list of the FP64 path: partners of every atom (symmetric CSR),
   // or one flag per pair with PAIR; cell of every atom for NLISTCELLS
@@ -391,7 +392,8 @@ static int reimage(p2a_coord* c, const double* box9) {
 // and a fixed order; with gather_dir >= 0 the same launch also forms that
 // direction's derivatives
 static int reduce_partials(p2a_coord* c, int nred, double* value_dev, double* virial_dev,
-                           int gather_dir = -1, double* deriv_dev = nullptr, bool with_stats = true) {
+                           int gather_dir = -1, double* deriv_dev = nullptr, bool with_stats = true,
+                           const int* skip = nullptr) {
   hipStream_t s = c->stream;
   int ngather = 0, ncl = 0;
   const float4* gp = nullptr;
@@ -415,6 +417,7 @@ static int reduce_partials(p2a_coord* c, int nred, double* value_dev, double* vi
   R.value = value_dev;
   R.virial = virial_dev;
   R.stats = with_stats ? c->stats.p : nullptr;
+  R.skip = skip;
   hipLaunchKernelGGL(k_gather_reduce, dim3(ngather + 1), dim3(256), 0, s, ngather, ncl, (const int*)nullptr, gp,
                      istart, items_cap, fpart, deriv_dev, 1, R);
   return P2A_OK;
@@ -436,8 +439,10 @@ static void pack_positions(p2a_coord* c, const double* pos_dev) {
   }
 }
 
+// skip: a device flag; when it reads nonzero at run time the pair and
+// reduction kernels do nothing (a pass queued before the skin test's answer)
 static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, double* value_dev, double* deriv_dev,
-                     double* virial_dev, bool value_only) {
+                     double* virial_dev, bool value_only, const int* skip = nullptr) {
   int rc;
   hipStream_t s = c->stream;
   const int ngroups = c->spec.nb > 0 ? 2 : 1;
@@ -451,6 +456,7 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
   A.cl_sys = nullptr;
   A.nitems_dev = nullptr;
   A.pos64 = pos_dev;
+  A.skip = skip;
   A.item_partials = 0;  // value / virial / pair count per workgroup
   // value only: the counting direction (the first) holds every counted pair
   const int ndir_run = value_only ? 1 : c->ndir;
@@ -521,10 +527,11 @@ static int run_pairs(p2a_coord* c, const double* box9, const double* pos_dev, do
     const int ng = grid1((size_t)GI.ncl * CL * 3, 256);
     RedArgs R;
     std::memset(&R, 0, sizeof(R));
+    R.skip = skip;
     hipLaunchKernelGGL(k_gather_reduce, dim3(ng), dim3(256), 0, s, ng, GI.ncl, (const int*)nullptr, GI.p.p,
                        D.istart.p, (int)D.items.n, D.fpart.p, deriv_dev, 0, R);
   }
-  if ((rc = reduce_partials(c, nred, value_dev, virial_dev, last, deriv_dev))) return rc;
+  if ((rc = reduce_partials(c, nred, value_dev, virial_dev, last, deriv_dev, true, skip))) return rc;
   return launch_check("finalize");
 }
 
@@ -751,6 +758,29 @@ static int run_rows(p2a_coord* c, const double* box9, const double* pos_dev, dou
 // neighbour list every step, or, with a skin, when some atom has moved more
 // than skin/2 since the build: every pair within D_MAX now was within D_MAX +
 // skin then, so the pairs evaluated are those of a list rebuilt now.
+// The skin test on the device: moved = 1 when some atom has moved more than
+// skin/2 since the build; the flag is copied to host_ro[32] and moved_ev
+// recorded after the copy.
+static int launch_moved_test(p2a_coord* c, const double* pos_dev) {
+  int rc;
+  if ((rc = c->moved.reserve(1))) return rc;
+  if (!c->moved_ev) P2A_HIP(hipEventCreateWithFlags(&c->moved_ev, hipEventDisableTiming));
+  const double half = 0.5 * c->skin_active;
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->moved.p, 1, 0);
+  hipLaunchKernelGGL(k_moved, dim3(grid1((size_t)c->natoms, 256)), dim3(256), 0, c->stream, c->natoms, pos_dev,
+                     (const double*)c->pos_ref.p, half * half, c->moved.p);
+  P2A_HIP(hipMemcpyAsync(c->host_ro + 32, c->moved.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  P2A_HIP(hipEventRecord(c->moved_ev, c->stream));
+  return launch_check("skin test");
+}
+
+// A list that the skin test may keep: no neighbour list, a skin in use, a
+// list built on an earlier call in this same cell.
+static bool skin_kept_candidate(p2a_coord* c, const double* box9) {
+  return c->spec.nl_mode == P2A_NL_NONE && c->skin_active > 0.0 && c->built && !c->firsttime &&
+         std::memcmp(c->box_at_build, box9, sizeof(c->box_at_build)) == 0;
+}
+
 static int prepare_cluster_list(p2a_coord* c, int64_t step, const double* box9, const double* pos_dev) {
   int rc;
   bool need = !c->built || c->firsttime;
@@ -759,18 +789,13 @@ static int prepare_cluster_list(p2a_coord* c, int64_t step, const double* box9, 
     // the skin test measures motion in one cell: a new cell rebuilds
     if (new_cell) need = true;
     if (!need && c->skin_active > 0.0) {
-      if ((rc = c->moved.reserve(1))) return rc;
-      const double half = 0.5 * c->skin_active;
-      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->moved.p, 1, 0);
-      hipLaunchKernelGGL(k_moved, dim3(grid1((size_t)c->natoms, 256)), dim3(256), 0, c->stream, c->natoms, pos_dev,
-                         (const double*)c->pos_ref.p, half * half, c->moved.p);
-      P2A_HIP(hipMemcpyAsync(c->host_ro + 32, c->moved.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      if ((rc = launch_moved_test(c, pos_dev))) return rc;
       // the positions are packed for the kept list while the host waits for
       // the answer (a rebuild packs them again)
       pack_positions(c, pos_dev);
       c->packed_pos = pos_dev;
       c->packed_gen = c->rebuilds;
-      P2A_HIP(hipStreamSynchronize(c->stream));
+      P2A_HIP(hipEventSynchronize(c->moved_ev));
       need = c->host_ro[32] != 0;
     } else {
       need = true;
@@ -823,6 +848,17 @@ static int calc_device(p2a_coord* c, int64_t step, const double* box9, const dou
       return run_fallback_nl(c, build, box9, pos_dev, vdev, ddev, wdev);
     }
     return run_fallback(c, box9, pos_dev, vdev, ddev, wdev);
+  }
+  if (skin_kept_candidate(c, box9)) {
+    // The pass on the kept list is queued behind the displacement test and
+    // does nothing on the device if the test finds an atom that moved too far;
+    // the host waits only for the test's answer, while the pass runs.
+    if ((rc = launch_moved_test(c, pos_dev))) return rc;
+    if ((rc = run_pairs(c, box9, pos_dev, vdev, ddev, wdev, value_only, c->moved.p))) return rc;
+    P2A_HIP(hipEventSynchronize(c->moved_ev));
+    if (c->host_ro[32] == 0) return P2A_OK;
+    if ((rc = rebuild(c, box9, pos_dev))) return rc;
+    return run_pairs(c, box9, pos_dev, vdev, ddev, wdev, value_only);
   }
   if ((rc = prepare_cluster_list(c, step, box9, pos_dev))) return rc;
   return run_pairs(c, box9, pos_dev, vdev, ddev, wdev, value_only);
@@ -990,6 +1026,7 @@ int p2a_coord_destroy(p2a_coord* c) {
     c->ev_pool.push_back(pr.second);
   }
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->moved_ev) (void)hipEventDestroy(c->moved_ev);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   if (c->host_ro) (void)hipHostFree(c->host_ro);
   delete c;

@@ -1230,9 +1230,24 @@ struct PairArgs {
   double* ipart;
   int* bflag;             // band pairs of each item from a wave's 65th item on
   // Items of each wave of the grid: [wave_start[g], wave_start[g + 1]) for
-  // wave g = blockIdx * 4 + wave (k_wave_sched); null: items g, g + waves, ...
+  // wave g = xcd_block() * 4 + wave (k_wave_sched); null: items g, g + waves, ...
   const int* wave_start;
+  // when set and *skip != 0 (written by an earlier kernel on the stream), the
+  // launch does nothing: a pass queued on a list that may be stale
+  const int* skip;
 };
+
+// Workgroups are dispatched to the 8 XCDs of the chip in turn (workgroup b to
+// XCD b mod 8), and each XCD has its own L2.  This index puts the workgroups
+// of one XCD on one contiguous run of indices, so that an XCD takes one
+// contiguous share of the spatially sorted work items and its L2 holds the
+// positions of one region instead of all of them.  A bijection on
+// [0, gridDim.x) for any grid size.
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
 
 // FP64 re-evaluation of the squared distance of one pair, used only when the
 // FP32 value lies within a narrow band around D_MAX^2: the inclusive cut of
@@ -1648,6 +1663,7 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
   __shared__ int sabs[TWO ? NW : 1][TWO ? CAP : 1];
   __shared__ int srec[NW][kBandSlots];
   __shared__ double stot[NW][8];
+  if (A.skip && *A.skip) return;
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int ii = lane & 7, jj = lane >> 3;
@@ -1662,7 +1678,8 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
   // computes (a wave handles several items when the grid is persistent)
   // the wave's items: a contiguous range of balanced cost, or every
   // (grid waves)-th item
-  const int gw = blockIdx.x * NW + w;
+  const int blk = xcd_block();
+  const int gw = blk * NW + w;
   int kfirst = gw, kstride = gridDim.x * NW, kend = nitems;
   if (A.wave_start) {
     kfirst = A.wave_start[gw];
@@ -1790,7 +1807,7 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
     if (w == 0 && lane < 8) {
       double s = 0.0;
       for (int v = 0; v < NW; v++) s += stot[v][lane];
-      partial[(size_t)blockIdx.x * 8 + lane] = s;
+      partial[(size_t)blk * 8 + lane] = s;
     }
   }
 }
@@ -1811,6 +1828,7 @@ struct RedArgs {
   double* value;
   double* virial;
   double* stats;
+  const int* skip;  // as PairArgs::skip
 };
 
 // After the pair kernels, in one launch.  With do_red, workgroup 0 forms the
@@ -1824,6 +1842,7 @@ __global__ __launch_bounds__(256) void k_gather_reduce(int ngather, int ncl, con
                                                        const int* __restrict__ item_start, int items_cap,
                                                        const double* __restrict__ fpart, double* __restrict__ deriv,
                                                        int do_red, RedArgs R) {
+  if (R.skip && *R.skip) return;
   const int t = threadIdx.x;
   const int red = do_red ? 1 : 0;
   if ((int)blockIdx.x >= red) {
