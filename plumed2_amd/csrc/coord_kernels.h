@@ -755,6 +755,10 @@ struct ListArgs {
 // by the list radius: a pair closer than r differs by at most r*|c*| in
 // fractional c, so the window is exact, and the clusters of a column are
 // sorted along c, so each window is one binary-searched run of clusters.
+// Each column lane writes its runs into the wave's candidate list in LDS
+// (j cluster | image code, in column order), kCandCap at a time, and the
+// lanes then test the listed candidates, two per lane and trip.
+constexpr int kCandCap = 512;
 template <int FILL>
 __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict__ cl_colI,
                                               const int* __restrict__ cl_sysI, const SysArg* __restrict__ sysI,
@@ -764,7 +768,7 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
                                               const double4* __restrict__ orgI, const double4* __restrict__ orgJ,
                                               int* __restrict__ count, const int* __restrict__ start,
                                               Entry* __restrict__ out, unsigned* __restrict__ ell) {
-  __shared__ int s_first[4][3][WAVE], s_n[4][2][WAVE], s_off[4][WAVE], s_img[4][WAVE];
+  __shared__ unsigned s_cand[4][kCandCap];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x / WAVE;
   const int ci = blockIdx.x * (blockDim.x / WAVE) + wv;
@@ -838,78 +842,69 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
     }
     int total;
     const int off = wave_excl_scan(cnt[0] + cnt[1] + cnt[2], lane, total);
-    for (int m = 0; m < 3; m++) s_first[wv][m][lane] = f[m];
-    s_n[wv][0][lane] = cnt[0];
-    s_n[wv][1][lane] = cnt[1];
-    s_off[wv][lane] = off;
-    s_img[wv][lane] = img;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    // two candidates per lane and trip: their box loads are in flight together
-    for (int u0 = 0; u0 < total; u0 += 2 * WAVE) {
-      int cj[2] = {0, 0}, mx[2] = {0, 0}, my[2] = {0, 0}, mz[2] = {0, 0};
-      float4 jc[2], jh[2];
+    for (int c0 = 0; c0 < total; c0 += kCandCap) {
+      const int nc = min(total - c0, kCandCap);
+      // this lane's candidates with list index in [c0, c0 + nc)
+      int k = off;
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int u = u0 + h * WAVE + lane;
-        jc[h] = jh[h] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (u < total) {
-          int a = 0, b = WAVE - 1;  // last column lane whose offset is <= u
-          while (a < b) {
-            const int q = (a + b + 1) >> 1;
-            if (s_off[wv][q] <= u) a = q;
-            else b = q - 1;
+      for (int m = 0; m < 3; m++) {
+        const unsigned code = (unsigned)(((img & 3) + 3 * ((img >> 2) & 3) + 9 * m) << 26);
+        const int lo = max(k, c0), hi = min(k + cnt[m], c0 + nc);
+        for (int u = lo; u < hi; u++) s_cand[wv][u - c0] = (unsigned)(f[m] + (u - k)) | code;
+        k += cnt[m];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      // two candidates per lane and trip: their box loads are in flight together
+      for (int u0 = 0; u0 < nc; u0 += 2 * WAVE) {
+        int cj[2] = {0, 0}, mx[2] = {0, 0}, my[2] = {0, 0}, mz[2] = {0, 0};
+        float4 jc[2], jh[2];
+  #pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const int u = u0 + h * WAVE + lane;
+          jc[h] = jh[h] = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (u < nc) {
+            const unsigned pk = s_cand[wv][u];
+            const int im = (int)(pk >> 26);
+            cj[h] = (int)(pk & kEllCj);
+            mx[h] = im % 3 - 1;
+            my[h] = (im / 3) % 3 - 1;
+            mz[h] = im / 9 - 1;
+            jc[h] = bbJ[2 * cj[h]];
+            jh[h] = bbJ[2 * cj[h] + 1];
           }
-          int r = u - s_off[wv][a];
-          const int n0 = s_n[wv][0][a], n1 = s_n[wv][1][a];
-          int m = 0;
-          if (r >= n0) {
-            r -= n0;
-            m = 1;
-            if (r >= n1) {
-              r -= n1;
-              m = 2;
+        }
+  #pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const int u = u0 + h * WAVE + lane;
+          bool pass = false;
+          if (u < nc) {
+            const float Tx = mx[h] * SJ.af[0] + my[h] * SJ.af[3] + mz[h] * SJ.af[6];
+            const float Ty = mx[h] * SJ.af[1] + my[h] * SJ.af[4] + mz[h] * SJ.af[7];
+            const float Tz = mx[h] * SJ.af[2] + my[h] * SJ.af[5] + mz[h] * SJ.af[8];
+            const float gx = fmaxf(fabsf(jc[h].x + Tx - ic.x) - ih.x - jh[h].x, 0.f);
+            const float gy = fmaxf(fabsf(jc[h].y + Ty - ic.y) - ih.y - jh[h].y, 0.f);
+            const float gz = fmaxf(fabsf(jc[h].z + Tz - ic.z) - ih.z - jh[h].z, 0.f);
+            pass = gx * gx + gy * gy + gz * gz < L.rlist2;
+          }
+          const unsigned long long bal = __ballot(pass);
+          if (pass) {
+            const int k = n + __popcll(bal & ((1ull << lane) - 1));
+            if (FILL) {
+              if (base + k < L.ent_cap)
+                out[base + k] = make_entry(L.mode, ci, cj[h], mx[h], my[h], mz[h], SJ, orgJ, oi, L.pscale);
+            } else if (k < kEll) {
+              ell[(size_t)ci * kEll + k] =
+                  (unsigned)cj[h] | ((unsigned)((mx[h] + 1) + 3 * (my[h] + 1) + 9 * (mz[h] + 1)) << 26);
             }
           }
-          cj[h] = s_first[wv][m][a] + r;
-          const int im = s_img[wv][a];
-          mx[h] = (im & 3) - 1;
-          my[h] = ((im >> 2) & 3) - 1;
-          mz[h] = m - 1;
-          jc[h] = bbJ[2 * cj[h]];
-          jh[h] = bbJ[2 * cj[h] + 1];
+          n += __popcll(bal);
         }
       }
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int u = u0 + h * WAVE + lane;
-        bool pass = false;
-        if (u < total) {
-          const float Tx = mx[h] * SJ.af[0] + my[h] * SJ.af[3] + mz[h] * SJ.af[6];
-          const float Ty = mx[h] * SJ.af[1] + my[h] * SJ.af[4] + mz[h] * SJ.af[7];
-          const float Tz = mx[h] * SJ.af[2] + my[h] * SJ.af[5] + mz[h] * SJ.af[8];
-          const float gx = fmaxf(fabsf(jc[h].x + Tx - ic.x) - ih.x - jh[h].x, 0.f);
-          const float gy = fmaxf(fabsf(jc[h].y + Ty - ic.y) - ih.y - jh[h].y, 0.f);
-          const float gz = fmaxf(fabsf(jc[h].z + Tz - ic.z) - ih.z - jh[h].z, 0.f);
-          pass = gx * gx + gy * gy + gz * gz < L.rlist2;
-        }
-        const unsigned long long bal = __ballot(pass);
-        if (pass) {
-          const int k = n + __popcll(bal & ((1ull << lane) - 1));
-          if (FILL) {
-            if (base + k < L.ent_cap)
-              out[base + k] = make_entry(L.mode, ci, cj[h], mx[h], my[h], mz[h], SJ, orgJ, oi, L.pscale);
-          } else if (k < kEll) {
-            ell[(size_t)ci * kEll + k] =
-                (unsigned)cj[h] | ((unsigned)((mx[h] + 1) + 3 * (my[h] + 1) + 9 * (mz[h] + 1)) << 26);
-          }
-        }
-        n += __popcll(bal);
-      }
+      // the next chunk overwrites the candidate list read above
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
     }
-    // the next column chunk overwrites the tables read above
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
   }
   if (!FILL && lane == 0) count[ci] = n;
 }
