@@ -121,7 +121,10 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
                      (const SysArg*)b->sys.p, (const SysArg*)b->sys.p, b->bb0.p, b->bb0.p, b->kr.p, b->kr.p,
                      b->col_cl.p, b->origin.p, b->origin.p, b->ecount.p, (const int*)nullptr, (Entry*)nullptr, b->ell.p);
   launch_list_scans(s, clc, ncl_dev, b->ecount.p, chunk, b->scan_bsum.p, b->estart.p, b->istart.p, b->totals.p, 2, 3);
-  hipLaunchKernelGGL(k_list<1>, dim3(grid1(clc, wpb)), dim3(256), 0, s, L, b->cl_col.p, (const int*)b->cl_sys.p,
+  hipLaunchKernelGGL(k_list_stream, dim3(grid1(clc, wpb)), dim3(256), 0, s, L, (const int*)b->cl_sys.p,
+                     (const SysArg*)b->sys.p, b->origin.p, b->origin.p, b->ecount.p, (const int*)b->estart.p, b->ents.p,
+                     b->ell.p);
+  hipLaunchKernelGGL(k_list<2>, dim3(grid1(clc, wpb)), dim3(256), 0, s, L, b->cl_col.p, (const int*)b->cl_sys.p,
                      (const SysArg*)b->sys.p, (const SysArg*)b->sys.p, b->bb0.p, b->bb0.p, b->kr.p, b->kr.p,
                      b->col_cl.p, b->origin.p, b->origin.p, b->ecount.p, (const int*)b->estart.p, b->ents.p, b->ell.p);
   hipLaunchKernelGGL(k_items, dim3(grid1(clc, 256)), dim3(256), 0, s, clc, ncl_dev, b->ecount.p, b->estart.p,

@@ -183,6 +183,18 @@ __device__ __forceinline__ int floordiv(int a, int n) { return (a >= 0) ? a / n 
 
 __device__ __forceinline__ int nsys_of(const int* idx, int i) { return idx ? idx[i] : 0; }
 
+// Workgroups are dispatched to the 8 XCDs of the chip in turn (workgroup b to
+// XCD b mod 8), and each XCD has its own L2.  This index puts the workgroups
+// of one XCD on one contiguous run of indices, so that an XCD takes one
+// contiguous share of the spatially sorted work items and its L2 holds the
+// positions of one region instead of all of them.  A bijection on
+// [0, gridDim.x) for any grid size.
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
 // ------------------------------------------------------------------------
 // rebuild kernels
 // ------------------------------------------------------------------------
@@ -749,7 +761,8 @@ struct ListArgs {
 // Cluster-pair search: one wave per i cluster.  FILL=0 counts and keeps the
 // first kEll passing candidates of the cluster (j cluster | image code) in
 // `ell`; FILL=1 writes the entries, streaming them from `ell` when the
-// cluster's count fits there and searching again otherwise.
+// cluster's count fits there and searching again otherwise; FILL=2 does only
+// the second part, after k_list_stream has done the first.
 // Candidates are the clusters of the (column, image) window around the i
 // cluster whose range of fractional c overlaps the i cluster's range widened
 // by the list radius: a pair closer than r differs by at most r*|c*| in
@@ -774,6 +787,8 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
   const int ci = blockIdx.x * (blockDim.x / WAVE) + wv;
   const int nclI = L.nclI_dev ? *L.nclI_dev : L.nclI;
   if (ci >= nclI) return;  // uniform per wave; no workgroup barrier below
+  // FILL=2: a cluster whose candidates all fit in ell is k_list_stream's
+  if (FILL == 2 && count[ci] <= kEll) return;
   const int s = nsys_of(cl_sysI, ci);
   const SysArg& SI = sysI[s];
   const SysArg& SJ = sysJ[s];
@@ -795,7 +810,7 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
   const double4 oi = orgI[ci];
   int n = 0;
   const int base = FILL ? start[ci] : 0;
-  if (FILL) {
+  if (FILL == 1) {
     const int nc = count[ci];
     if (nc <= kEll) {
       for (int e = lane; e < nc; e += WAVE) {
@@ -907,6 +922,41 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
     }
   }
   if (!FILL && lane == 0) count[ci] = n;
+}
+
+// Fill pass of the clusters whose candidates all fit in `ell` (count <=
+// kEll): their entries in the order the count pass kept them, one wave per
+// i cluster, two entries per lane and trip.  The search kernel's registers
+// are not held here, so more waves are resident to cover the two dependent
+// loads (kept candidate, then the j cluster's origin) of each entry.  Used
+// with k_list<2> where the fill covers many clusters (the batch); a single
+// system's list is filled by k_list<1> in one launch.
+__global__ __launch_bounds__(256) void k_list_stream(ListArgs L, const int* __restrict__ cl_sysI,
+                                                     const SysArg* __restrict__ sysJ,
+                                                     const double4* __restrict__ orgI,
+                                                     const double4* __restrict__ orgJ, const int* __restrict__ count,
+                                                     const int* __restrict__ start, Entry* __restrict__ out,
+                                                     const unsigned* __restrict__ ell) {
+  const int lane = threadIdx.x & 63;
+  const int ci = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+  const int nclI = L.nclI_dev ? *L.nclI_dev : L.nclI;
+  if (ci >= nclI) return;
+  const int nc = count[ci];
+  if (nc > kEll) return;
+  const int base = start[ci];
+  const SysArg& SJ = sysJ[nsys_of(cl_sysI, ci)];
+  const double4 oi = orgI[ci];
+  const unsigned* kept = ell + (size_t)ci * kEll;
+  for (int e0 = 0; e0 < nc; e0 += 2 * WAVE) {
+    const int ea = e0 + lane, eb = e0 + WAVE + lane;
+    const unsigned pa = ea < nc ? kept[ea] : 0u, pb = eb < nc ? kept[eb] : 0u;
+    const int ja = (int)(pa & kEllCj), ia = (int)(pa >> 26);
+    const int jb = (int)(pb & kEllCj), ib = (int)(pb >> 26);
+    if (ea < nc && base + ea < L.ent_cap)
+      out[base + ea] = make_entry(L.mode, ci, ja, ia % 3 - 1, (ia / 3) % 3 - 1, ia / 9 - 1, SJ, orgJ, oi, L.pscale);
+    if (eb < nc && base + eb < L.ent_cap)
+      out[base + eb] = make_entry(L.mode, ci, jb, ib % 3 - 1, (ib / 3) % 3 - 1, ib / 9 - 1, SJ, orgJ, oi, L.pscale);
+  }
 }
 
 // work items {i cluster, first entry, end entry} of `chunk` entries each;
@@ -1231,18 +1281,6 @@ struct PairArgs {
   // launch does nothing: a pass queued on a list that may be stale
   const int* skip;
 };
-
-// Workgroups are dispatched to the 8 XCDs of the chip in turn (workgroup b to
-// XCD b mod 8), and each XCD has its own L2.  This index puts the workgroups
-// of one XCD on one contiguous run of indices, so that an XCD takes one
-// contiguous share of the spatially sorted work items and its L2 holds the
-// positions of one region instead of all of them.  A bijection on
-// [0, gridDim.x) for any grid size.
-__device__ __forceinline__ int xcd_block() {
-  const int nb = gridDim.x, b = blockIdx.x;
-  const int q = nb >> 3, r = nb & 7, x = b & 7;
-  return x * q + min(x, r) + (b >> 3);
-}
 
 // FP64 re-evaluation of the squared distance of one pair, used only when the
 // FP32 value lies within a narrow band around D_MAX^2: the inclusive cut of
