@@ -102,8 +102,9 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
                      b->col_count.p, ncol + 1, 0, b->fill.p, ncol + 1, 0, b->slot_loc.p, (int)nslot, -1);
   hipLaunchKernelGGL(k_bin, dim3(grid1(N, 256)), dim3(256), 0, s, N, pos, 0, (const SysArg*)b->sys.p,
                      (const int*)b->atom_sys.p, b->info.p, b->key.p, b->col_count.p);
-  hipLaunchKernelGGL(k_scan_cols, dim3(1), dim3(1024), 0, s, ncol, b->col_count.p, b->col_start.p, b->col_cl.p,
-                     b->totals.p);
+  // atom and cluster starts of the columns; totals[0], [1]
+  launch_list_scans(s, ncol, nullptr, b->col_count.p, CL, b->scan_bsum.p, b->col_start.p, b->col_cl.p, b->totals.p, 0,
+                    1);
   hipLaunchKernelGGL(k_scatter, dim3(grid1(N, 256)), dim3(256), 0, s, N, b->info.p, b->col_start.p, b->fill.p,
                      b->tmp.p);
   hipLaunchKernelGGL(k_rank, dim3(ncol), dim3(256), 0, s, ncol, b->key.p, b->col_start.p, b->col_cl.p, b->tmp.p,
@@ -358,7 +359,7 @@ int p2a_batch_set_boxes(p2a_batch* b, const double* boxes) {
       (rc = b->origin.reserve(clc)) || (rc = b->bb0.reserve(2 * (size_t)clc)) || (rc = b->bb.reserve(2 * (size_t)clc)) ||
       (rc = b->p.reserve((size_t)clc * CL)) || (rc = b->kr.reserve(clc)) || (rc = b->ecount.reserve(clc + 1)) ||
       (rc = b->estart.reserve(clc + 1)) || (rc = b->istart.reserve(clc + 1)) ||
-      (rc = b->scan_bsum.reserve(2 * (size_t)grid1(clc, kScanBlock) + 2)) ||
+      (rc = b->scan_bsum.reserve(2 * (size_t)grid1(std::max<int64_t>(clc, (int64_t)ncol), kScanBlock) + 2)) ||
       (rc = b->ell.reserve((size_t)clc * kEll)))
     return rc;
   P2A_HIP(hipMemcpy(b->sys.p, S.data(), S.size() * sizeof(SysArg), hipMemcpyHostToDevice));

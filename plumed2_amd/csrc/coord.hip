@@ -213,8 +213,10 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
       hipLaunchKernelGGL(k_bin, dim3(grid1(nthr, 256)), dim3(256), 0, s, G.n, pos_dev, G.off, (const SysArg*)G.sys.p,
                          (const int*)nullptr, G.info.p, G.key.p, G.col_count.p, c->totals.p + wslot,
                          c->totals.p + (25 - wslot), order, norder);
-    hipLaunchKernelGGL(k_scan_cols, dim3(1), dim3(1024), 0, s, G.ncol, G.col_count.p, G.col_start.p, G.col_cl.p,
-                       c->totals.p + 4 + 2 * gi);
+    // atom and cluster starts of the columns; totals[4 + 2 gi], [5 + 2 gi]
+    if ((rc = c->scan_bsum.reserve(2 * (size_t)grid1(std::max(G.ncol, 1), kScanBlock) + 2))) return rc;
+    launch_list_scans(s, G.ncol, nullptr, G.col_count.p, CL, c->scan_bsum.p, G.col_start.p, G.col_cl.p, c->totals.p,
+                      4 + 2 * gi, 5 + 2 * gi);
     if (G.n > 0) {
       hipLaunchKernelGGL(k_scatter, dim3(grid1(nthr, 256)), dim3(256), 0, s, G.n, G.info.p, G.col_start.p, G.fill.p,
                          G.tmp.p, order, norder);

@@ -283,43 +283,6 @@ __global__ __launch_bounds__(256) void k_bin(int n, const double* __restrict__ p
     if (hist[j]) atomicAdd(&col_count[lo + j], hist[j]);
 }
 
-// exclusive scans of column counts (atoms) and of ceil(count/8) (clusters),
-// one workgroup of 1024 threads, a contiguous run of columns per thread
-__global__ __launch_bounds__(1024) void k_scan_cols(int ncol, const int* __restrict__ cnt,
-                                                    int* __restrict__ col_start, int* __restrict__ col_cl,
-                                                    int* __restrict__ totals) {
-  __shared__ int sw[32];
-  const int t = threadIdx.x, T = blockDim.x;
-  const int per = (ncol + T - 1) / T;
-  const int b = min(ncol, t * per), e = min(ncol, b + per);
-  int la = 0, lc = 0;
-  for (int i0 = b; i0 < e; i0 += 8) {
-    int v[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) v[k] = (i0 + k < e) ? cnt[i0 + k] : 0;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      la += v[k];
-      lc += (v[k] + CL - 1) / CL;
-    }
-  }
-  int ra = la, rc = lc, ta, tc;
-  block_excl_scan2(ra, rc, ta, tc, sw);
-  for (int i = b; i < e; i++) {
-    const int v = cnt[i];
-    col_start[i] = ra;
-    col_cl[i] = rc;
-    ra += v;
-    rc += (v + CL - 1) / CL;
-  }
-  if (t == 0) {
-    col_start[ncol] = ta;
-    col_cl[ncol] = tc;
-    totals[0] = ta;
-    totals[1] = tc;
-  }
-}
-
 // Two exclusive scans of one count array in one pass over it (out0 of in[i],
 // out1 of ceil(in[i]/chunk)), across many workgroups: per-block sums, a
 // scan of the block sums, then per-block local scans with the block offsets.
