@@ -109,7 +109,7 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
                      b->tmp.p);
   hipLaunchKernelGGL(k_rank, dim3(ncol), dim3(256), 0, s, ncol, b->key.p, b->col_start.p, b->col_cl.p, b->tmp.p,
                      b->slot_loc.p, b->info.p, pos, 0, (const SysArg*)b->sys.p, (const int*)b->atom_sys.p,
-                     b->origin.p, b->bb0.p, b->kr.p, b->cl_sys.p, b->cl_col.p);
+                     b->origin.p, b->bb0.p, b->kr.p, b->cl_sys.p, b->cl_col.p, b->p.p, b->bb.p, pack_scale(b->sw));
   ListArgs L;
   L.nclI = clc;
   L.nclI_dev = ncl_dev;
@@ -130,9 +130,7 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
                      b->col_cl.p, b->origin.p, b->origin.p, b->ecount.p, (const int*)b->estart.p, b->ents.p, b->ell.p);
   hipLaunchKernelGGL(k_items, dim3(grid1(clc, 256)), dim3(256), 0, s, clc, ncl_dev, b->ecount.p, b->estart.p,
                      b->istart.p, b->items.p, chunk, b->items_cap);
-  hipLaunchKernelGGL(k_pack_bbox, dim3(grid1(nslot, 256)), dim3(256), 0, s, (int)nslot, ncl_dev, b->slot_loc.p,
-                     b->info.p, pos, 0, (const SysArg*)b->sys.p, (const int*)b->atom_sys.p, b->origin.p, b->p.p,
-                     b->bb.p, pack_scale(b->sw));
+  // k_rank has packed the positions of this step (the list is built at every step)
   int rc;
   if ((rc = launch_check("batch: list build"))) return rc;
   PairArgs A;

@@ -222,7 +222,8 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
                          G.tmp.p, order, norder);
       hipLaunchKernelGGL(k_rank, dim3(G.ncol), dim3(256), 0, s, G.ncol, G.key.p, G.col_start.p, G.col_cl.p,
                          G.tmp.p, G.slot_loc.p, G.info.p, pos_dev, G.off, (const SysArg*)G.sys.p,
-                         (const int*)nullptr, G.origin.p, G.bb0.p, G.kr.p, (int*)nullptr, G.cl_col.p);
+                         (const int*)nullptr, G.origin.p, G.bb0.p, G.kr.p, (int*)nullptr, G.cl_col.p, G.p.p, G.bb.p,
+                         pack_scale(c->spec.sw));
     }
     if ((rc = launch_check("rebuild: binning"))) return rc;
   }
@@ -314,6 +315,9 @@ static int rebuild(p2a_coord* c, const double* box9, const double* pos_dev) {
   std::memcpy(c->box_at_rebuild, box9, sizeof(c->box_at_rebuild));
   c->built = true;
   c->rebuilds++;
+  // k_rank has packed these positions for the new list
+  c->packed_pos = pos_dev;
+  c->packed_gen = c->rebuilds;
   return P2A_OK;
 }
 

@@ -517,7 +517,11 @@ __global__ __launch_bounds__(256) void k_scatter(int n, const int4* __restrict__
 //     fractional c (widened outwards so that the FP32 range contains the
 //     FP64 keys) and system.  Padding slots come last in a cluster, so its
 //     first slot always holds an atom.
-constexpr int kRankLds = 2048;
+// With pk, also what k_pack_bbox would write for these positions.
+// Columns hold ~8 atoms per column width of height (a few hundred in cells up
+// to ~30 column widths tall); kRankLds bounds the LDS of a workgroup to
+// ~21 KB, so that 7 columns share a CU.
+constexpr int kRankLds = 1024;
 constexpr int kRankBuckets = 256;  // the workgroup size: one scan
 __device__ __forceinline__ int rank_bucket(double k) {
   return max(0, min((int)(k * kRankBuckets), kRankBuckets - 1));
@@ -529,7 +533,8 @@ __global__ __launch_bounds__(256) void k_rank(int ncol, const double* __restrict
                                               const SysArg* __restrict__ sys, const int* __restrict__ atom_sys,
                                               double4* __restrict__ origin, float4* __restrict__ bb0,
                                               float2* __restrict__ kr, int* __restrict__ cl_sys,
-                                              int* __restrict__ cl_col) {
+                                              int* __restrict__ cl_col, float4* __restrict__ pk,
+                                              float4* __restrict__ pbb, double pscale) {
   __shared__ double sk[kRankLds];
   __shared__ int si[kRankLds];
   __shared__ int ss[kRankLds];
@@ -600,12 +605,15 @@ __global__ __launch_bounds__(256) void k_rank(int ncol, const double* __restrict
     }
   }
   __syncthreads();
-  // (2) the column's clusters; the loop bound is uniform over the workgroup
+  // (2) the column's clusters; the loop bound is uniform over the workgroup.
+  // With pk, also what k_pack_bbox would write for these positions: the
+  // packed slots (pk) and the boxes of the packed clusters (pbb).
   for (int t0 = 0; t0 < ncl * CL; t0 += blockDim.x) {
     const int t = t0 + (int)threadIdx.x;
     int a = -1;
     if (t < n) a = lds ? ss[t] : slot_loc[base + t];
     double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    double w[3] = {0.0, 0.0, 0.0};
     double klo = DBL_MAX, khi = -DBL_MAX;
     int sy = 0;
     if (a >= 0) {
@@ -614,9 +622,9 @@ __global__ __launch_bounds__(256) void k_rank(int ncol, const double* __restrict
       const int4 m = info[a];
       const double* x = pos + 3 * (size_t)(off + a);
       for (int d = 0; d < 3; d++) {
-        const double w = x[d] - (m.x * S.a[0 * 3 + d] + m.y * S.a[1 * 3 + d] + m.z * S.a[2 * 3 + d]);
-        lo[d] = w;
-        hi[d] = w;
+        w[d] = x[d] - (m.x * S.a[0 * 3 + d] + m.y * S.a[1 * 3 + d] + m.z * S.a[2 * 3 + d]);
+        lo[d] = w[d];
+        hi[d] = w[d];
       }
       klo = khi = key[a];
     }
@@ -629,15 +637,43 @@ __global__ __launch_bounds__(256) void k_rank(int ncol, const double* __restrict
       klo = fmin(klo, __shfl_xor(klo, o, 64));
       khi = fmax(khi, __shfl_xor(khi, o, 64));
     }
+    const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
+    const int c = c0 + t / CL;
     if ((t & (CL - 1)) == 0 && t < ncl * CL) {
-      const int c = c0 + t / CL;
       if (cl_sys) cl_sys[c] = sy;
-      const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
       origin[c] = make_double4(cx, cy, cz, 0.0);
       bb0[2 * c] = make_float4((float)cx, (float)cy, (float)cz, 0.f);
       bb0[2 * c + 1] = make_float4((float)(0.5 * (hi[0] - lo[0])), (float)(0.5 * (hi[1] - lo[1])),
                                    (float)(0.5 * (hi[2] - lo[2])), 0.f);
       kr[c] = make_float2((float)klo - 1.0e-6f, (float)khi + 1.0e-6f);
+    }
+    if (pk) {
+      const int sl = base + t;
+      float4 q;
+      float ql[3], qh[3];
+      if (a < 0) {
+        q = make_float4(1.0e5f + 1.0e3f * (float)(sl & 1023), 1.0e5f + 1.0e3f * (float)((sl >> 10) & 1023),
+                        1.0e5f + 1.0e3f * (float)(sl >> 20), __int_as_float(-1));
+        ql[0] = ql[1] = ql[2] = FLT_MAX;
+        qh[0] = qh[1] = qh[2] = -FLT_MAX;
+      } else {
+        q = make_float4((float)((w[0] - cx) * pscale), (float)((w[1] - cy) * pscale), (float)((w[2] - cz) * pscale),
+                        __int_as_float(off + a));
+        ql[0] = qh[0] = q.x;
+        ql[1] = qh[1] = q.y;
+        ql[2] = qh[2] = q.z;
+      }
+      if (t < ncl * CL) pk[sl] = q;
+      for (int o = 1; o < CL; o <<= 1) {
+        for (int d = 0; d < 3; d++) {
+          ql[d] = fminf(ql[d], __shfl_xor(ql[d], o, 64));
+          qh[d] = fmaxf(qh[d], __shfl_xor(qh[d], o, 64));
+        }
+      }
+      if ((t & (CL - 1)) == 0 && t < ncl * CL) {
+        pbb[2 * c] = make_float4(0.5f * (ql[0] + qh[0]), 0.5f * (ql[1] + qh[1]), 0.5f * (ql[2] + qh[2]), 0.f);
+        pbb[2 * c + 1] = make_float4(0.5f * (qh[0] - ql[0]), 0.5f * (qh[1] - ql[1]), 0.5f * (qh[2] - ql[2]), 0.f);
+      }
     }
   }
 }

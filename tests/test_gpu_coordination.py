@@ -105,6 +105,22 @@ def test_triclinic(tilt):
     assert act.stats()["path"] == 0 and act.stats()["contributing_pairs"] == npair
 
 
+def test_tall_cell_columns_ranked_in_global_memory():
+    """A 3 x 3 x 60 nm cell: 16 columns of ~1,130 atoms, more than a column
+    ranked in LDS holds (1,024), so the rebuild ranks them in global memory;
+    CV, forces, virial and pair count match the oracle."""
+    P = _P()
+    box = np.diag([3.0, 3.0, 60.0])
+    n = int(33.4 * 3.0 * 3.0 * 60.0)
+    x = np.random.default_rng(12).uniform(0.0, 1.0, (n, 3)) * np.array([3.0, 3.0, 60.0])
+    defn = "RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2"
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{n} SWITCH={{{defn}}}")
+    v, d, w = act.calculate(x, box)
+    ov, od, ow, npair = O.coordination(O.Switch(defn), box, x, n, 0)
+    check(v, d, w, ov, od, ow)
+    assert act.stats()["path"] == 0 and act.stats()["contributing_pairs"] == npair
+
+
 @pytest.mark.parametrize("defn", ["EXP R_0=0.2 D_MAX=1.0", "GAUSSIAN R_0=0.3 D_0=0.1 D_MAX=1.0",
                                   "RATIONAL R_0=0.3 NN=5 MM=11 D_MAX=1.0", "RATIONAL R_0=0.35 NN=8 MM=16 D_MAX=1.0",
                                   "TANH R_0=0.3 D_MAX=1.0", "COSINUS R_0=1.0", "SMAP R_0=0.3 A=3 B=2 D_MAX=1.0",
