@@ -692,6 +692,7 @@ __global__ void k_abs_slot(int nslot, const int* __restrict__ ncl_dev, const int
 // candidates kept per i cluster between the count and fill passes of the
 // list search (j cluster in bits 0-25, image code (mx+1)+3(my+1)+9(mz+1) above)
 constexpr int kEll = 384;
+static_assert(kEll >= 2 * WAVE, "k_list_stream reads two candidates per lane ahead");
 constexpr unsigned kEllCj = 0x3FFFFFFu;
 
 // list entry of i cluster ci for j cluster cj seen through image (mx,my,mz)
@@ -940,15 +941,20 @@ __global__ __launch_bounds__(256) void k_list_stream(ListArgs L, const int* __re
   const int ci = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
   const int nclI = L.nclI_dev ? *L.nclI_dev : L.nclI;
   if (ci >= nclI) return;
-  const int nc = count[ci];
-  if (nc > kEll) return;
-  const int base = start[ci];
-  const SysArg& SJ = sysJ[nsys_of(cl_sysI, ci)];
-  const double4 oi = orgI[ci];
+  // everything that depends on ci only, and the first two kept candidates of
+  // each lane (ell has kEll >= 2 * WAVE slots per cluster, written or not),
+  // is loaded before the count decides anything
   const unsigned* kept = ell + (size_t)ci * kEll;
+  const unsigned p0 = kept[lane], p1 = kept[WAVE + lane];
+  const int nc = count[ci];
+  const int base = start[ci];
+  const int sy = nsys_of(cl_sysI, ci);
+  const double4 oi = orgI[ci];
+  if (nc > kEll) return;
+  const SysArg& SJ = sysJ[sy];
   for (int e0 = 0; e0 < nc; e0 += 2 * WAVE) {
     const int ea = e0 + lane, eb = e0 + WAVE + lane;
-    const unsigned pa = ea < nc ? kept[ea] : 0u, pb = eb < nc ? kept[eb] : 0u;
+    const unsigned pa = ea < nc ? (e0 == 0 ? p0 : kept[ea]) : 0u, pb = eb < nc ? (e0 == 0 ? p1 : kept[eb]) : 0u;
     const int ja = (int)(pa & kEllCj), ia = (int)(pa >> 26);
     const int jb = (int)(pb & kEllCj), ib = (int)(pb >> 26);
     if (ea < nc && base + ea < L.ent_cap)
