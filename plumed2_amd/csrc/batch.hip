@@ -125,7 +125,8 @@ static int batch_step(p2a_batch* b, const double* pos, double* values, double* d
   hipLaunchKernelGGL(k_list_stream, dim3(grid1(clc, wpb)), dim3(256), 0, s, L, (const int*)b->cl_sys.p,
                      (const SysArg*)b->sys.p, b->origin.p, b->origin.p, b->ecount.p, (const int*)b->estart.p, b->ents.p,
                      b->ell.p);
-  hipLaunchKernelGGL(k_list<2>, dim3(grid1(clc, wpb)), dim3(256), 0, s, L, b->cl_col.p, (const int*)b->cl_sys.p,
+  // k_list<2>: 64 clusters per wave
+  hipLaunchKernelGGL(k_list<2>, dim3(grid1(clc, wpb * WAVE)), dim3(256), 0, s, L, b->cl_col.p, (const int*)b->cl_sys.p,
                      (const SysArg*)b->sys.p, (const SysArg*)b->sys.p, b->bb0.p, b->bb0.p, b->kr.p, b->kr.p,
                      b->col_cl.p, b->origin.p, b->origin.p, b->ecount.p, (const int*)b->estart.p, b->ents.p, b->ell.p);
   hipLaunchKernelGGL(k_items, dim3(grid1(clc, 256)), dim3(256), 0, s, clc, ncl_dev, b->ecount.p, b->estart.p,

@@ -784,11 +784,25 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
   __shared__ unsigned s_cand[4][kCandCap];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x / WAVE;
-  const int ci = blockIdx.x * (blockDim.x / WAVE) + wv;
   const int nclI = L.nclI_dev ? *L.nclI_dev : L.nclI;
-  if (ci >= nclI) return;  // uniform per wave; no workgroup barrier below
-  // FILL=2: a cluster whose candidates all fit in ell is k_list_stream's
-  if (FILL == 2 && count[ci] <= kEll) return;
+  // FILL=0/1: cluster ci = the wave.  FILL=2: the wave's 64 clusters, of which
+  // it searches those whose candidates did not all fit in ell (the others are
+  // k_list_stream's), in order.  No workgroup barrier below.
+  int ci = blockIdx.x * (blockDim.x / WAVE) + wv;
+  unsigned long long pending = 0ull;
+  if (FILL == 2) {
+    const int c0 = ci * WAVE;
+    pending = __ballot(c0 + lane < nclI && count[c0 + lane] > kEll);
+    ci = c0;
+  } else if (ci >= nclI) {
+    return;
+  }
+  const int cfirst = ci;
+  while (FILL != 2 || pending) {
+  if (FILL == 2) {
+    ci = cfirst + __builtin_ctzll(pending);
+    pending &= pending - 1ull;
+  }
   const int s = nsys_of(cl_sysI, ci);
   const SysArg& SI = sysI[s];
   const SysArg& SJ = sysJ[s];
@@ -922,6 +936,8 @@ __global__ __launch_bounds__(256) void k_list(ListArgs L, const int* __restrict_
     }
   }
   if (!FILL && lane == 0) count[ci] = n;
+  if (FILL != 2) break;
+  }
 }
 
 // Fill pass of the clusters whose candidates all fit in `ell` (count <=

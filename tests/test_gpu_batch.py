@@ -223,3 +223,33 @@ def test_batch_with_generic_switch():
         assert abs(v[k] - ov) <= CV_TOL * abs(ov)
         assert np.max(np.abs(d[o[k]:o[k + 1]] - od)) <= F_TOL * np.max(np.abs(od))
     b.close()
+
+
+def test_long_list_radius_searches_overflowing_clusters():
+    """A 2.8 nm list radius with D_MAX 1.2: ~800 candidate j clusters per i
+    cluster, more than the count pass keeps (384), so the fill pass searches
+    again for almost every cluster; per system the CV, derivatives, virial
+    and exact pair counts match the oracle."""
+    import torch
+
+    import plumed2_amd as P
+    rng = np.random.default_rng(21)
+    natoms = np.array([10000, 12000, 11000])
+    boxes, pos = [], []
+    for n in natoms:
+        t = rng.uniform(-0.05, 0.05, 3)
+        box = np.array([[1.0, 0.0, 0.0], [t[0], 1.0, 0.0], [t[1], t[2], 1.0]])
+        box *= (n / 33.4 / abs(np.linalg.det(box))) ** (1.0 / 3.0)
+        boxes.append(box)
+        pos.append(rng.random((n, 3)) @ box)
+    boxes = np.array(boxes)
+    x = np.concatenate(pos)
+    b = P.CoordinationBatch(natoms, SWITCH, 2.8)
+    b.set_boxes(boxes)
+    xt = torch.tensor(x, dtype=torch.float64, device="cuda")
+    v, d, w = (t.cpu().numpy() for t in b.calculate_torch(xt))
+    counts = b.pair_counts()
+    for k in range(len(natoms)):
+        check_system(k, natoms, boxes, x, v, d, w, counts)
+    b.close()
+

@@ -105,6 +105,23 @@ def test_triclinic(tilt):
     assert act.stats()["path"] == 0 and act.stats()["contributing_pairs"] == npair
 
 
+def test_long_list_radius_searches_overflowing_clusters():
+    """NLISTCELLS NL_CUTOFF=2.8 with D_MAX 1.2 on 12,000 atoms: ~800
+    candidate j clusters per i cluster, more than the count pass keeps (384),
+    so the fill pass searches again; CV, forces, virial and pair count match
+    the oracle."""
+    P = _P()
+    n = 12000
+    x, L = uniform(n, seed=13)
+    box = np.diag([L] * 3)
+    defn = "RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2"
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{n} SWITCH={{{defn}}} NLIST NL_CUTOFF=2.8 NL_STRIDE=10")
+    v, d, w = act.calculate(x, box, step=0)
+    ov, od, ow, npair = O.coordination(O.Switch(defn), box, x, n, 0)
+    check(v, d, w, ov, od, ow)
+    assert act.stats()["path"] == 0 and act.stats()["contributing_pairs"] == npair
+
+
 def test_tall_cell_columns_ranked_in_global_memory():
     """A 3 x 3 x 60 nm cell: 16 columns of ~1,130 atoms, more than a column
     ranked in LDS holds (1,024), so the rebuild ranks them in global memory;
