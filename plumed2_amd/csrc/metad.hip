@@ -60,6 +60,7 @@ struct GridArg {
   double period[kMaxCv], inv_period[kMaxCv];
   double invsigma[kMaxCv];
   double height0, biasf, kbt, stretchA, stretchB;
+  double inv_wt;          // 1 / (kbt (biasf - 1)) for biasf > 1, else 1 / kbt
   int welltemp;
   int support;            // prod(2*nneigh+1)
   int may_alias;          // a periodic support wider than the grid visits a node twice
@@ -231,10 +232,13 @@ __device__ void add_hill_block(const GridArg& G, double* __restrict__ val, doubl
   }
 }
 
+// MetaD::getHeight (MetaD.cpp:2280-2305): height0 exp(-V / (kbt (biasf - 1)));
+// the division is a product with its reciprocal, formed once on the host, so
+// that the sequential chain of heights carries no division (the heights move
+// by about one rounding of the exponent's argument)
 __device__ __forceinline__ double wt_height(const GridArg& G, double vbias) {
   if (!G.welltemp) return G.height0;
-  if (G.biasf > 1.0) return G.height0 * exp(-vbias / (G.kbt * (G.biasf - 1.0)));
-  return G.height0 * exp(-vbias / G.kbt);
+  return G.height0 * exp(-vbias * G.inv_wt);
 }
 
 // spline value (no gradient) at x from the corner ip of its cell: the term
@@ -635,7 +639,13 @@ __global__ __launch_bounds__(kSeqThreads) void k_deposit_seq_tile2(GridArg G, do
       X3[j] = X2[j] * X[j];
       sg[j] = x0 ? -1.0 : 1.0;
     }
+    // this lane's corner coordinates (the grid node of corner q of hill r)
+    const double xn0 = G.min[0] + (double)ni[0] * G.dx[0], xn1 = G.min[1] + (double)ni[1] * G.dx[1];
     for (int k = 0; k < nb; k++) {
+      // hill k's shape at this lane's corner does not depend on hill k's
+      // height: read and scaled before the height is found
+      const double ek = sh_e[k][lane];
+      const double dpk[2] = {gscaled(G, 0, sh_c[b][k][0], xn0), gscaled(G, 1, sh_c[b][k][1], xn1)};
       // corner term of this lane (used by the lanes of hill k only): the
       // arithmetic of spline_corner_vals
       double v = 0.0;
@@ -660,12 +670,9 @@ __global__ __launch_bounds__(kSeqThreads) void k_deposit_seq_tile2(GridArg G, do
         if (heights) heights[kfirst + k] = h;
       }
       if (r > k && mine) {
-        const double e = sh_e[k][lane];
-        if (e >= 0.0) {
-          const double dp[2] = {gscaled(G, 0, sh_c[b][k][0], G.min[0] + (double)ni[0] * G.dx[0]),
-                                gscaled(G, 1, sh_c[b][k][1], G.min[1] + (double)ni[1] * G.dx[1])};
+        if (ek >= 0.0) {
           double d[2];
-          const double bb = gauss_apply<2>(G, dp, e, h, d);
+          const double bb = gauss_apply<2>(G, dpk, ek, h, d);
           acc0 += bb;
           acc1 += d[0];
           acc2 += d[1];
@@ -1015,6 +1022,7 @@ int p2a_metad_create(p2a_metad** out, int device, const p2a_metad_spec* spec) {
   G.biasf = spec->biasf;
   G.kbt = spec->kbt;
   G.welltemp = spec->biasf >= 1.0;
+  G.inv_wt = spec->biasf > 1.0 ? 1.0 / (spec->kbt * (spec->biasf - 1.0)) : (spec->kbt != 0.0 ? 1.0 / spec->kbt : 0.0);
   // dp2cutoff stretch (Tools.h:71-79, MetaD.cpp:762-765)
   G.stretchA = spec->nostretch ? 1.0 : 1.00193418799744762399;
   G.stretchB = spec->nostretch ? 0.0 : -.00193418799744762399;
