@@ -1773,10 +1773,14 @@ __global__ __launch_bounds__(PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
     b1 = __builtin_amdgcn_readfirstlane(b1);
     b2 = __builtin_amdgcn_readfirstlane(b2);
     int nrec = 0, npair = 0;
+    // the pair loops issue ahead of the other waves' staging and item ends
+    // (measured: the pass takes ~3% less time than with one priority)
+    __builtin_amdgcn_s_setprio(1);
     if (n0) pair_term<P, GENERIC, TWO, VO, 0>(A, L.sj[jj], pi, ii, jj, 0, ai, L, nrec, npair, acc);
     pair_groups<P, GENERIC, TWO, VO, 1>(A, b1, b1 + n1, pi, ii, jj, ai, L, nrec, npair, acc);
     if constexpr (!VO)
       pair_groups<P, GENERIC, TWO, VO, 2>(A, b2, b2 + n2, pi, ii, jj, ai, L, nrec, npair, acc);
+    __builtin_amdgcn_s_setprio(0);
     if (j < 64) {
       if (nrec > 0) bandbits |= 1ull << j;
     } else if (lane == 0) {
