@@ -823,6 +823,9 @@ static int calc_device(p2a_coord* c, int64_t step, const double* box9, const dou
                        double* deriv_dev, double* virial_dev) {
   int rc;
   if (!box9) return fail(P2A_ERR_ARG, "box is required");
+  // positions packed ahead count only within the call that packed them (the
+  // same address may hold other positions at a later call)
+  c->packed_pos = nullptr;
   if (!c->stats.p) {
     if ((rc = c->stats.reserve(8))) return rc;
     hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(64), 0, c->stream, c->stats.p, (size_t)8);
@@ -1068,6 +1071,7 @@ int p2a_coord_backward_device(p2a_coord* c, const double* box9, const double* po
                               double* virial_dev) {
   if (!c || !pos_dev || !box9 || !deriv_dev) return fail(P2A_ERR_ARG, "null argument");
   P2A_HIP(hipSetDevice(c->device));
+  c->packed_pos = nullptr;  // packed ahead by an earlier call: pack again
   int rc;
   if ((rc = c->out_buf.reserve(16))) return rc;
   double* wdev = virial_dev ? virial_dev : c->out_buf.p + 1;
@@ -1229,6 +1233,7 @@ int p2a_coord_rows_device(p2a_coord* c, int64_t step, const double* box9, const 
   if (weights_dev && !vjp_dev) return fail(P2A_ERR_ARG, "weights need an output for their derivatives");
   if (c->spec.pair) return fail(P2A_ERR_UNSUPPORTED, "PAIR gives no per-atom rows");
   P2A_HIP(hipSetDevice(c->device));
+  c->packed_pos = nullptr;  // packed ahead by an earlier call: pack again
   int rc;
   if (!c->stats.p) {
     if ((rc = c->stats.reserve(8))) return rc;

@@ -474,6 +474,31 @@ def test_skin_keeps_the_list_and_matches_rebuilding(groups):
     b.close()
 
 
+def test_rebuild_then_new_positions_in_the_same_buffer():
+    """A rebuild on device positions packs those positions for the pair pass;
+    a later call on the same buffer holding other positions (step 1 of an
+    NLIST stride keeps the list) must pack its own: CV, forces and virial
+    match the oracle at the new positions."""
+    import torch
+    P = _P()
+    n = 6000
+    x, L = uniform(n, seed=14)
+    box = np.diag([L] * 3)
+    defn = "RATIONAL R_0=0.3 NN=6 MM=12 D_MAX=1.2"
+    act = P.Coordination(f"c: COORDINATION GROUPA=1-{n} SWITCH={{{defn}}} NLIST NL_CUTOFF=1.5 NL_STRIDE=10")
+    xt = torch.tensor(x, dtype=torch.float64, device="cuda")
+    act.calculate_torch(xt, box, step=0)
+    act.rebuild_device(xt.data_ptr(), box)
+    # displacements of ~0.05 nm at most: the list built at x (1.5 nm) holds
+    # every pair within D_MAX at x2
+    x2 = x + np.random.default_rng(15).normal(0.0, 0.01, x.shape)
+    xt.copy_(torch.tensor(x2, dtype=torch.float64))
+    v, d, w = (t.cpu().numpy() for t in act.calculate_torch(xt, box, step=1))
+    ov, od, ow, npair = O.coordination(O.Switch(defn), box, x2, n, 0)
+    check(float(v[0]), d, w, ov, od, ow)
+    act.close()
+
+
 def test_skin_in_a_cell_too_narrow_for_it():
     """A cell wide enough for D_MAX but not for D_MAX + skin rebuilds at every
     call and stays correct; a negative skin is refused."""
