@@ -1594,17 +1594,33 @@ __device__ __forceinline__ double item_row(const PairArgs& A, const PairAcc& acc
 //     (self / counting / derivative-only), each kind padded to a multiple
 //     of 8 with far-away atoms, so that every group of 8 has one kind.
 // Returns the group bounds (uniform): n0 (0 or 8), [b1, b1+n1), [b2, b2+n2).
+constexpr int kFewEntries = 16;
 template <bool TWO, bool VO, int CH>
 __device__ __forceinline__ void stage_item(const PairArgs& A, const Entry& my_e, int lane, int e0, int e1,
                                            const float4& ic, const float4& ih, const float4* __restrict__ posJ,
                                            const float4* __restrict__ bbJ, const int* __restrict__ absJ,
                                            const WaveLds& L, int& n0, int& b1, int& n1, int& b2, int& n2) {
+  // Two groups: an item of at most kFewEntries entries (the dense group's
+  // side of a sparse group) stages each entry with four lanes of two atoms
+  // each instead of one lane of eight; lanes are ordered by (entry, atom)
+  // either way, so the staged order is the same.  Single groups keep the
+  // one-lane form (their items are mostly full, and the branch costs them).
+  const bool few = TWO && e1 - e0 <= kFewEntries;  // uniform over the wave
+  const int ent = few ? (lane >> 2) : lane;  // this lane's entry of the item
+  const int sub2 = few ? 2 * (lane & 3) : 0;  // its first atom (few)
   Entry my = {0, 0.f, 0.f, 0.f};
   bool pass = false;
   {
-    const int e = e0 + lane;
-    if (lane < CH && e < e1) {
-      my = my_e;
+    Entry src = my_e;
+    if (few) {
+      src.code = __shfl(my_e.code, ent);
+      src.dx = __shfl(my_e.dx, ent);
+      src.dy = __shfl(my_e.dy, ent);
+      src.dz = __shfl(my_e.dz, ent);
+    }
+    const int e = e0 + ent;
+    if (ent < CH && e < e1) {
+      my = src;
       if (!VO || (my.code & kCountBit) || my.code < 0) {
         const int cj = my.code & kCjMask;
         const float4 jc = bbJ[2 * cj], jh = bbJ[2 * cj + 1];
@@ -1617,7 +1633,27 @@ __device__ __forceinline__ void stage_item(const PairArgs& A, const Entry& my_e,
   }
   float4 q[CL];
   int m8 = 0, kind = 3;
-  if (pass) {
+  if (pass && few) {
+    const int cj = my.code & kCjMask;
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const float4 v = posJ[cj * CL + sub2 + u];
+      q[u] = make_float4(v.x + my.dx, v.y + my.dy, v.z + my.dz, 0.f);
+    }
+    if (my.code < 0) {
+      kind = 0;
+      m8 = 3 << sub2;
+    } else {
+      kind = (my.code & kCountBit) ? 1 : 2;
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const float gx = fmaxf(fabsf(q[u].x - ic.x) - ih.x, 0.f);
+        const float gy = fmaxf(fabsf(q[u].y - ic.y) - ih.y, 0.f);
+        const float gz = fmaxf(fabsf(q[u].z - ic.z) - ih.z, 0.f);
+        if (gx * gx + gy * gy + gz * gz <= A.prune2) m8 |= 1 << (sub2 + u);
+      }
+    }
+  } else if (pass) {
     const int cj = my.code & kCjMask;
 #pragma unroll
     for (int t = 0; t < CL; t++) {
@@ -1649,16 +1685,29 @@ __device__ __forceinline__ void stage_item(const PairArgs& A, const Entry& my_e,
   b1 = n0;
   b2 = n0 + n1;
   if (pass) {
-    L.sc[lane] = my.code;
+    if (sub2 == 0) L.sc[ent] = my.code;
     const int cj = my.code & kCjMask;
-    int p = kind == 0 ? 0 : (kind == 1 ? b1 + o1 : b2 + o2);
+    int p = kind == 0 ? sub2 : (kind == 1 ? b1 + o1 : b2 + o2);
+    if (few) {
 #pragma unroll
-    for (int t = 0; t < CL; t++) {
-      if ((m8 >> t) & 1) {
-        L.sj[p] = P3{q[t].x, q[t].y, q[t].z};
-        L.sm[p] = (short)((lane << 3) | t);
-        if constexpr (TWO) L.sa[p] = absJ[cj * CL + t];
-        p++;
+      for (int u = 0; u < 2; u++) {
+        const int t = sub2 + u;
+        if ((m8 >> t) & 1) {
+          L.sj[p] = P3{q[u].x, q[u].y, q[u].z};
+          L.sm[p] = (short)((ent << 3) | t);
+          if constexpr (TWO) L.sa[p] = absJ[cj * CL + t];
+          p++;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < CL; t++) {
+        if ((m8 >> t) & 1) {
+          L.sj[p] = P3{q[t].x, q[t].y, q[t].z};
+          L.sm[p] = (short)((lane << 3) | t);
+          if constexpr (TWO) L.sa[p] = absJ[cj * CL + t];
+          p++;
+        }
       }
     }
   }
